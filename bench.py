@@ -1,0 +1,143 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-element pushes/s including SCHEFF space charge.
+
+One step = one simulate() of the whole lattice for the 1,048,576-particle
+DTL-like workload (200 Drift/Quad/Drift/DTL-gap periods, gap voltage ramped
+0.3 -> 0.6 MV, quad gradients scaled with beta*gamma, SCHEFF 32x128 after
+every element), started from the saved initial beam.  value = particles x
+elements x steps / wall time of the timed steps.
+
+The JSON line also carries
+  roofline:      HBM roofline of the fused element-map kernel (k_segment):
+                 algorithmic bytes per launch / average launch time, the
+                 average measured with HIP events around every launch in the
+                 timed steps;
+  cpu_baseline:  the serial FP64 oracle (oracle/liboracle.so, one core) on a
+                 bounded sample of the same workload: the full beam through
+                 the first lattice period(s), timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
+SEG_BYTES_PER_PARTICLE = 104   # k_segment: reads 6 FP64 + 2 u32, writes 6 FP64
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--num", type=int, default=None, help="override particle count")
+    p.add_argument("--periods", type=int, default=None, help="override lattice periods")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-periods", type=int, default=1,
+                   help="lattice periods of the CPU-baseline sample")
+    return p.parse_args()
+
+
+def cpu_baseline(wl, periods):
+    """Serial oracle on the first `periods` periods of the same workload."""
+    import copy
+
+    from hpsim_amd import configs as cfg
+    from oracle import oracle as O
+
+    w2 = copy.deepcopy(wl)
+    elems = cfg.build_lattice(w2.lattice)[: 4 * periods]
+    b = w2.beam
+    dist = cfg.make_distribution(b)
+    beam = O.Beam(b.num, b.mass, b.charge, b.current)
+    beam.set_distribution(*dist)
+    beam.set_frequency(b.freq)
+    beam.set_ref_w(b.w0)
+    sc = O.SpaceCharge(w2.nr, w2.nz, w2.interval)
+    sim = O.Simulator(beam, elems, sc)
+    t0 = time.perf_counter()
+    sim.simulate()
+    dt = time.perf_counter() - t0
+    pushes = b.num * len(elems)
+    return {"value": pushes / dt, "unit": "particle-element pushes/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{b.num} particles x first {len(elems)} elements "
+                      f"({periods} period(s)) of the same lattice, SCHEFF on, {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent beams "
+              "shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    from hpsim_amd import HPSim
+    from hpsim_amd import configs as cfg
+
+    wl = cfg.config(1, num=args.num, periods=args.periods)
+    b = wl.beam
+    elems = cfg.build_lattice(wl.lattice)
+    dist = cfg.make_distribution(b)
+    beam = HPSim.Beam(mass=b.mass, charge=b.charge, current=b.current, num=b.num)
+    beam.set_distribution(*dist)
+    beam.set_frequency(b.freq)
+    beam.set_ref_w(b.w0)
+    beam.save_initial_beam()
+    bl = HPSim.BeamLine(elements=elems)
+    sc = HPSim.SpaceCharge(nr=wl.nr, nz=wl.nz, interval=wl.interval)
+    sim = HPSim.Simulator(beam, bl, sc)
+    nel = len(elems)
+
+    def step():
+        beam.restore_initial_beam()
+        sim.simulate()
+
+    for _ in range(args.warmup):
+        step()
+    sim.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dt = time.perf_counter() - t0
+    prof = sim.profile()
+
+    pushes = b.num * nel * args.steps
+    seg_ms, seg_n = prof["segment"]
+    avg_s = seg_ms / 1e3 / max(seg_n, 1)
+    achieved = SEG_BYTES_PER_PARTICLE * b.num / avg_s / 1e9
+    lost = beam.get_loss_num("t")
+    out = {
+        "metric": "particle-element pushes/s incl. space charge",
+        "value": pushes / dt,
+        "unit": "particle-element pushes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox-seeded truncated Gaussian beam, generated lattice)",
+        "config": {"workload": wl.name, "particles": b.num, "elements": nel,
+                   "mesh": f"{wl.nr}x{wl.nz}", "sc_kicks_per_step": prof["phase"][1] // args.steps,
+                   "transmitted": b.num - lost},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_segment", "avg_launch_us": avg_s * 1e6,
+                     "bytes_per_launch": SEG_BYTES_PER_PARTICLE * b.num},
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_periods)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,411 @@
+// hps_device.hpp -- per-particle maps of the tracking engine, register form.
+//
+// Each map restates one reference CUDA kernel body for a single particle held
+// in registers, so that any sequence of maps between two space-charge kicks
+// runs in one pass over HBM.  Expression order follows the reference source
+// line by line (the library is built with -ffp-contract=off), which is what
+// makes the results bit-identical to a serial host restatement.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hpsim_amd.h"
+#include "hps_math.hpp"
+
+namespace hps {
+
+constexpr double CLIGHT = 299.792458;                 // constant.h:7
+constexpr double PI = 3.14159265358979323846;        // constant.h:8
+constexpr double TWOPI = 3.14159265358979323846 * 2.0;
+
+constexpr uint32_t kLanes = 256;                 // canonical reduction lanes
+constexpr uint32_t kPerLane = 4;                 // particles per lane per chunk
+constexpr uint32_t kChunk = kLanes * kPerLane;   // particles per chunk
+constexpr int kDepositShift = 36;                // fixed-point deposit: 2^-36
+
+struct Particle {
+  double x, xp, y, yp, phi, w;
+  uint32_t loss, lloss;
+};
+
+enum OpKind : int32_t {
+  OP_DRIFT = 1,     // SimulatePartialDriftKernel
+  OP_QUAD = 2,      // SimulateHalfQuadKernel
+  OP_GAP1 = 3,      // SimulateRFGapFirstHalfKernel
+  OP_GAP2 = 4,      // SimulateRFGapSecondHalfKernel
+  OP_ROT = 5,       // SimulateRotationKernel
+  OP_STEER = 6,     // SimulateSteererKernel
+  OP_APER_C = 7,    // SimulateCircularApertureKernel
+  OP_APER_R = 8     // SimulateRectangularApertureKernel
+};
+
+// One map step.  a[] meaning per kind:
+//  DRIFT  h, aperture, wavelength
+//  QUAD   length, aperture, gradient, wavelength
+//  GAP1   design_w_in, phi_in, length, qlen1, qlen2
+//  GAP2   design_w_in, phi_in, length, qlen1, qlen2, wavelength
+//  ROT    angle
+//  STEER  bl_h, bl_v
+//  APER_C aperture        APER_R  xl, xr, yt, yb
+struct Op {
+  int32_t kind;
+  uint32_t elem;          // loss code written on loss (element model index)
+  int32_t ccl;            // gap form: 0 DTL, 1 CCL
+  uint32_t ccl_cell_num;
+  double a[6];
+  hps_gap_param g;
+};
+
+// Bessi0c, simulation_kernel.cu:15-36
+__device__ __forceinline__ double bessi0c(double r_x)
+{
+  double ans, y;
+  const double ax = __builtin_fabs(r_x);
+  if (ax < 3.75) {
+    y = r_x / 3.75;
+    y = y * y;
+    ans = 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 +
+          y * (0.360768e-1 + y * 0.45813e-2)))));
+  } else {
+    y = 3.75 / ax;
+    ans = (hps::exp(ax) * (1.0 / ::sqrt(ax))) * (0.39894228 + y * (0.1328592e-1 +
+          y * (0.225319e-2 + y * (-0.157565e-2 + y * (0.916281e-2 + y * (-0.2057706e-1 +
+          y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+  }
+  return ans;
+}
+
+// Bessi1c, simulation_kernel.cu:38-64
+__device__ __forceinline__ double bessi1c(double r_x)
+{
+  double y, ans;
+  const double ax = __builtin_fabs(r_x);
+  if (ax < 3.75) {
+    y = r_x / 3.75;
+    y = y * y;
+    ans = ax * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 + y * (0.2658733e-1 +
+          y * (0.301532e-2 + y * 0.32411e-3))))));
+  } else {
+    y = 3.75 / ax;
+    ans = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
+    ans = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 + y * (0.163801e-2 + y *
+          (-0.1031555e-1 + y * ans))));
+    ans = ans * (hps::exp(ax) * (1.0 / ::sqrt(ax)));
+  }
+  return r_x < 0.0 ? -ans : ans;
+}
+
+// Bessi1p, simulation_kernel.cu:66-73
+__device__ __forceinline__ double bessi1p(double r_x)
+{
+  if (r_x != 0.0) return bessi0c(r_x) - bessi1c(r_x) / r_x;
+  return 0.5;
+}
+
+// GetTransitTimeFactors, simulation_kernel.cu:290-308
+__device__ __forceinline__ void ttf(double r_beta, const hps_gap_param& g, double& t,
+                                    double& tp, double& sp)
+{
+  double beta = r_beta;
+  if (r_beta > 1.1 * g.fit_beta_center) beta = g.fit_beta_center;
+  else if (r_beta < g.fit_beta_min) beta = g.fit_beta_min;
+  t = g.fit_t0 + beta * (g.fit_t1 + beta * (g.fit_t2 + beta * (g.fit_t3 + beta *
+      (g.fit_t4 + beta * g.fit_t5))));
+  const double coef = beta * beta / (TWOPI * g.fit_beta_center);
+  tp = coef * (g.fit_t1 + beta * (2 * g.fit_t2 + beta * (3 * g.fit_t3 + beta *
+       (4 * g.fit_t4 + 5 * beta * g.fit_t5))));
+  sp = -coef * (g.fit_s1 + beta * (2 * g.fit_s2 + beta * (3 * g.fit_s3 + beta *
+       (4 * g.fit_s4 + 5 * beta * g.fit_s5))));
+}
+
+// SimulatePartialDriftKernel body, simulation_kernel.cu:95-115
+__device__ __forceinline__ void drift(Particle& p, double h, double aper, double wavelen,
+                                      uint32_t elem, double mass)
+{
+  const double aper2 = aper * aper;
+  double x = p.x;
+  double y = p.y;
+  x += p.xp * h;
+  y += p.yp * h;
+  if (aper2 != 0.0 && x * x + y * y > aper2) {
+    p.loss = elem;
+  } else {
+    p.x = x;
+    p.y = y;
+    const double gm1 = p.w / mass;
+    const double bt = ::sqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
+    p.phi += TWOPI * h / (wavelen * bt);
+  }
+}
+
+// SimulateHalfQuadKernel body, simulation_kernel.cu:147-214
+__device__ __forceinline__ void half_quad(Particle& p, double length, double aper,
+                                          double gradient, double wavelen, uint32_t elem,
+                                          double mass, double charge)
+{
+  double aper2 = aper;
+  aper2 *= aper2;
+  const double qg = gradient * charge;
+  double x = p.x, y = p.y, xp = p.xp, yp = p.yp;
+  const double w = p.w;
+  if (qg == 0) {
+    // the loss test below uses the coordinates before this half-drift
+    p.x = x + xp * length * 0.5;
+    p.y = y + yp * length * 0.5;
+  } else {
+    const double gamma1 = w / mass;
+    const double bg = ::sqrt(gamma1 * (gamma1 + 2.0) * (1.0 + xp * xp + yp * yp));
+    const double absqg = qg > 0.0 ? qg : -qg;
+    const double k = ::sqrt(absqg / (bg * mass) * CLIGHT);
+    const double inv_k = 1.0 / k;
+    const double kds = length * 0.25 * k;
+    double s, c;
+    const double sh = hps::sinh(kds), ch = hps::cosh(kds);
+    hps::sincos(kds, &s, &c);
+    const double sovk = s * inv_k, shovk = sh * inv_k, sk = s * k, shk = sh * k;
+    double xx, yy;
+#pragma unroll
+    for (int rep = 0; rep < 2; ++rep) {
+      if (qg > 0.0) {
+        xx = c * x + sovk * xp; xp = -sk * x + c * xp; x = xx;
+        yy = ch * y + shovk * yp; yp = shk * y + ch * yp; y = yy;
+      } else {
+        xx = ch * x + shovk * xp; xp = shk * x + ch * xp; x = xx;
+        yy = c * y + sovk * yp; yp = -sk * y + c * yp; y = yy;
+      }
+    }
+    p.x = x; p.y = y; p.xp = xp; p.yp = yp;
+  }
+  if (aper2 != 0 && x * x + y * y > aper2) {
+    p.loss = elem;
+  } else {
+    const double gm1 = w / mass;
+    const double bt = ::sqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
+    p.phi += TWOPI * length * 0.5 / (wavelen * bt);
+  }
+}
+
+// SimulateRFGapFirstHalfKernel body, simulation_kernel.cu:239-285
+__device__ __forceinline__ void gap_first_half(Particle& p, const hps_gap_param& gap,
+                                               double design_w_in, double phi_in, double length,
+                                               double q1, double q2, bool ccl, double mass)
+{
+  const double x = p.x, y = p.y, xp = p.xp, yp = p.yp, phi = p.phi, w = p.w;
+  const double cell_len = length + q1 * 0.5 + q2 * 0.5;
+  const double dd1 = 0.5 * cell_len - gap.dg - 0.5 * q1;
+  p.x = x + dd1 * xp;
+  p.y = y + dd1 * yp;
+  const double wave_len = CLIGHT / gap.frequency;
+  if (gap.amplitude == 0.0) {
+    const double gm1 = w / mass;
+    const double bt = ::sqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
+    p.phi += TWOPI * dd1 / (wave_len * bt);
+  } else {
+    double gam = design_w_in / mass;
+    const double beta_in = ::sqrt((gam + 2.0) * gam) / (gam + 1.0);
+    gam = w / mass;
+    const double beta = ::sqrt((gam + 2.0) * gam) / (gam + 1.0);
+    if (!ccl) {
+      const double dps = gap.phase_ref - phi_in;
+      const double d_cell_len = cell_len - gap.beta_center * wave_len;
+      const double dphi_qlen1 = TWOPI * 0.5 * q1 / (beta * wave_len);
+      double dphi_clen = 0.0;
+      if (d_cell_len > 1e-15 || d_cell_len < -1e-15)
+        dphi_clen = PI * d_cell_len / (beta * wave_len);
+      p.phi = phi + (1.0 - (beta - beta_in) / beta) *
+              (PI * gap.cell_length_over_beta_lambda + dps) + dphi_clen - dphi_qlen1;
+    } else {
+      const double dps = gap.phase_ref - phi_in;
+      const double betag = cell_len / (gap.cell_length_over_beta_lambda * wave_len);
+      p.phi = phi + (1.0 - (beta - beta_in) * betag / (beta * beta_in)) *
+              (PI * gap.cell_length_over_beta_lambda + dps);
+    }
+  }
+}
+
+// SimulateRFGapSecondHalfKernel body, simulation_kernel.cu:326-493
+__device__ __forceinline__ void gap_second_half(Particle& p, const hps_gap_param& gap,
+                                                double design_w_in, double phi_in,
+                                                double length, double q1, double q2,
+                                                double wavelen, uint32_t ccl_cell_num, bool ccl,
+                                                double mass, double charge)
+{
+  const double cell_len = length + q1 * 0.5 + q2 * 0.5;
+  const double dd2 = 0.5 * cell_len + gap.dg - q2 * 0.5;
+  double x = p.x, y = p.y, xp = p.xp, yp = p.yp;
+  const double inv_mass = 1.0 / mass;
+  const double phi = p.phi;
+  double w = p.w;
+  const double gm_in = design_w_in * inv_mass + 1.0;
+  const double beta_in = ::sqrt(1.0 - 1.0 / (gm_in * gm_in));
+  double gam = w * inv_mass;
+  const double bg1 = ::sqrt(gam * (gam + 2.0));
+  double beta = bg1 / (gam + 1.0);
+  double t_fac, tp_fac, sp_fac;
+  ttf(beta, gap, t_fac, tp_fac, sp_fac);
+  if (gap.amplitude == 0.0 || (ccl && beta < gap.fit_beta_min)) {
+    p.x = x + dd2 * xp;
+    p.y = y + dd2 * yp;
+    const double gm1 = p.w / mass;
+    const double bt = ::sqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
+    p.phi += TWOPI * dd2 / (wavelen * bt);
+    return;
+  }
+  const double wave_len = CLIGHT / gap.frequency;
+  const double cl = cell_len / gap.cell_length_over_beta_lambda;
+  const double betag_ccl = cl / wave_len;
+  const double betag = ccl ? gap.beta_center : betag_ccl;
+  const double betag2 = betag * betag;
+  const double gmg = 1.0 / ::sqrt(1.0 - betag2);
+  const double inv_gmg2 = 1.0 / (gmg * gmg);
+  double q = charge;
+  q = q > 0.0 ? q : -q;
+  const double gm_out = gap.energy_out * inv_mass + 1.0;
+  const double beta_out = ::sqrt(1.0 - 1.0 / (gm_out * gm_out));
+  const double efl = gap.amplitude * cell_len;
+  const double concomm = q * efl / (mass * betag2);
+  const double concomm1 = concomm * inv_gmg2 / gmg;
+  const double con1 = PI * concomm1;
+  const double con2nex = concomm * inv_gmg2;
+  double con5nex, tw, inv_fkis, inv_gmgcl;
+  if (ccl) {
+    inv_gmgcl = 1.0 / (gmg * betag * wave_len);
+    inv_fkis = betag * wave_len / TWOPI;
+    con5nex = concomm * (t_fac * inv_fkis - betag_ccl * wave_len * tp_fac * inv_gmg2);
+    tw = t_fac - TWOPI * tp_fac * (beta_in / beta - 1.0) * betag_ccl / beta_in;
+  } else {
+    inv_gmgcl = 1.0 / (gmg * cl);
+    inv_fkis = cl / TWOPI;
+    con5nex = concomm * (t_fac * inv_fkis - beta_in * wave_len * tp_fac * inv_gmg2);
+    tw = t_fac;
+  }
+  const double con5nex1 = concomm1 * t_fac;
+  const double con5nex2 = -con2nex * t_fac * inv_fkis;
+  const double fkr = TWOPI * inv_gmgcl;
+  double sinps, cosps;
+  hps::sincos(gap.phase_ref, &sinps, &cosps);
+  double phi1, sinp1, cosp1;
+  if (ccl)
+    phi1 = phi + gap.phase_shift - TWOPI * gap.dg / (beta * wave_len) + ccl_cell_num * PI;
+  else
+    phi1 = phi + gap.phase_shift;
+  hps::sincos(phi1, &sinp1, &cosp1);
+  const double prome1 = -con1 * (tp_fac * (sinp1 - sinps) + sp_fac * (cosp1 - cosps));
+  hps::sincos(phi1 + prome1, &sinp1, &cosp1);
+  const double rs_p = x * x + y * y;
+  const double rrp = x * xp + y * yp;
+  double rpris = 0.0, bes1covr;
+  const double ris = ::sqrt(rs_p);
+  const double fkrris = fkr * ris;
+  const double b0c = bessi0c(fkrris);
+  const double b1c = bessi1c(fkrris);
+  const double b1p = bessi1p(fkrris);
+  if (ris != 0.0) bes1covr = b1c / ris;
+  else bes1covr = fkr * 0.5;
+  const double delte = q * efl * (tw * b0c * cosp1 + ((tw - TWOPI * tp_fac) / gmg * bes1covr +
+                       tw / gmg * fkr * b1p - tw * gmg * bes1covr) * rrp * sinp1);
+  w += delte;
+  gam = w * inv_mass;
+  const double bg = ::sqrt(gam * (gam + 2.0));
+  beta = bg / (gam + 1.0);
+  const double bgr = bg1 / bg;
+  const double onema = 1.0 - (con5nex * bes1covr + con5nex1 * b1p + con5nex2 * bes1covr) * cosp1;
+  const double dovbl = con2nex * bes1covr * tw * sinp1;
+  const double bgrovonema = bgr / onema;
+  xp = -dovbl * x + bgrovonema * xp;
+  x = x * onema + dd2 * xp;
+  yp = -dovbl * y + bgrovonema * yp;
+  y = y * onema + dd2 * yp;
+  if (rs_p > 0.0) rpris = rrp / ris;
+  const double prome = 2.0 * con1 * (-tp_fac * b0c + ris * tw * b1c * inv_gmgcl) * (sinp1 - sinps) -
+                       efl * inv_mass * tw * b1c * rpris * (cosp1 - cosps);
+  double phi_new;
+  if (ccl) {
+    const double dps = phi_in - gap.phase_ref;
+    phi_new = phi + (1.0 - (beta - beta_out) * betag_ccl / (beta * beta_out)) *
+              (PI * gap.cell_length_over_beta_lambda + dps) + prome;
+  } else {
+    const double d_cell_len = cell_len - gap.beta_center * wave_len;
+    const double dphi_qlen2 = TWOPI * 0.5 * q2 / (beta * wave_len);
+    double dphi_clen = 0.0;
+    if (d_cell_len > 1e-15 || d_cell_len < -1e-15)
+      dphi_clen = PI * d_cell_len / (beta * wave_len);
+    const double dps = gap.phase_ref - phi_in;
+    phi_new = phi + (1.0 - (beta - beta_out) / beta) * (PI * gap.cell_length_over_beta_lambda +
+              dps) + prome + dphi_clen - dphi_qlen2;
+  }
+  if (w <= 0.0 || x != x || y != y || xp != xp || yp != yp || w != w || phi_new != phi_new) {
+    p.loss = 33333333;
+  } else {
+    p.x = x; p.y = y; p.xp = xp; p.yp = yp; p.w = w; p.phi = phi_new;
+  }
+}
+
+// SimulateRotationKernel body, simulation_kernel.cu:516-528
+__device__ __forceinline__ void rotation(Particle& p, double angle)
+{
+  double sina, cosa;
+  hps::sincos(angle, &sina, &cosa);
+  const double x = p.x, y = p.y, xp = p.xp, yp = p.yp;
+  p.x = x * cosa + y * sina;
+  p.y = y * cosa - x * sina;
+  p.xp = xp * cosa + yp * sina;
+  p.yp = yp * cosa - xp * sina;
+}
+
+// SimulateSteererKernel body, simulation_kernel.cu:128-137
+__device__ __forceinline__ void steerer(Particle& p, double blh, double blv, double mass,
+                                        double charge)
+{
+  const double gm1 = p.w / mass;
+  const double btgm = ::sqrt(gm1 * (gm1 + 2.0));
+  const double pm = btgm * mass / CLIGHT;
+  if (blh > 1e-10) p.xp += blh * charge / pm;
+  if (blv > 1e-10) p.yp += blv * charge / pm;
+}
+
+// Apply one map step to an alive particle.
+__device__ __forceinline__ void apply_op(const Op& op, Particle& p, uint32_t index, double mass,
+                                         double charge, double x_avg, double y_avg)
+{
+  switch (op.kind) {
+    case OP_DRIFT: drift(p, op.a[0], op.a[1], op.a[2], op.elem, mass); break;
+    case OP_QUAD: half_quad(p, op.a[0], op.a[1], op.a[2], op.a[3], op.elem, mass, charge); break;
+    case OP_GAP1:
+      gap_first_half(p, op.g, op.a[0], op.a[1], op.a[2], op.a[3], op.a[4], op.ccl != 0, mass);
+      break;
+    case OP_GAP2:
+      gap_second_half(p, op.g, op.a[0], op.a[1], op.a[2], op.a[3], op.a[4], op.a[5],
+                      op.ccl_cell_num, op.ccl != 0, mass, charge);
+      break;
+    case OP_ROT: rotation(p, op.a[0]); break;
+    case OP_STEER: steerer(p, op.a[0], op.a[1], mass, charge); break;
+    case OP_APER_C: {   // simulation_kernel.cu:548-555 (particle 0 exempt)
+      if (index == 0) break;
+      const double aper2 = op.a[0] * op.a[0];
+      const double x = p.x - x_avg, y = p.y - y_avg;
+      if (x * x + y * y > aper2) p.loss = op.elem;
+      break;
+    }
+    case OP_APER_R: {   // simulation_kernel.cu:564-574 (particle 0 exempt)
+      if (index == 0) break;
+      const double xmin = x_avg - op.a[0], xmax = x_avg + op.a[1];
+      const double ymin = y_avg - op.a[3], ymax = y_avg + op.a[2];
+      if (p.x < xmin || p.x > xmax || p.y < ymin || p.y > ymax) p.loss = op.elem;
+      break;
+    }
+    default: break;
+  }
+}
+
+// UpdateRelativePhiKernel body, beam_kernel.cu:1514-1520
+__device__ __forceinline__ double relative_phi(double phi, double ref)
+{
+  double tmp = phi - ref + PI;
+  double sign = 1.0;
+  if (tmp < 0) { sign = -1.0; tmp = -tmp; }
+  return sign * (hps::fmod_pos(tmp, TWOPI) - PI);
+}
+
+}  // namespace hps

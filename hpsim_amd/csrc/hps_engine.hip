@@ -1,0 +1,1199 @@
+// hps_engine.hip -- host side of the tracking engine and its C ABI.
+//
+// Beam, BeamLine, SpaceCharge and Simulator mirror the reference C++ classes
+// (src/beam, src/beamline, src/simulation) behind the C ABI of hpsim_amd.h.
+// simulate() compiles the element range into fused segments (all maps between
+// two space-charge kicks) and launches them back to back on one stream; the
+// Scheff::Start decisions run on the device, so the host never waits inside a
+// simulate() call except for the rare beam frequency change.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hpsim_amd.h"
+#include "hps_internal.hpp"
+
+using namespace hps;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(const std::string& m)
+{
+  g_error = m;
+  return -1;
+}
+
+#define HPS_CHECK(expr)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(std::string(#expr) + ": " + hipGetErrorString(e_));                     \
+  } while (0)
+
+uint32_t next_pow2(uint32_t x)   // utility.cpp:7-16
+{
+  --x;
+  x |= x >> 1; x |= x >> 2; x |= x >> 4; x |= x >> 8; x |= x >> 16;
+  return ++x;
+}
+
+hipStream_t g_stream[16];
+
+int stream_for(int dev, hipStream_t* out)
+{
+  if (dev < 0 || dev >= 16) return fail("device index out of range");
+  if (!g_stream[dev]) {
+    HPS_CHECK(hipSetDevice(dev));
+    HPS_CHECK(hipStreamCreate(&g_stream[dev]));
+  }
+  *out = g_stream[dev];
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Beam
+// ---------------------------------------------------------------------------
+struct hps_beam {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t n = 0;
+  double mass = 0, charge = 0, current = 0, freq = 0, design_w = 0;
+  uint32_t grid_size = 1, blck_size = 1;
+  BeamArrays d{};
+  hps_beam_stats* st = nullptr;     // device
+  Partials pr{};
+  double* snap[2][6] = {{nullptr}};
+  uint32_t* snap_l[2][2] = {{nullptr}};
+  bool saved[2] = {false, false};
+  double snap_dw[2] = {0, 0}, snap_f[2] = {0, 0}, snap_c[2] = {0, 0};
+};
+
+namespace {
+
+// Canonical-order sums over the beam (k_sums), copied to host.
+struct Sums {
+  double s[5];
+  uint32_t nloss, ngood, nlloss;
+};
+
+int run_sums(hps_beam* b, const double* a, const double* bb, int filter, Sums* out)
+{
+  HPS_CHECK(launch_sums(b->d, a, bb, filter, b->pr, b->stream));
+  HPS_CHECK(hipMemcpyAsync(out->s, b->pr.out, 5 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  uint32_t c[3];
+  HPS_CHECK(hipMemcpyAsync(c, b->pr.out_cnt, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  out->nloss = c[0];
+  out->ngood = c[1];
+  out->nlloss = c[2];
+  return 0;
+}
+
+int pull(hps_beam* b, hps_beam_stats* s)
+{
+  HPS_CHECK(hipMemcpyAsync(s, b->st, sizeof(*s), hipMemcpyDeviceToHost, b->stream));
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int push(hps_beam* b, const hps_beam_stats* s)
+{
+  HPS_CHECK(hipMemcpyAsync(b->st, s, sizeof(*s), hipMemcpyHostToDevice, b->stream));
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+double surv(const hps_beam* b, const hps_beam_stats& s) { return (double)(uint32_t)(b->n - s.num_loss); }
+
+// UpdateLossCountKernelCall, beam.cu:493-516
+int update_loss(hps_beam* b, hps_beam_stats& s)
+{
+  Sums r;
+  if (run_sums(b, nullptr, nullptr, 2, &r)) return -1;
+  s.num_loss = r.nloss;
+  return 0;
+}
+
+// UpdateGoodParticleCountKernelCall, beam.cu:1233-1255
+int update_good(hps_beam* b, hps_beam_stats& s)
+{
+  Sums r;
+  if (run_sums(b, nullptr, nullptr, 2, &r)) return -1;
+  s.num_good = r.ngood;
+  return 0;
+}
+
+// UpdateAvgOfOneVariableKernelCall + finalise kernels, beam.cu:411-436,
+// beam_kernel.cu:1396-1410
+int avg_of(hps_beam* b, hps_beam_stats& s, const double* a, bool good, double* out)
+{
+  Sums r;
+  if (run_sums(b, a, nullptr, good ? 1 : 0, &r)) return -1;
+  if (!good) *out = r.s[0] / surv(b, s);
+  else *out = s.num_good > 0 ? r.s[0] / (double)s.num_good : -99999;
+  return 0;
+}
+
+// UpdateSigmaOfOneVariableKernelCall, beam.cu:651-677
+int sig_of(hps_beam* b, hps_beam_stats& s, const double* a, bool good, double* out)
+{
+  Sums r;
+  if (run_sums(b, a, nullptr, good ? 1 : 0, &r)) return -1;
+  const double d = good ? (double)s.num_good : surv(b, s);
+  const double avg = r.s[0] / d, avg2 = r.s[1] / d;
+  *out = std::sqrt(avg2 - avg * avg);
+  return 0;
+}
+
+// UpdateRelativePhiKernelCall, beam.cu:1122-1131 (writes phi_r)
+int relative_phi(hps_beam* b, bool use_good)
+{
+  const double* ref = use_good ? &b->st->phi_avg_good : b->d.phi;
+  HPS_CHECK(launch_relative_phi(b->d, ref, b->stream));
+  return 0;
+}
+
+// Beam::UpdateLongitudinalLoss, beam.cpp:776-782
+int update_longitudinal_loss(hps_beam* b, hps_beam_stats& s)
+{
+  if (update_good(b, s)) return -1;
+  if (avg_of(b, s, b->d.phi, true, &s.phi_avg_good)) return -1;
+  if (push(b, &s)) return -1;
+  HPS_CHECK(launch_lloss_flag(b->d, &b->st->phi_avg_good, b->stream));
+  Sums r;
+  if (run_sums(b, nullptr, nullptr, 2, &r)) return -1;
+  s.num_lloss = r.nlloss;
+  return 0;
+}
+
+// Beam::InitPhiAvgGood, beam.cpp:655-661
+int init_phi_avg_good(hps_beam* b)
+{
+  hps_beam_stats s;
+  if (pull(b, &s) || update_loss(b, s) || avg_of(b, s, b->d.phi, false, &s.phi_avg)) return -1;
+  s.phi_avg_good = s.phi_avg;
+  return push(b, &s);
+}
+
+// Beam::ChangeFrequency, beam.cpp:1654-1668
+int change_frequency(hps_beam* b, double f)
+{
+  const double ratio = f / b->freq;
+  b->current *= ratio;
+  HPS_CHECK(launch_scale(b->d.phi, b->n, ratio, b->stream));
+  b->freq = f;
+  hps_beam_stats s;
+  if (pull(b, &s) || avg_of(b, s, b->d.phi, false, &s.phi_avg) || push(b, &s)) return -1;
+  if (relative_phi(b, false)) return -1;
+  if (avg_of(b, s, b->d.phi_r, false, &s.phi_avg_r)) return -1;
+  return push(b, &s);
+}
+
+int emittance(hps_beam* b, hps_beam_stats& s)   // Beam::UpdateEmittance, beam.cpp:985-994
+{
+  if (b->n <= 2) return 0;
+  double w0;
+  HPS_CHECK(hipMemcpyAsync(&w0, b->d.w, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  for (int axis = 0; axis < 2; ++axis) {   // beam.cu:1007-1064, beam_kernel.cu:1333-1354
+    Sums r;
+    if (run_sums(b, axis == 0 ? b->d.x : b->d.y, axis == 0 ? b->d.xp : b->d.yp, 0, &r)) return -1;
+    const double sv = surv(b, s);
+    const double x = r.s[0] / sv, x2 = r.s[1] / sv, xp = r.s[2] / sv, xp2 = r.s[3] / sv,
+                 xxp = r.s[4] / sv;
+    const double gamma1 = w0 / b->mass;
+    const double betagamma = std::sqrt(gamma1 * (gamma1 + 2.0));
+    const double sig = std::sqrt(x2 - x * x), sigp = std::sqrt(xp2 - xp * xp);
+    const double emit = std::sqrt((x2 - x * x) * (xp2 - xp * xp) - (xxp - x * xp) * (xxp - x * xp)) *
+                        betagamma;
+    if (axis == 0) { s.x_avg = x; s.x_sig = sig; s.xp_avg = xp; s.xp_sig = sigp; s.x_emit = emit; }
+    else { s.y_avg = x; s.y_sig = sig; s.yp_avg = xp; s.yp_sig = sigp; s.y_emit = emit; }
+  }
+  if (relative_phi(b, false)) return -1;
+  Sums r;   // beam.cu:1069-1092, beam_kernel.cu:1356-1373
+  if (run_sums(b, b->d.phi_r, b->d.w, 0, &r)) return -1;
+  const double sv = surv(b, s);
+  const double phi = r.s[0] / sv, phi2 = r.s[1] / sv, w = r.s[2] / sv, w2 = r.s[3] / sv,
+               phi_w = r.s[4] / sv;
+  const double phi_var = phi2 - phi * phi;
+  const double w_var = w2 - w * w;
+  const double cov = phi_w - phi * w;
+  s.phi_avg_r = phi;
+  s.w_avg = w;
+  s.phi_sig_r = std::sqrt(phi_var);
+  s.w_sig = std::sqrt(w_var);
+  s.z_emit = std::sqrt(phi_var * w_var - cov * cov);
+  return 0;
+}
+
+void free_beam(hps_beam* b)
+{
+  double* arrs[7] = {b->d.x, b->d.xp, b->d.y, b->d.yp, b->d.phi, b->d.phi_r, b->d.w};
+  for (double* p : arrs) (void)hipFree(p);
+  (void)hipFree(b->d.loss);
+  (void)hipFree(b->d.lloss);
+  (void)hipFree(b->st);
+  (void)hipFree(b->pr.sums);
+  (void)hipFree(b->pr.cnt);
+  (void)hipFree(b->pr.pmax);
+  (void)hipFree(b->pr.ticket);
+  (void)hipFree(b->pr.out);
+  (void)hipFree(b->pr.out_cnt);
+  for (int k = 0; k < 2; ++k) {
+    for (int j = 0; j < 6; ++j) (void)hipFree(b->snap[k][j]);
+    (void)hipFree(b->snap_l[k][0]);
+    (void)hipFree(b->snap_l[k][1]);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// BeamLine / SpaceCharge / Simulator
+// ---------------------------------------------------------------------------
+struct hps_beamline {
+  std::vector<std::string> names;
+  std::vector<hps_element> el;
+};
+
+struct hps_spch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hps_spch_state cfg{};   // host copy of the configuration
+  ScDev* dev = nullptr;
+  double* fld1 = nullptr;
+  double* fld2 = nullptr;
+  int64_t* bins = nullptr;
+};
+
+struct hps_sim {
+  hps_beam* b = nullptr;
+  hps_beamline* bl = nullptr;
+  hps_spch* sc = nullptr;
+  bool sc_on = false;
+  uint32_t ccl_cell_num = 0;
+  double design_w = 0, wavelen = 0;
+  int32_t prev_end = -1;      // uint max stored in int, simulation_engine.cpp:52
+  Op* d_ops = nullptr;
+  size_t ops_cap = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+  // per-kernel-family device time (HIP events around each launch)
+  bool profile = false;
+  std::vector<hipEvent_t> evpool;
+  double prof_ms[5] = {0, 0, 0, 0, 0};
+  uint32_t prof_n[5] = {0, 0, 0, 0, 0};
+};
+
+extern "C" {
+
+const char* hps_last_error(void) { return g_error.c_str(); }
+
+int hps_set_device(int device)
+{
+  HPS_CHECK(hipSetDevice(device));
+  return 0;
+}
+
+int hps_beam_create(hps_beam** out, uint32_t num, double mass, double charge, double current)
+{
+  if (!out) return fail("null output handle");
+  if (num == 0) return fail("Beam needs num > 0 particles");
+  hps_beam* b = new hps_beam();
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) { delete b; return fail("no HIP device"); }
+  b->device = dev;
+  if (stream_for(dev, &b->stream)) { delete b; return -1; }
+  b->n = num; b->mass = mass; b->charge = charge; b->current = current;
+  // Beam::SetNumThreadsPerBlock(256), beam.cpp:148-168
+  const uint32_t nt = next_pow2(num);
+  if (nt > 256) { b->grid_size = nt / 256; b->blck_size = 256; }
+  else { b->grid_size = 1; b->blck_size = nt; }
+  const size_t nd = (size_t)num * sizeof(double), nu = (size_t)num * sizeof(uint32_t);
+  double** arrs[7] = {&b->d.x, &b->d.xp, &b->d.y, &b->d.yp, &b->d.phi, &b->d.phi_r, &b->d.w};
+  bool ok = true;
+  for (double** p : arrs) ok = ok && hipMalloc(p, nd) == hipSuccess && hipMemset(*p, 0, nd) == hipSuccess;
+  ok = ok && hipMalloc(&b->d.loss, nu) == hipSuccess && hipMemset(b->d.loss, 0, nu) == hipSuccess;
+  ok = ok && hipMalloc(&b->d.lloss, nu) == hipSuccess && hipMemset(b->d.lloss, 0, nu) == hipSuccess;
+  ok = ok && hipMalloc(&b->st, sizeof(hps_beam_stats)) == hipSuccess &&
+       hipMemset(b->st, 0, sizeof(hps_beam_stats)) == hipSuccess;
+  b->pr.nch = (num + kChunk - 1) / kChunk;
+  b->pr.mpad = next_pow2(b->pr.nch);
+  const size_t nsum = (size_t)kMaxQ * b->pr.mpad + b->pr.mpad / 2 + 256;
+  ok = ok && hipMalloc(&b->pr.sums, nsum * sizeof(double)) == hipSuccess &&
+       hipMemset(b->pr.sums, 0, nsum * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc(&b->pr.cnt, 3 * (size_t)b->pr.mpad * 4) == hipSuccess &&
+       hipMemset(b->pr.cnt, 0, 3 * (size_t)b->pr.mpad * 4) == hipSuccess;
+  ok = ok && hipMalloc(&b->pr.pmax, (size_t)b->pr.mpad * 8) == hipSuccess &&
+       hipMemset(b->pr.pmax, 0, (size_t)b->pr.mpad * 8) == hipSuccess;
+  ok = ok && hipMalloc(&b->pr.ticket, 16) == hipSuccess && hipMemset(b->pr.ticket, 0, 16) == hipSuccess;
+  ok = ok && hipMalloc(&b->pr.out, kMaxQ * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc(&b->pr.out_cnt, 4 * sizeof(uint32_t)) == hipSuccess;
+  if (!ok) { free_beam(b); delete b; return fail("device allocation failed for the beam"); }
+  b->d.n = num; b->d.mass = mass; b->d.charge = charge;
+  *out = b;
+  return 0;
+}
+
+int hps_beam_destroy(hps_beam* b)
+{
+  if (!b) return 0;
+  (void)hipStreamSynchronize(b->stream);
+  free_beam(b);
+  delete b;
+  return 0;
+}
+
+uint32_t hps_beam_size(const hps_beam* b) { return b ? b->n : 0; }
+
+int hps_beam_set_distribution(hps_beam* b, const double* x, const double* xp, const double* y,
+    const double* yp, const double* phi, const double* w, const uint32_t* loss, const uint32_t* lloss)
+{
+  if (!b || !x || !xp || !y || !yp || !phi || !w) return fail("set_distribution needs six coordinate arrays");
+  const size_t nd = (size_t)b->n * sizeof(double), nu = (size_t)b->n * 4;
+  HPS_CHECK(hipMemcpy(b->d.x, x, nd, hipMemcpyHostToDevice));
+  HPS_CHECK(hipMemcpy(b->d.xp, xp, nd, hipMemcpyHostToDevice));
+  HPS_CHECK(hipMemcpy(b->d.y, y, nd, hipMemcpyHostToDevice));
+  HPS_CHECK(hipMemcpy(b->d.yp, yp, nd, hipMemcpyHostToDevice));
+  HPS_CHECK(hipMemcpy(b->d.phi, phi, nd, hipMemcpyHostToDevice));
+  HPS_CHECK(hipMemcpy(b->d.w, w, nd, hipMemcpyHostToDevice));
+  // the reference particle sits on axis, beam.cpp:344-345
+  const double zero = 0.0;
+  HPS_CHECK(hipMemcpy(b->d.x, &zero, 8, hipMemcpyHostToDevice));
+  HPS_CHECK(hipMemcpy(b->d.xp, &zero, 8, hipMemcpyHostToDevice));
+  HPS_CHECK(hipMemcpy(b->d.y, &zero, 8, hipMemcpyHostToDevice));
+  HPS_CHECK(hipMemcpy(b->d.yp, &zero, 8, hipMemcpyHostToDevice));
+  const uint32_t uz = 0;
+  if (loss) {
+    HPS_CHECK(hipMemcpy(b->d.loss, loss, nu, hipMemcpyHostToDevice));
+    HPS_CHECK(hipMemcpy(b->d.loss, &uz, 4, hipMemcpyHostToDevice));
+  } else {
+    HPS_CHECK(hipMemset(b->d.loss, 0, nu));
+  }
+  if (lloss) {
+    HPS_CHECK(hipMemcpy(b->d.lloss, lloss, nu, hipMemcpyHostToDevice));
+    HPS_CHECK(hipMemcpy(b->d.lloss, &uz, 4, hipMemcpyHostToDevice));
+  } else {
+    HPS_CHECK(hipMemset(b->d.lloss, 0, nu));
+  }
+  HPS_CHECK(hipDeviceSynchronize());
+  return init_phi_avg_good(b);
+}
+
+int hps_beam_download(hps_beam* b, double* x, double* xp, double* y, double* yp, double* phi,
+                      double* w, uint32_t* loss, uint32_t* lloss)
+{
+  if (!b) return fail("null beam");
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  const size_t nd = (size_t)b->n * sizeof(double), nu = (size_t)b->n * 4;
+  if (x) HPS_CHECK(hipMemcpy(x, b->d.x, nd, hipMemcpyDeviceToHost));
+  if (xp) HPS_CHECK(hipMemcpy(xp, b->d.xp, nd, hipMemcpyDeviceToHost));
+  if (y) HPS_CHECK(hipMemcpy(y, b->d.y, nd, hipMemcpyDeviceToHost));
+  if (yp) HPS_CHECK(hipMemcpy(yp, b->d.yp, nd, hipMemcpyDeviceToHost));
+  if (phi) HPS_CHECK(hipMemcpy(phi, b->d.phi, nd, hipMemcpyDeviceToHost));
+  if (w) HPS_CHECK(hipMemcpy(w, b->d.w, nd, hipMemcpyDeviceToHost));
+  if (loss) HPS_CHECK(hipMemcpy(loss, b->d.loss, nu, hipMemcpyDeviceToHost));
+  if (lloss) HPS_CHECK(hipMemcpy(lloss, b->d.lloss, nu, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int hps_beam_relative_phi(hps_beam* b, int use_good, double* out)
+{
+  if (!b) return fail("null beam");
+  if (relative_phi(b, use_good != 0)) return -1;
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  if (out) HPS_CHECK(hipMemcpy(out, b->d.phi_r, (size_t)b->n * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int hps_beam_set_ref_w(hps_beam* b, double w)   // beam.cpp:1074-1078
+{
+  if (!b) return fail("null beam");
+  HPS_CHECK(hipMemcpy(b->d.w, &w, 8, hipMemcpyHostToDevice));
+  b->design_w = w;
+  return 0;
+}
+
+int hps_beam_set_ref_phi(hps_beam* b, double phi)   // beam.cpp:1086-1089
+{
+  if (!b) return fail("null beam");
+  HPS_CHECK(hipMemcpy(b->d.phi, &phi, 8, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int hps_beam_set_frequency(hps_beam* b, double f)   // wrap_beam.cpp:315-341
+{
+  if (!b) return fail("null beam");
+  if (b->freq == 0) { b->freq = f; return 0; }
+  return change_frequency(b, f);
+}
+
+int hps_beam_get_scalar(hps_beam* b, int which, double* out)
+{
+  if (!b || !out) return fail("null argument");
+  double v = 0;
+  switch (which) {
+    case 0: v = b->mass; break;
+    case 1: v = b->charge; break;
+    case 2: v = b->freq; break;
+    case 3: HPS_CHECK(hipStreamSynchronize(b->stream)); HPS_CHECK(hipMemcpy(&v, b->d.w, 8, hipMemcpyDeviceToHost)); break;
+    case 4: HPS_CHECK(hipStreamSynchronize(b->stream)); HPS_CHECK(hipMemcpy(&v, b->d.phi, 8, hipMemcpyDeviceToHost)); break;
+    case 5: v = b->current; break;
+    case 6: v = b->design_w; break;
+    case 7: {   // BeamGetCurrent, wrap_beam.cpp:1286-1298
+      hps_beam_stats s;
+      if (pull(b, &s) || update_loss(b, s) || push(b, &s)) return -1;
+      v = b->current * (1.0 - ((double)s.num_loss) / ((double)b->n));
+      break;
+    }
+    default: return fail("unknown scalar");
+  }
+  *out = v;
+  return 0;
+}
+
+int hps_beam_get_loss_num(hps_beam* b, int which, uint32_t* out)   // wrap_beam.cpp:1245-1284
+{
+  if (!b || !out) return fail("null argument");
+  hps_beam_stats s;
+  if (pull(b, &s)) return -1;
+  if (which == 0) {
+    if (update_loss(b, s)) return -1;
+    *out = s.num_loss;
+  } else if (which == 1) {
+    if (update_longitudinal_loss(b, s)) return -1;
+    *out = s.num_lloss;
+  } else {
+    if (update_loss(b, s) || update_longitudinal_loss(b, s) || update_good(b, s)) return -1;
+    *out = b->n - s.num_good;
+  }
+  return push(b, &s);
+}
+
+int hps_beam_get_avg(hps_beam* b, int var, int good, double* out)   // wrap_beam.cpp:914-1049
+{
+  if (!b || !out) return fail("null argument");
+  hps_beam_stats s;
+  if (pull(b, &s) || update_loss(b, s)) return -1;
+  double v = 0;
+  int rc = 0;
+  switch (var) {
+    case 0: rc = avg_of(b, s, b->d.x, good != 0, good ? &s.x_avg_good : &s.x_avg); v = good ? s.x_avg_good : s.x_avg; break;
+    case 1: rc = avg_of(b, s, b->d.y, good != 0, good ? &s.y_avg_good : &s.y_avg); v = good ? s.y_avg_good : s.y_avg; break;
+    case 2:
+      if (!good) { rc = avg_of(b, s, b->d.phi, false, &s.phi_avg); v = s.phi_avg; }
+      else { rc = update_good(b, s) || avg_of(b, s, b->d.phi, true, &s.phi_avg_good); v = s.phi_avg_good; }
+      break;
+    case 3: rc = avg_of(b, s, b->d.w, good != 0, good ? &s.w_avg_good : &s.w_avg); v = good ? s.w_avg_good : s.w_avg; break;
+    default:
+      rc = relative_phi(b, false) || avg_of(b, s, b->d.phi_r, good != 0, &s.phi_avg_r);
+      v = s.phi_avg_r;
+      break;
+  }
+  if (rc || push(b, &s)) return -1;
+  *out = v;
+  return 0;
+}
+
+int hps_beam_get_sig(hps_beam* b, int var, int good, double* out)   // wrap_beam.cpp:1051-1186
+{
+  if (!b || !out) return fail("null argument");
+  hps_beam_stats s;
+  if (pull(b, &s) || update_loss(b, s)) return -1;
+  if (b->n < 2) { *out = 0.0; return push(b, &s); }
+  const bool upd = b->n > 2;
+  double v = 0;
+  int rc = 0;
+  switch (var) {
+    case 0:
+      if (upd) rc = sig_of(b, s, b->d.x, good != 0, good ? &s.x_sig_good : &s.x_sig);
+      v = good ? s.x_sig_good : s.x_sig;
+      break;
+    case 1:
+      if (upd) rc = sig_of(b, s, b->d.y, good != 0, good ? &s.y_sig_good : &s.y_sig);
+      v = good ? s.y_sig_good : s.y_sig;
+      break;
+    case 2:
+      if (!good) { if (upd) rc = sig_of(b, s, b->d.phi, false, &s.phi_sig); v = s.phi_sig; break; }
+      rc = update_good(b, s) || avg_of(b, s, b->d.phi, true, &s.phi_avg_good) || push(b, &s) ||
+           relative_phi(b, true);
+      if (!rc && upd) rc = sig_of(b, s, b->d.phi_r, true, &s.phi_sig_good);
+      v = s.phi_sig_good;
+      break;
+    case 3:
+      if (upd) rc = sig_of(b, s, b->d.w, good != 0, good ? &s.w_sig_good : &s.w_sig);
+      v = good ? s.w_sig_good : s.w_sig;
+      break;
+    default:
+      rc = relative_phi(b, false);
+      if (!rc && upd) rc = sig_of(b, s, b->d.phi_r, false, &s.phi_sig_r);
+      v = s.phi_sig_r;
+      break;
+  }
+  if (rc || push(b, &s)) return -1;
+  *out = v;
+  return 0;
+}
+
+int hps_beam_get_emittance(hps_beam* b, int axis, double* out)   // wrap_beam.cpp:1188-1243
+{
+  if (!b || !out) return fail("null argument");
+  hps_beam_stats s;
+  if (pull(b, &s) || update_loss(b, s) || emittance(b, s) || push(b, &s)) return -1;
+  if (b->n < 2) { *out = 0.0; return 0; }
+  *out = axis == 0 ? s.x_emit : (axis == 1 ? s.y_emit : s.z_emit);
+  return 0;
+}
+
+int hps_beam_get_stats(hps_beam* b, hps_beam_stats* out)
+{
+  if (!b || !out) return fail("null argument");
+  return pull(b, out);
+}
+
+int hps_beam_save(hps_beam* b, int slot)   // beam.cpp:547-595
+{
+  if (!b || slot < 0 || slot > 1) return fail("bad save slot");
+  const size_t nd = (size_t)b->n * 8, nu = (size_t)b->n * 4;
+  double* src[6] = {b->d.x, b->d.xp, b->d.y, b->d.yp, b->d.phi, b->d.w};
+  for (int j = 0; j < 6; ++j) {
+    if (!b->snap[slot][j]) HPS_CHECK(hipMalloc(&b->snap[slot][j], nd));
+    HPS_CHECK(hipMemcpyAsync(b->snap[slot][j], src[j], nd, hipMemcpyDeviceToDevice, b->stream));
+  }
+  for (int j = 0; j < 2; ++j) {
+    if (!b->snap_l[slot][j]) HPS_CHECK(hipMalloc(&b->snap_l[slot][j], nu));
+    HPS_CHECK(hipMemcpyAsync(b->snap_l[slot][j], j ? b->d.lloss : b->d.loss, nu,
+                             hipMemcpyDeviceToDevice, b->stream));
+  }
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  b->saved[slot] = true;
+  b->snap_dw[slot] = b->design_w; b->snap_f[slot] = b->freq; b->snap_c[slot] = b->current;
+  return 0;
+}
+
+int hps_beam_restore(hps_beam* b, int slot)   // beam.cpp:606-647
+{
+  if (!b || slot < 0 || slot > 1) return fail("bad restore slot");
+  if (!b->saved[slot]) return fail("Beam can't be restored because no previous beam has been saved.");
+  const size_t nd = (size_t)b->n * 8, nu = (size_t)b->n * 4;
+  double* dst[6] = {b->d.x, b->d.xp, b->d.y, b->d.yp, b->d.phi, b->d.w};
+  for (int j = 0; j < 6; ++j)
+    HPS_CHECK(hipMemcpyAsync(dst[j], b->snap[slot][j], nd, hipMemcpyDeviceToDevice, b->stream));
+  HPS_CHECK(hipMemcpyAsync(b->d.loss, b->snap_l[slot][0], nu, hipMemcpyDeviceToDevice, b->stream));
+  HPS_CHECK(hipMemcpyAsync(b->d.lloss, b->snap_l[slot][1], nu, hipMemcpyDeviceToDevice, b->stream));
+  b->design_w = b->snap_dw[slot]; b->freq = b->snap_f[slot]; b->current = b->snap_c[slot];
+  hps_beam_stats s;
+  if (pull(b, &s) || avg_of(b, s, b->d.phi, true, &s.phi_avg_good) || push(b, &s)) return -1;
+  if (relative_phi(b, true)) return -1;
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int hps_beam_translate(hps_beam* b, int axis, double val)   // beam.cpp:1612-1652
+{
+  if (!b || axis < 0 || axis > 5) return fail("bad translate axis");
+  double* arr[6] = {b->d.x, b->d.xp, b->d.y, b->d.yp, b->d.phi, b->d.w};
+  HPS_CHECK(launch_shift(arr[axis], b->n, val, b->stream));
+  if (axis == 4) return init_phi_avg_good(b);
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int hps_beam_apply_cut(hps_beam* b, char coord, double lo, double hi)   // beam.cpp:1595-1610
+{
+  if (!b) return fail("null beam");
+  const double* cd = nullptr;
+  switch (coord) {
+    case 'x': case 'X': cd = b->d.x; break;
+    case 'y': case 'Y': cd = b->d.y; break;
+    case 'w': case 'W': cd = b->d.w; break;
+    case 'p': case 'P': cd = b->d.phi; break;
+    default: return fail("apply_cut coordinate must be one of x, y, w, p");
+  }
+  HPS_CHECK(launch_cut(cd, b->d.loss, b->n, lo, hi, b->stream));
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+// ---- BeamLine ----
+int hps_beamline_create(hps_beamline** out)
+{
+  if (!out) return fail("null output handle");
+  *out = new hps_beamline();
+  return 0;
+}
+
+int hps_beamline_destroy(hps_beamline* bl)
+{
+  delete bl;
+  return 0;
+}
+
+int hps_beamline_add(hps_beamline* bl, const char* name, const hps_element* e)
+{
+  if (!bl || !e) return fail("null argument");
+  if (e->kind < HPS_DRIFT || e->kind > HPS_STEERER) return fail("unknown element kind");
+  bl->names.push_back(name ? name : "");
+  bl->el.push_back(*e);
+  return 0;
+}
+
+int hps_beamline_set(hps_beamline* bl, uint32_t index, const hps_element* e)
+{
+  if (!bl || !e || index >= bl->el.size()) return fail("element index out of range");
+  bl->el[index] = *e;
+  return 0;
+}
+
+uint32_t hps_beamline_size(const hps_beamline* bl) { return bl ? (uint32_t)bl->el.size() : 0; }
+
+int hps_beamline_index(const hps_beamline* bl, const char* name, uint32_t* out)
+{
+  if (!bl || !name || !out) return fail("null argument");
+  std::string key(name);
+  std::transform(key.begin(), key.end(), key.begin(), ::tolower);
+  for (size_t i = 0; i < bl->names.size(); ++i) {
+    std::string n = bl->names[i];
+    std::transform(n.begin(), n.end(), n.begin(), ::tolower);
+    if (n == key) { *out = (uint32_t)i; return 0; }
+  }
+  return fail(std::string("BeamLineElementNotFoundException: ") + name);
+}
+
+// ---- SpaceCharge ----
+static int spch_alloc_tables(hps_spch* s)
+{
+  (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins);
+  s->fld1 = nullptr; s->fld2 = nullptr; s->bins = nullptr;
+  const uint32_t nr = s->cfg.nx, nz = s->cfg.nz;
+  const size_t n1 = (size_t)(nr + 1) * nr * nz * 2, n2 = (size_t)(nz + 1) * (nr + 1) * 2;
+  HPS_CHECK(hipMalloc(&s->fld1, n1 * 8));
+  HPS_CHECK(hipMalloc(&s->fld2, n2 * 8));
+  HPS_CHECK(hipMalloc(&s->bins, (size_t)nr * nz * 8));
+  HPS_CHECK(hipMemset(s->fld1, 0, n1 * 8));
+  HPS_CHECK(hipMemset(s->fld2, 0, n2 * 8));
+  HPS_CHECK(hipMemset(s->bins, 0, (size_t)nr * nz * 8));
+  return 0;
+}
+
+static int spch_push_cfg(hps_spch* s)
+{
+  ScDev d;
+  HPS_CHECK(hipMemcpy(&d, s->dev, sizeof d, hipMemcpyDeviceToHost));
+  hps_spch_state& st = d.st;
+  st.nx = s->cfg.nx; st.nz = s->cfg.nz; st.nx_init = s->cfg.nx_init; st.nz_init = s->cfg.nz_init;
+  st.adj_bunch = s->cfg.adj_bunch; st.fraction = s->cfg.fraction; st.interval = s->cfg.interval;
+  st.interval_init = s->cfg.interval_init; st.adj_bunch_w = s->cfg.adj_bunch_w;
+  st.mesh_w = s->cfg.mesh_w; st.remesh_val = s->cfg.remesh_val;
+  HPS_CHECK(hipMemcpy(s->dev, &d, sizeof d, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int hps_spch_create(hps_spch** out, uint32_t nr, uint32_t nz, double interval, uint32_t adj_bunch)
+{
+  if (!out) return fail("null output handle");
+  if (nr == 0 || nz == 0) return fail("SpaceCharge needs nr > 0 and nz > 0");
+  hps_spch* s = new hps_spch();
+  if (hipGetDevice(&s->device) != hipSuccess || stream_for(s->device, &s->stream)) {
+    delete s;
+    return fail("no HIP device");
+  }
+  // SpaceCharge / Scheff constructors, space_charge.cpp:13-40; SetInterval
+  s->cfg.nx = nr; s->cfg.nz = nz; s->cfg.nx_init = nr; s->cfg.nz_init = nz;
+  s->cfg.adj_bunch = adj_bunch; s->cfg.fraction = 1.0;
+  s->cfg.interval = interval; s->cfg.interval_init = interval;
+  s->cfg.remesh_val = 0.05;
+  if (hipMalloc(&s->dev, sizeof(ScDev)) != hipSuccess ||
+      hipMemset(s->dev, 0, sizeof(ScDev)) != hipSuccess || spch_alloc_tables(s) || spch_push_cfg(s)) {
+    (void)hipFree(s->dev);
+    delete s;
+    return -1;
+  }
+  *out = s;
+  return 0;
+}
+
+int hps_spch_destroy(hps_spch* s)
+{
+  if (!s) return 0;
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipFree(s->dev); (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins);
+  delete s;
+  return 0;
+}
+
+int hps_spch_set(hps_spch* s, int what, double v)
+{
+  if (!s) return fail("null space charge");
+  switch (what) {
+    case 0: s->cfg.interval = v; s->cfg.interval_init = v; break;
+    case 1: s->cfg.remesh_val = v; break;
+    case 2: s->cfg.adj_bunch = (uint32_t)v; break;
+    case 3: s->cfg.adj_bunch_w = v; break;
+    case 4: s->cfg.mesh_w = v; break;
+    default: return fail("unknown space-charge setting");
+  }
+  HPS_CHECK(hipStreamSynchronize(s->stream));
+  return spch_push_cfg(s);
+}
+
+int hps_spch_set_mesh_size(hps_spch* s, uint32_t nr, uint32_t nz)   // space_charge.cpp:61-73
+{
+  if (!s || nr == 0 || nz == 0) return fail("bad mesh size");
+  if (nr == s->cfg.nx && nz == s->cfg.nz) return 0;
+  HPS_CHECK(hipStreamSynchronize(s->stream));
+  s->cfg.nx = nr; s->cfg.nz = nz;
+  if (spch_alloc_tables(s)) return -1;
+  return spch_push_cfg(s);
+}
+
+int hps_spch_get_state(hps_spch* s, hps_spch_state* out)
+{
+  if (!s || !out) return fail("null argument");
+  HPS_CHECK(hipStreamSynchronize(s->stream));
+  ScDev d;
+  HPS_CHECK(hipMemcpy(&d, s->dev, sizeof d, hipMemcpyDeviceToHost));
+  *out = d.st;
+  return 0;
+}
+
+int hps_spch_get_tables(hps_spch* s, double* fld1, double* fld2, int64_t* bins)
+{
+  if (!s) return fail("null space charge");
+  HPS_CHECK(hipStreamSynchronize(s->stream));
+  const uint32_t nr = s->cfg.nx, nz = s->cfg.nz;
+  if (fld1) {   // device layout [rp][dz][rs] -> reference layout [rp][rs][dz]
+    std::vector<double> tmp((size_t)(nr + 1) * nr * nz * 2);
+    HPS_CHECK(hipMemcpy(tmp.data(), s->fld1, tmp.size() * 8, hipMemcpyDeviceToHost));
+    for (uint32_t rp = 0; rp <= nr; ++rp)
+      for (uint32_t dz = 0; dz < nz; ++dz)
+        for (uint32_t rs = 0; rs < nr; ++rs) {
+          const size_t src = ((size_t)rp * nz + dz) * nr + rs;
+          const size_t dst = (size_t)rp * nr * nz + (size_t)rs * nz + dz;
+          fld1[2 * dst] = tmp[2 * src];
+          fld1[2 * dst + 1] = tmp[2 * src + 1];
+        }
+  }
+  if (fld2) HPS_CHECK(hipMemcpy(fld2, s->fld2, (size_t)(nz + 1) * (nr + 1) * 16, hipMemcpyDeviceToHost));
+  if (bins) HPS_CHECK(hipMemcpy(bins, s->bins, (size_t)nr * nz * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ---- Simulator ----
+int hps_sim_reset(hps_sim* m)   // Reset, simulation_engine.cu:101-111
+{
+  if (!m) return fail("null simulator");
+  m->ccl_cell_num = 0;
+  m->design_w = m->b->design_w;
+  m->wavelen = CLIGHT / m->b->freq;
+  return 0;
+}
+
+int hps_sim_create(hps_sim** out, hps_beam* b, hps_beamline* bl, hps_spch* s)
+{
+  if (!out || !b || !bl) return fail("Simulator needs a beam and a beamline");
+  hps_sim* m = new hps_sim();
+  m->b = b; m->bl = bl; m->sc = s; m->sc_on = s != nullptr;
+  if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess) {
+    delete m;
+    return fail("event creation failed");
+  }
+  hps_sim_reset(m);
+  m->prev_end = -1;
+  *out = m;
+  return 0;
+}
+
+int hps_sim_destroy(hps_sim* m)
+{
+  if (!m) return 0;
+  (void)hipStreamSynchronize(m->b->stream);
+  for (hipEvent_t e : m->evpool) (void)hipEventDestroy(e);
+  (void)hipFree(m->d_ops);
+  (void)hipEventDestroy(m->ev0);
+  (void)hipEventDestroy(m->ev1);
+  delete m;
+  return 0;
+}
+
+int hps_sim_set_space_charge(hps_sim* m, int on)
+{
+  if (!m) return fail("null simulator");
+  m->sc_on = on != 0;
+  return 0;
+}
+
+int hps_sim_set_profiling(hps_sim* m, int on)
+{
+  if (!m) return fail("null simulator");
+  m->profile = on != 0;
+  for (int k = 0; k < 5; ++k) { m->prof_ms[k] = 0; m->prof_n[k] = 0; }
+  return 0;
+}
+
+int hps_sim_get_profile(hps_sim* m, double* ms, uint32_t* launches)
+{
+  if (!m || !ms || !launches) return fail("null argument");
+  for (int k = 0; k < 5; ++k) { ms[k] = m->prof_ms[k]; launches[k] = m->prof_n[k]; }
+  return 0;
+}
+
+int hps_sim_last_time_ms(hps_sim* m, double* out)
+{
+  if (!m || !out) return fail("null argument");
+  *out = m->last_ms;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// One launch unit of a simulate() call.
+struct Step {
+  int kind;            // 0 segment, 1 frequency change
+  int32_t op_begin, op_end;
+  int32_t kick;        // segment starts with the pending SCHEFF kick
+  double kick_len;
+  int32_t stats;       // StatKind computed by the segment kernel
+  int32_t kick_point;  // Scheff::Start control + rebuild follow the segment
+  double frac;         // current fraction of that kick point
+  double new_freq;     // kind 1
+};
+
+struct Compiler {
+  hps_sim* m;
+  std::vector<Op> ops;
+  std::vector<Step> steps;
+  int32_t seg_begin = 0;
+  bool pending_kick = false;
+  double pending_len = 0;
+  double frac = 1.0;
+  double freq = 0;           // beam frequency as the program runs
+  bool quad_gap_run = true;  // launches with blck/2 threads are empty for N = 1
+  bool sc_stats = true;      // UpdateStatForSpaceCharge runs only for N > 2
+
+  void close(int32_t stats, bool kick_point, double kick_len_next)
+  {
+    Step s{};
+    s.kind = 0;
+    s.op_begin = seg_begin;
+    s.op_end = (int32_t)ops.size();
+    s.kick = pending_kick ? 1 : 0;
+    s.kick_len = pending_len;
+    s.stats = stats;
+    s.kick_point = kick_point ? 1 : 0;
+    s.frac = frac;
+    steps.push_back(s);
+    seg_begin = (int32_t)ops.size();
+    pending_kick = kick_point;
+    pending_len = kick_len_next;
+  }
+
+  // Scheff::Start at this point with kick length len.
+  void kick_point(double len) { close(sc_stats ? STAT_SC : STAT_NONE, true, len); }
+};
+
+Op make_op(int32_t kind, uint32_t elem)
+{
+  Op o{};
+  o.kind = kind;
+  o.elem = elem;
+  return o;
+}
+
+bool is_kind(const hps_beamline* bl, int64_t i, int kind)
+{
+  return i >= 0 && (uint64_t)i < bl->el.size() && bl->el[i].kind == kind;
+}
+
+int compile_element(Compiler& c, uint32_t id)
+{
+  hps_sim* m = c.m;
+  const hps_element& e = m->bl->el[id];
+  const bool sc = m->sc_on;
+  switch (e.kind) {
+    case HPS_DRIFT: {   // SimulateDrift, simulation_engine.cu:125-145
+      int nk = 1;
+      if (sc) {
+        if (!m->sc) return fail("space charge is on but no SpaceCharge object was given");
+        nk = (int)std::ceil(e.length / m->sc->cfg.interval);
+      }
+      const double h = 0.5 * e.length / nk;
+      for (int k = 0; k < nk; ++k) {
+        Op o = make_op(OP_DRIFT, id);
+        o.a[0] = h; o.a[1] = e.aperture; o.a[2] = m->wavelen;
+        c.ops.push_back(o);
+        if (sc) c.kick_point(2.0 * h);
+        c.ops.push_back(o);
+      }
+      return 0;
+    }
+    case HPS_QUAD: {   // SimulateQuad, simulation_engine.cu:147-161
+      Op o = make_op(OP_QUAD, id);
+      o.a[0] = e.length; o.a[1] = e.aperture; o.a[2] = e.gradient; o.a[3] = m->wavelen;
+      if (c.quad_gap_run) c.ops.push_back(o);
+      if (sc) {
+        if (!m->sc) return fail("space charge is on but no SpaceCharge object was given");
+        c.kick_point(e.length);
+      }
+      if (c.quad_gap_run) c.ops.push_back(o);
+      return 0;
+    }
+    case HPS_GAP_DTL:
+    case HPS_GAP_CCL: {
+      const bool ccl = e.kind == HPS_GAP_CCL;
+      // PrepSimulateRFGap, simulation_engine.cu:163-175
+      if (c.freq != e.gap.frequency) {
+        Step s{};
+        s.kind = 1;
+        s.new_freq = e.gap.frequency;
+        c.close(STAT_NONE, false, 0);
+        c.steps.push_back(s);
+        c.freq = e.gap.frequency;
+        m->wavelen = CLIGHT / e.gap.frequency;
+      }
+      const hps_beamline* bl = m->bl;
+      const int64_t i = id;
+      const double ref = e.gap.phase_ref;
+      double phi_in = 0, phi_out = 0, q1 = 0, q2 = 0;
+      uint32_t cell = 0;
+      if (!ccl) {   // SimulateDTLRFGap, simulation_engine.cu:182-241
+        if (i == 1 || (i > 1 && !is_kind(bl, i - 2, HPS_GAP_DTL))) {
+          if (is_kind(bl, i + 2, HPS_GAP_DTL))
+            phi_in = ref - 0.5 * (bl->el[i + 2].gap.phase_ref - ref);
+          else
+            phi_in = ref;   // only gap in tank (also for a final gap)
+        } else if (i > 1) {
+          phi_in = 0.5 * (bl->el[i - 2].gap.phase_ref + ref);
+        } else {
+          return fail("A quad is missing before the DTL gap!");
+        }
+        q1 = is_kind(bl, i - 1, HPS_QUAD) ? bl->el[i - 1].length : 0.0;
+        q2 = is_kind(bl, i + 1, HPS_QUAD) ? bl->el[i + 1].length : 0.0;
+        phi_out = phi_in;
+      } else {      // SimulateCCLRFGap, simulation_engine.cu:243-293
+        if (i == 0 || !is_kind(bl, i - 1, HPS_GAP_CCL)) m->ccl_cell_num = 0;
+        const auto is_gap = [&](int64_t k) { return is_kind(bl, k, HPS_GAP_DTL) || is_kind(bl, k, HPS_GAP_CCL); };
+        if (m->ccl_cell_num == 0) {
+          if (!is_gap(i + 1)) return fail("a CCL tank needs a gap after its first gap");
+          const double next = bl->el[i + 1].gap.phase_ref;
+          phi_in = ref - 0.5 * (next - ref);
+          phi_out = 0.5 * (next + ref);
+        } else if (!is_kind(bl, i + 1, HPS_GAP_CCL)) {
+          const double prev = bl->el[i - 1].gap.phase_ref;
+          phi_in = 0.5 * (ref + prev);
+          phi_out = ref + 0.5 * (ref - prev);
+        } else {
+          phi_in = 0.5 * (ref + bl->el[i - 1].gap.phase_ref);
+          phi_out = 0.5 * (ref + bl->el[i + 1].gap.phase_ref);
+        }
+        cell = m->ccl_cell_num++;
+      }
+      const bool run = ccl || c.quad_gap_run;
+      Op o1 = make_op(OP_GAP1, id);
+      o1.ccl = ccl; o1.g = e.gap;
+      o1.a[0] = m->design_w; o1.a[1] = phi_in; o1.a[2] = e.length; o1.a[3] = q1; o1.a[4] = q2;
+      if (run) {
+        c.ops.push_back(o1);
+        m->wavelen = CLIGHT / e.gap.frequency;   // every thread writes d_wavelen
+      }
+      if (sc) {
+        if (!m->sc) return fail("space charge is on but no SpaceCharge object was given");
+        c.kick_point(e.length);
+      }
+      Op o2 = make_op(OP_GAP2, id);
+      o2.ccl = ccl; o2.ccl_cell_num = cell; o2.g = e.gap;
+      o2.a[0] = m->design_w; o2.a[1] = ccl ? phi_out : phi_in; o2.a[2] = e.length;
+      o2.a[3] = q1; o2.a[4] = q2; o2.a[5] = m->wavelen;
+      if (run) c.ops.push_back(o2);
+      m->design_w = e.gap.energy_out;
+      m->b->design_w = m->design_w;
+      return 0;
+    }
+    case HPS_APERTURE_C:
+    case HPS_APERTURE_R: {   // simulation_engine.cu:307-329
+      if (!e.aperture_in) return 0;
+      c.close(STAT_APERTURE, false, 0);
+      Op o = make_op(e.kind == HPS_APERTURE_C ? OP_APER_C : OP_APER_R, id);
+      if (e.kind == HPS_APERTURE_C) o.a[0] = e.aperture;
+      else { o.a[0] = e.aper_xl; o.a[1] = e.aper_xr; o.a[2] = e.aper_yt; o.a[3] = e.aper_yb; }
+      c.ops.push_back(o);
+      return 0;
+    }
+    case HPS_SPCH_COMP:   // simulation_engine.cu:301-305
+      if (sc && m->sc) { c.frac = e.fraction; m->sc->cfg.fraction = e.fraction; }
+      return 0;
+    case HPS_ROTATION: {
+      Op o = make_op(OP_ROT, id);
+      o.a[0] = e.angle;
+      c.ops.push_back(o);
+      return 0;
+    }
+    case HPS_STEERER: {   // simulation_engine.cu:337-344
+      if (e.bl_h > 1e-10 || e.bl_v > 1e-10) {
+        Op o = make_op(OP_STEER, id);
+        o.a[0] = e.bl_h; o.a[1] = e.bl_v;
+        c.ops.push_back(o);
+      }
+      return 0;
+    }
+    case HPS_DIAGNOSTICS: return 0;
+    default: return fail("unknown element kind");
+  }
+}
+
+}  // namespace
+
+extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
+{
+  if (!m) return fail("null simulator");
+  hps_beam* b = m->b;
+  const uint32_t nel = (uint32_t)m->bl->el.size();
+  if (nel == 0) return 0;
+  const int32_t start_index = start < 0 ? 0 : start;
+  const int32_t end_index = end < 0 ? (int32_t)nel - 1 : end;
+  if (end_index >= (int32_t)nel || start_index >= (int32_t)nel) return fail("element index out of range");
+  if (m->sc_on && m->sc) {
+    if (m->sc->cfg.adj_bunch != 0) return fail("adjacent-bunch space charge (adj_bunch > 0) is not implemented");
+    if (m->sc->cfg.mesh_w != 0.0) return fail("mesh_size_cutoff_w is not implemented");
+  }
+  // SimulationEngine::Simulate, simulation_engine.cpp:73-95
+  if (start_index < m->prev_end) hps_sim_reset(m);
+  m->prev_end = end_index;
+  Compiler c;
+  c.m = m;
+  c.frac = m->sc ? m->sc->cfg.fraction : 1.0;
+  c.freq = b->freq;
+  c.quad_gap_run = b->blck_size / 2 != 0;
+  c.sc_stats = b->n > 2;
+  for (uint32_t i = 0; (int32_t)i <= end_index; ++i)
+    if ((int32_t)i >= start_index || m->bl->el[i].kind == HPS_SPCH_COMP)
+      if (compile_element(c, i)) return -1;
+  c.close(STAT_NONE, false, 0);
+
+  if (c.ops.size() > m->ops_cap) {
+    (void)hipFree(m->d_ops);
+    m->ops_cap = c.ops.size() + 64;
+    HPS_CHECK(hipMalloc(&m->d_ops, m->ops_cap * sizeof(Op)));
+  }
+  hipStream_t s = b->stream;
+  if (!c.ops.empty())
+    HPS_CHECK(hipMemcpyAsync(m->d_ops, c.ops.data(), c.ops.size() * sizeof(Op),
+                             hipMemcpyHostToDevice, s));
+  const uint32_t kstride = b->grid_size * (b->blck_size / 4);
+  const bool kick_ok = kstride != 0;
+  const bool deposit_ok = b->blck_size / 4 != 0;
+  // (family, start event, stop event) of the profiled launches
+  std::vector<std::pair<int, size_t>> marks;
+  size_t nev = 0;
+  auto mark = [&](int fam) -> hipError_t {
+    if (!m->profile) return hipSuccess;
+    while (m->evpool.size() < nev + 2) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      m->evpool.push_back(e);
+    }
+    marks.push_back({fam, nev});
+    hipError_t r = hipEventRecord(m->evpool[nev], s);
+    nev += 2;
+    return r;
+  };
+  auto mark_end = [&]() -> hipError_t {
+    if (!m->profile) return hipSuccess;
+    return hipEventRecord(m->evpool[marks.back().second + 1], s);
+  };
+  HPS_CHECK(hipEventRecord(m->ev0, s));
+  for (const Step& st : c.steps) {
+    if (st.kind == 1) {
+      if (change_frequency(b, st.new_freq)) return -1;
+      continue;
+    }
+    if (st.op_end > st.op_begin || st.kick || st.stats != STAT_NONE) {
+      SegArgs a{};
+      a.b = b->d;
+      a.st = b->st;
+      a.pr = b->pr;
+      a.ops = m->d_ops;
+      a.op_begin = st.op_begin;
+      a.op_end = st.op_end;
+      a.kick = (st.kick && kick_ok && m->sc) ? 1 : 0;
+      a.kick_mask = kstride ? kstride - 1 : 0;
+      a.kick_len = st.kick_len;
+      a.sc = m->sc ? m->sc->dev : nullptr;
+      a.fld2 = m->sc ? m->sc->fld2 : nullptr;
+      a.stats = st.stats;
+      HPS_CHECK(mark(0));
+      HPS_CHECK(launch_segment(a, s));
+      HPS_CHECK(mark_end());
+    }
+    if (st.kick_point) {
+      hps_spch* sc = m->sc;
+      PhaseArgs p{};
+      p.b = b->d; p.st = b->st; p.pr = b->pr; p.sc = sc->dev; p.bins = sc->bins;
+      p.do_sums = b->n > 2 ? 1 : 0;
+      p.frac = st.frac; p.current = b->current; p.freq = b->freq;
+      HPS_CHECK(mark(1));
+      HPS_CHECK(launch_phase(p, s));
+      HPS_CHECK(mark_end());
+      HPS_CHECK(mark(2));
+      HPS_CHECK(launch_greens(sc->dev, sc->fld1, sc->cfg.nx, sc->cfg.nz, b->n, b->mass, s));
+      HPS_CHECK(mark_end());
+      if (deposit_ok) {
+        HPS_CHECK(mark(3));
+        HPS_CHECK(launch_deposit(b->d, b->st, sc->dev, sc->bins, sc->cfg.nx, sc->cfg.nz, s));
+        HPS_CHECK(mark_end());
+      }
+      HPS_CHECK(mark(4));
+      HPS_CHECK(launch_field(sc->dev, sc->bins, sc->fld1, sc->fld2, sc->cfg.nx, sc->cfg.nz, s));
+      HPS_CHECK(mark_end());
+    }
+  }
+  HPS_CHECK(hipEventRecord(m->ev1, s));
+  HPS_CHECK(hipEventSynchronize(m->ev1));
+  float ms = 0;
+  HPS_CHECK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+  m->last_ms = ms;
+  for (const auto& mk : marks) {
+    float t = 0;
+    HPS_CHECK(hipEventElapsedTime(&t, m->evpool[mk.second], m->evpool[mk.second + 1]));
+    m->prof_ms[mk.first] += t;
+    m->prof_n[mk.first] += 1;
+  }
+  HPS_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int hps_probe_math(int fn, const double* x, const double* y, double* out, uint32_t n)
+{
+  if (!x || !out) return fail("null argument");
+  if (n == 0) return 0;
+  int dev = 0;
+  HPS_CHECK(hipGetDevice(&dev));
+  hipStream_t s;
+  if (stream_for(dev, &s)) return -1;
+  double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+  const size_t nb = (size_t)n * 8;
+  HPS_CHECK(hipMalloc(&dx, nb));
+  HPS_CHECK(hipMalloc(&dy, nb));
+  HPS_CHECK(hipMalloc(&dout, nb));
+  HPS_CHECK(hipMemcpy(dx, x, nb, hipMemcpyHostToDevice));
+  if (y) HPS_CHECK(hipMemcpy(dy, y, nb, hipMemcpyHostToDevice));
+  else HPS_CHECK(hipMemset(dy, 0, nb));
+  HPS_CHECK(launch_probe(fn, dx, dy, dout, n, s));
+  HPS_CHECK(hipStreamSynchronize(s));
+  HPS_CHECK(hipMemcpy(out, dout, nb, hipMemcpyDeviceToHost));
+  (void)hipFree(dx); (void)hipFree(dy); (void)hipFree(dout);
+  return 0;
+}

@@ -1,0 +1,99 @@
+// hps_internal.hpp -- device data layout and kernel launch interface shared by
+// the kernels (hps_kernels.hip) and the host engine (hps_engine.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hpsim_amd.h"
+#include "hps_device.hpp"
+
+namespace hps {
+
+// Structure-of-arrays particle state in HBM (reference beam.h:156-174).
+struct BeamArrays {
+  double *x, *xp, *y, *yp, *phi, *phi_r, *w;
+  uint32_t *loss, *lloss;
+  uint32_t n;
+  double mass, charge;
+};
+
+// Device-resident SCHEFF state: the SpaceCharge members the reference keeps
+// on the host (space_charge.h:83-124) plus the values of the last control
+// step (Scheff::Start, space_charge.cpp:82-139).
+struct ScDev {
+  hps_spch_state st;
+  int32_t remesh;       // last control step rebuilt the tables
+  int32_t valid;        // tables hold a field
+  double beta, freq, frac, current;
+  double ratio_r, ratio_z, ratio_q, ratio_g;
+};
+
+// Scratch for the canonical reductions: per-chunk partial sums, padded to a
+// power of two with zeros, plus the grid ticket of the last-block pattern.
+struct Partials {
+  double* sums;        // [kMaxQ][mpad]
+  uint32_t* cnt;       // [mpad]
+  double* pmax;        // [mpad]
+  uint32_t* ticket;    // one counter per kernel family
+  double* out;         // [kMaxQ] final sums for host-side getters
+  uint32_t* out_cnt;   // [1]
+  uint32_t mpad;       // NextPow2(nchunks)
+  uint32_t nch;        // chunks
+};
+constexpr int kMaxQ = 8;
+
+enum StatKind : int32_t { STAT_NONE = 0, STAT_SC = 1, STAT_APERTURE = 2 };
+
+struct SegArgs {
+  BeamArrays b;
+  hps_beam_stats* st;
+  Partials pr;
+  const Op* ops;
+  int32_t op_begin, op_end;
+  int32_t kick;            // apply the pending SCHEFF kick first
+  uint32_t kick_mask;      // particles i with (i & mask) == 0 are not kicked (stride - 1)
+  double kick_len;
+  const ScDev* sc;
+  const double* fld2;      // (nz+1)*(nr+1) double2, [zp][rp]
+  int32_t stats;           // StatKind computed after the maps
+};
+
+struct PhaseArgs {
+  BeamArrays b;
+  hps_beam_stats* st;
+  Partials pr;
+  ScDev* sc;
+  int64_t* bins;
+  int32_t do_sums;         // n > 2 (UpdateStatForSpaceCharge guard)
+  double frac;             // current fraction at this kick point
+  double current;          // beam current (A)
+  double freq;             // beam frequency (MHz)
+};
+
+// Launchers (hps_kernels.hip).
+hipError_t launch_segment(const SegArgs& a, hipStream_t s);
+hipError_t launch_phase(const PhaseArgs& a, hipStream_t s);
+hipError_t launch_greens(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, uint32_t num,
+                         double mass, hipStream_t s);
+hipError_t launch_deposit(BeamArrays b, const hps_beam_stats* st, ScDev* sc, int64_t* bins,
+                          uint32_t nr, uint32_t nz, hipStream_t s);
+hipError_t launch_field(const ScDev* sc, const int64_t* bins, const double* fld1, double* fld2,
+                        uint32_t nr, uint32_t nz, hipStream_t s);
+
+// Generic canonical sums for the getters: over the filtered particles
+// (filter 0: loss == 0, 1: loss + lloss == 0, 2: all) of a, a*a, b, b*b, a*b,
+// and the count of loss != 0, of (loss+lloss) == 0 and of lloss != 0.
+// Results land in pr.out[0..4] and pr.out_cnt[0..2].
+hipError_t launch_sums(BeamArrays b, const double* a, const double* bb, int filter,
+                       Partials pr, hipStream_t s);
+hipError_t launch_relative_phi(BeamArrays b, const double* ref, hipStream_t s);
+hipError_t launch_lloss_flag(BeamArrays b, const double* avg, hipStream_t s);
+hipError_t launch_shift(double* a, uint32_t n, double v, hipStream_t s);
+hipError_t launch_scale(double* a, uint32_t n, double v, hipStream_t s);
+hipError_t launch_cut(const double* a, uint32_t* loss, uint32_t n, double lo, double hi,
+                      hipStream_t s);
+hipError_t launch_probe(int fn, const double* x, const double* y, double* out, uint32_t n,
+                        hipStream_t s);
+
+}  // namespace hps

@@ -1,0 +1,896 @@
+// hps_kernels.hip -- gfx950 kernels of the tracking engine.
+//
+// Kernel families (one HBM sweep each):
+//   k_segment  SCHEFF kick of the previous kick point + every element map up
+//              to the next kick point, fused in registers, + the beam moments
+//              the next kick needs (reference: SimulatePartialDrift/HalfQuad/
+//              RFGap kernels, ApplySpchKickKernel, and the reductions of
+//              Beam::UpdateStatForSpaceCharge up to the x/y moments).
+//   k_phase    relative phase and its moments, then the Scheff::Start control
+//              logic on the device (no host round trip per kick).
+//   k_greens / k_deposit / k_field   the SCHEFF rebuild, run only when the
+//              control step asks for a remesh (flag in device memory).
+// Reductions follow one canonical order (see kChunk): lane sums of 4
+// particles strided by 256, halving tree over 256 lanes, halving tree over
+// the chunk partials zero-padded to a power of two.  The last workgroup of a
+// launch (ticket counter) performs the cross-chunk tree, so each reduction
+// costs one launch.
+#include "hps_internal.hpp"
+
+namespace hps {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// --- canonical reductions -------------------------------------------------
+
+// Halving tree over the 256 lane values of Q quantities stored in red[q*256+t].
+template <int Q>
+__device__ __forceinline__ void lane_tree(double* red, int t)
+{
+#pragma unroll
+  for (int o = kBlock / 2; o >= 1; o >>= 1) {
+    __syncthreads();
+    if (t < o) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) red[q * kBlock + t] = red[q * kBlock + t] + red[q * kBlock + t + o];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t block_count(uint32_t v, uint32_t* ured, int t)
+{
+  ured[t] = v;
+  for (int o = kBlock / 2; o >= 1; o >>= 1) {
+    __syncthreads();
+    if (t < o) ured[t] += ured[t + o];
+  }
+  __syncthreads();
+  uint32_t r = ured[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double* red, int t)
+{
+  red[t] = v;
+  for (int o = kBlock / 2; o >= 1; o >>= 1) {
+    __syncthreads();
+    if (t < o) red[t] = ::fmax(red[t], red[t + o]);
+  }
+  __syncthreads();
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// Halving tree over P[0, mpad) (zero padded), done by one workgroup.  The
+// levels above 256 go through scratch so that P itself is left intact.
+__device__ double chunk_tree(const double* P, double* scratch, uint32_t mpad, double* red, int t)
+{
+  const double* src = P;
+  for (uint32_t o = mpad / 2; o >= (uint32_t)kBlock; o >>= 1) {
+    for (uint32_t c = t; c < o; c += kBlock) scratch[c] = src[c] + src[c + o];
+    __syncthreads();
+    src = scratch;
+  }
+  red[t] = (uint32_t)t < mpad ? src[t] : 0.0;
+  lane_tree<1>(red, t);
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ uint32_t chunk_count(const uint32_t* C, uint32_t nch, uint32_t* ured, int t)
+{
+  uint32_t s = 0;
+  for (uint32_t c = t; c < nch; c += kBlock) s += C[c];
+  return block_count(s, ured, t);
+}
+
+// Returns true in every thread of the last workgroup to finish.
+__device__ __forceinline__ bool last_block(uint32_t* ticket, int t)
+{
+  __shared__ int am_last;
+  __syncthreads();
+  if (t == 0) {
+    __threadfence();
+    const uint32_t prev = atomicAdd(ticket, 1u);
+    am_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  const bool r = am_last != 0;
+  if (r) __threadfence();
+  return r;
+}
+
+__device__ __forceinline__ bool is_good(uint32_t loss, uint32_t lloss)
+{
+  return (uint32_t)(loss + lloss) == 0u;
+}
+
+// --- SCHEFF kick (ApplySpchKickKernel, space_charge_kernel.cu:572-706) -----
+
+struct KickConst {
+  double phimc, xbar, ybar, epsq, gmg, dr, dzg, hl, c2, c3, xfac, yfac;
+  double er_ratio, ez_ratio, cbgcoef, r_beta, phi_ref;
+  uint32_t nx, nz;
+};
+
+__device__ __forceinline__ KickConst kick_const(const ScDev* sc, const hps_beam_stats* st,
+                                                double r_length, double r_mass, uint32_t r_num_part)
+{
+  KickConst k;
+  const double r_beta = sc->beta, r_freq = sc->freq;
+  const double r_current = sc->current * sc->frac;
+  k.phimc = st->phi_avg_r;
+  k.xbar = st->x_avg_good;
+  k.ybar = st->y_avg_good;
+  k.epsq = st->x_sig_good / st->y_sig_good;
+  k.gmg = 1.0 / ::sqrt(1.0 - r_beta * r_beta);
+  k.dr = sc->st.dr * sc->ratio_r;
+  k.dzg = sc->st.dz * k.gmg * sc->ratio_z;
+  const double wave_len = CLIGHT / r_freq;
+  k.nx = sc->st.nx;
+  k.nz = sc->st.nz;
+  k.hl = k.nz * 0.5 * k.dzg;
+  const double c1 = 5721.67 * r_current / (r_freq * 1e6 * (double)r_num_part * r_mass);
+  k.c2 = r_beta * k.gmg * wave_len / TWOPI;
+  k.c3 = r_length / (r_beta * k.gmg);
+  k.xfac = 2.0 / (k.epsq + 1.0);
+  k.yfac = k.epsq * k.xfac;
+  k.er_ratio = sc->ratio_q / (sc->ratio_r * sc->ratio_z * sc->ratio_g * sc->ratio_g);
+  k.ez_ratio = sc->ratio_q / (sc->ratio_z * sc->ratio_z);
+  k.cbgcoef = c1 * k.c3 * PI * 0.5 * sc->st.num_on_mesh;
+  k.r_beta = r_beta;
+  k.phi_ref = st->phi_avg_good;
+  return k;
+}
+
+__device__ __forceinline__ void kick(Particle& p, const KickConst& k, const double* tbl,
+                                     double r_mass)
+{
+  const double phi = relative_phi(p.phi, k.phi_ref);   // phi_r of UpdateRelativePhi(true)
+  const double gm = p.w / r_mass + 1.0;
+  const double bgz = ::sqrt((gm - 1.0) * (gm + 1.0));
+  double bgx = bgz * p.xp;
+  double bgy = bgz * p.yp;
+  double bgzstar = k.gmg * (bgz - k.r_beta * gm);
+  const double xxbar = p.x - k.xbar;
+  const double yybar = p.y - k.ybar;
+  double r = ::sqrt(xxbar * xxbar / k.epsq + yybar * yybar * k.epsq);
+  if (r == 0.0) r = 1.0e-8;
+  const double xovr = xxbar * k.xfac / r;
+  const double yovr = yybar * k.yfac / r;
+  double ddz = phi - k.phimc;
+  for (int guard = 0; (ddz > PI || ddz < -PI) && guard < 4096; ++guard) {
+    if (ddz > 0.0) ddz -= TWOPI;
+    else ddz += TWOPI;
+  }
+  double z = -k.c2 * ddz;
+  double cbgr, cbgzs;
+  if (r > k.nx * k.dr || __builtin_fabs(z) > k.hl) {
+    z = -k.c2 * (phi - k.phimc);
+    const double d = ::sqrt(z * z + r * r);
+    const double invd3 = 1.0 / (d * d * d);
+    const double rod3 = r * invd3;
+    const double zod3 = z * invd3;
+    cbgr = k.cbgcoef * rod3;
+    cbgzs = k.cbgcoef * zod3;
+  } else {
+    const uint32_t i = (uint32_t)(1 + to_int(r / k.dr));
+    const double a = r / k.dr - (i - 1.0);
+    const double b = 1.0 - a;
+    const uint32_t j = (uint32_t)(1 + to_int((z + k.hl) / k.dzg));
+    const double c = (z + k.hl) / k.dzg - (j - 1.0);
+    const double d = 1.0 - c;
+    const uint32_t tbl_sz = (k.nx + 1) * (k.nz + 1);
+    uint32_t l = (j - 1) * (k.nx + 1) + i;
+    uint32_t m = l + (k.nx + 1);
+    if (m >= tbl_sz) m = tbl_sz - 1;
+    if (l >= tbl_sz) l = tbl_sz - 1;
+    cbgr = k.er_ratio * k.c3 * (d * (a * tbl[2 * l] + b * tbl[2 * (l - 1)]) +
+           c * (a * tbl[2 * m] + b * tbl[2 * (m - 1)]));
+    cbgzs = k.ez_ratio * k.c3 * (d * (a * tbl[2 * l + 1] + b * tbl[2 * (l - 1) + 1]) +
+            c * (a * tbl[2 * m + 1] + b * tbl[2 * (m - 1) + 1]));
+  }
+  bgx += cbgr * xovr;
+  bgy += cbgr * yovr;
+  bgzstar += cbgzs;
+  const double gstar = 1.0 + 0.5 * bgzstar * bgzstar;
+  const double bgzf = k.gmg * (bgzstar + k.r_beta * gstar);
+  if (bgzf <= 0.0) {
+    p.loss = 9999;
+  } else {
+    p.xp = bgx / bgzf;
+    p.yp = bgy / bgzf;
+    p.w *= 1.0 + (gm + 1.0) * (bgzf - bgz) / (gm * bgz);
+  }
+}
+
+// --- k_segment ---------------------------------------------------------------
+
+// Per-chunk moments of the SCHEFF statistics (quantity order in partials):
+//   0 phi, 1 x, 2 x^2, 3 y, 4 y^2, 5 w   over (loss + lloss) == 0
+// plus the good count and max(phi) over all particles.
+constexpr int kScQ = 6;
+
+__device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, double* red,
+                               uint32_t* ured, const Partials& pr, const Particle* p, bool fresh)
+{
+  double acc[kScQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  uint32_t cnt = 0;
+  double mx = -__builtin_huge_val();
+#pragma unroll
+  for (int j = 0; j < (int)kPerLane; ++j) {
+    const uint32_t i = c * kChunk + j * kBlock + t;
+    double phi = 0.0, x = 0.0, y = 0.0, w = 0.0;
+    uint32_t loss = 1, lloss = 0;
+    if (i < b.n) {
+      if (fresh) { phi = b.phi[i]; x = b.x[i]; y = b.y[i]; w = b.w[i]; loss = b.loss[i]; lloss = b.lloss[i]; }
+      else { phi = p[j].phi; x = p[j].x; y = p[j].y; w = p[j].w; loss = p[j].loss; lloss = p[j].lloss; }
+      mx = ::fmax(mx, phi);
+    }
+    const bool g = (i < b.n) && is_good(loss, lloss);
+    const double vphi = g ? phi : 0.0, vx = g ? x : 0.0, vy = g ? y : 0.0, vw = g ? w : 0.0;
+    acc[0] = acc[0] + vphi;
+    acc[1] = acc[1] + vx;
+    acc[2] = acc[2] + vx * vx;
+    acc[3] = acc[3] + vy;
+    acc[4] = acc[4] + vy * vy;
+    acc[5] = acc[5] + vw;
+    cnt += g ? 1u : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < kScQ; ++q) red[q * kBlock + t] = acc[q];
+  lane_tree<kScQ>(red, t);
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = red[q * kBlock];
+  }
+  __syncthreads();
+  const uint32_t cs = block_count(cnt, ured, t);
+  const double ms = block_max(mx, red, t);
+  if (t == 0) { pr.cnt[c] = cs; pr.pmax[c] = ms; }
+}
+
+// Aperture statistics (UpdateLoss + UpdateAvgXY, simulation_engine.cu:311-312):
+// count of loss != 0, sums of x and y over loss == 0.
+__device__ void aper_chunk_stats(const BeamArrays& b, uint32_t c, int t, double* red,
+                                 uint32_t* ured, const Partials& pr, const Particle* p)
+{
+  double sx = 0.0, sy = 0.0;
+  uint32_t nl = 0;
+#pragma unroll
+  for (int j = 0; j < (int)kPerLane; ++j) {
+    const uint32_t i = c * kChunk + j * kBlock + t;
+    const bool v = i < b.n;
+    const bool ok = v && p[j].loss == 0;
+    sx = sx + (ok ? p[j].x : 0.0);
+    sy = sy + (ok ? p[j].y : 0.0);
+    nl += (v && p[j].loss != 0) ? 1u : 0u;
+  }
+  red[t] = sx;
+  red[kBlock + t] = sy;
+  lane_tree<2>(red, t);
+  if (t == 0) { pr.sums[c] = red[0]; pr.sums[pr.mpad + c] = red[kBlock]; }
+  __syncthreads();
+  const uint32_t cs = block_count(nl, ured, t);
+  if (t == 0) pr.cnt[c] = cs;
+}
+
+__global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
+{
+  __shared__ double red[kScQ * kBlock];
+  __shared__ uint32_t ured[kBlock];
+  __shared__ uint32_t rescan_list[64];
+  __shared__ uint32_t rescan_n;
+  const int t = threadIdx.x;
+  const uint32_t c = blockIdx.x;
+  const BeamArrays& b = a.b;
+  Particle p[kPerLane];
+  bool alive_in[kPerLane];
+#pragma unroll
+  for (int j = 0; j < (int)kPerLane; ++j) {
+    const uint32_t i = c * kChunk + j * kBlock + t;
+    alive_in[j] = false;
+    if (i < b.n) {
+      p[j].loss = b.loss[i];
+      p[j].lloss = b.lloss[i];
+      alive_in[j] = p[j].loss == 0;
+      p[j].x = b.x[i]; p[j].xp = b.xp[i]; p[j].y = b.y[i]; p[j].yp = b.yp[i];
+      p[j].phi = b.phi[i]; p[j].w = b.w[i];
+    } else {
+      p[j].loss = 1; p[j].lloss = 0;
+      p[j].x = p[j].xp = p[j].y = p[j].yp = p[j].phi = p[j].w = 0.0;
+    }
+  }
+  if (a.kick) {
+    const KickConst k = kick_const(a.sc, a.st, a.kick_len, b.mass, b.n);
+#pragma unroll
+    for (int j = 0; j < (int)kPerLane; ++j) {
+      const uint32_t i = c * kChunk + j * kBlock + t;
+      if (alive_in[j] && p[j].lloss == 0 && (i & a.kick_mask) != 0) kick(p[j], k, a.fld2, b.mass);
+    }
+  }
+  const double xa = a.st->x_avg, ya = a.st->y_avg;
+  for (int o = a.op_begin; o < a.op_end; ++o) {
+    const Op& op = a.ops[o];
+#pragma unroll
+    for (int j = 0; j < (int)kPerLane; ++j) {
+      const uint32_t i = c * kChunk + j * kBlock + t;
+      if (p[j].loss == 0) apply_op(op, p[j], i, b.mass, b.charge, xa, ya);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < (int)kPerLane; ++j) {
+    const uint32_t i = c * kChunk + j * kBlock + t;
+    if (alive_in[j]) {
+      b.x[i] = p[j].x; b.xp[i] = p[j].xp; b.y[i] = p[j].y; b.yp[i] = p[j].yp;
+      b.phi[i] = p[j].phi; b.w[i] = p[j].w;
+      if (p[j].loss != 0) b.loss[i] = p[j].loss;
+    }
+  }
+  if (a.stats == STAT_NONE) return;
+
+  const Partials& pr = a.pr;
+  double* scratch = pr.sums + kMaxQ * pr.mpad;
+  if (a.stats == STAT_APERTURE) {
+    aper_chunk_stats(b, c, t, red, ured, pr, p);
+    if (!last_block(pr.ticket, t)) return;
+    const uint32_t nloss = chunk_count(pr.cnt, pr.nch, ured, t);
+    const double sx = chunk_tree(pr.sums, scratch, pr.mpad, red, t);
+    const double sy = chunk_tree(pr.sums + pr.mpad, scratch, pr.mpad, red, t);
+    if (t == 0) {
+      hps_beam_stats* st = a.st;
+      st->num_loss = nloss;
+      const double surv = (double)(uint32_t)(b.n - nloss);
+      st->x_avg = sx / surv;
+      st->y_avg = sy / surv;
+      *pr.ticket = 0;
+    }
+    return;
+  }
+
+  // STAT_SC: moments after the maps, then (last block) the longitudinal-loss
+  // update of Beam::UpdateLongitudinalLoss (beam.cpp:776-782) and the good
+  // moments of beam.cpp:1008-1014.
+  sc_chunk_stats(b, c, t, red, ured, pr, p, false);
+  if (!last_block(pr.ticket, t)) return;
+  hps_beam_stats* st = a.st;
+  uint32_t g1 = chunk_count(pr.cnt, pr.nch, ured, t);
+  const double sphi1 = chunk_tree(pr.sums, scratch, pr.mpad, red, t);
+  const double avg1 = g1 > 0 ? sphi1 / (double)g1 : -99999;
+  const double thr = avg1 + TWOPI + PI;
+  // Chunks holding a particle beyond the threshold get their lloss flags set
+  // and their partials recomputed (usually there are none).  A rescanned
+  // chunk's max is cleared so that the scan terminates.
+  for (uint32_t it = 0; it < pr.nch / 64 + 2; ++it) {
+    if (t == 0) rescan_n = 0;
+    __syncthreads();
+    for (uint32_t cc = t; cc < pr.nch; cc += kBlock) {
+      if (pr.pmax[cc] > thr) {
+        const uint32_t slot = atomicAdd(&rescan_n, 1u);
+        if (slot < 64) rescan_list[slot] = cc;
+      }
+    }
+    __syncthreads();
+    const uint32_t nfound = rescan_n;
+    if (nfound == 0) break;
+    const uint32_t take = nfound < 64 ? nfound : 64;
+    for (uint32_t r = 0; r < take; ++r) {
+      const uint32_t cc = rescan_list[r];
+#pragma unroll
+      for (int j = 0; j < (int)kPerLane; ++j) {
+        const uint32_t i = cc * kChunk + j * kBlock + t;
+        if (i < b.n && b.phi[i] > thr) b.lloss[i] = 1;
+      }
+      __syncthreads();
+      sc_chunk_stats(b, cc, t, red, ured, pr, p, true);
+      if (t == 0) pr.pmax[cc] = -__builtin_huge_val();
+      __syncthreads();
+    }
+  }
+  const uint32_t g2 = chunk_count(pr.cnt, pr.nch, ured, t);
+  double s[kScQ];
+#pragma unroll
+  for (int q = 0; q < kScQ; ++q) s[q] = chunk_tree(pr.sums + q * pr.mpad, scratch, pr.mpad, red, t);
+  if (t == 0) {
+    st->num_good = g2;
+    st->phi_avg_good = g2 > 0 ? s[0] / (double)g2 : -99999;
+    const double surv = (double)(uint32_t)(b.n - st->num_loss);
+    const double xavg = s[1] / surv, x2avg = s[2] / surv, yavg = s[3] / surv, y2avg = s[4] / surv;
+    st->x_avg_good = xavg;
+    st->x_sig_good = ::sqrt(x2avg - xavg * xavg);
+    st->y_avg_good = yavg;
+    st->y_sig_good = ::sqrt(y2avg - yavg * yavg);
+    st->w_avg_good = g2 > 0 ? s[5] / (double)g2 : -99999;
+    *pr.ticket = 0;
+  }
+}
+
+// --- k_phase -------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
+{
+  __shared__ double red[3 * kBlock];
+  const int t = threadIdx.x;
+  const uint32_t c = blockIdx.x;
+  const BeamArrays& b = a.b;
+  const Partials& pr = a.pr;
+  hps_beam_stats* st = a.st;
+  if (a.do_sums) {
+    const double ref = st->phi_avg_good;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < (int)kPerLane; ++j) {
+      const uint32_t i = c * kChunk + j * kBlock + t;
+      double pr_ = 0.0;
+      uint32_t loss = 1, lloss = 0;
+      if (i < b.n) { pr_ = relative_phi(b.phi[i], ref); loss = b.loss[i]; lloss = b.lloss[i]; }
+      const bool tg = (i < b.n) && loss == 0;
+      const bool g = (i < b.n) && is_good(loss, lloss);
+      s0 = s0 + (tg ? pr_ : 0.0);
+      const double vg = g ? pr_ : 0.0;
+      s1 = s1 + vg;
+      s2 = s2 + vg * vg;
+    }
+    red[t] = s0; red[kBlock + t] = s1; red[2 * kBlock + t] = s2;
+    lane_tree<3>(red, t);
+    if (t == 0) {
+      pr.sums[c] = red[0];
+      pr.sums[pr.mpad + c] = red[kBlock];
+      pr.sums[2 * pr.mpad + c] = red[2 * kBlock];
+    }
+    if (!last_block(pr.ticket, t)) return;
+    double* scratch = pr.sums + kMaxQ * pr.mpad;
+    const double sr = chunk_tree(pr.sums, scratch, pr.mpad, red, t);
+    const double sg = chunk_tree(pr.sums + pr.mpad, scratch, pr.mpad, red, t);
+    const double sg2 = chunk_tree(pr.sums + 2 * pr.mpad, scratch, pr.mpad, red, t);
+    if (t == 0) {
+      st->phi_avg_r = sr / (double)(uint32_t)(b.n - st->num_loss);
+      const double gd = (double)st->num_good;
+      const double avg = sg / gd, avg2 = sg2 / gd;
+      st->phi_sig_good = ::sqrt(avg2 - avg * avg);
+      *pr.ticket = 0;
+    }
+  }
+  // Scheff::Start control logic, space_charge.cpp:84-138 (adj_bunch = 0,
+  // mesh_size_cutoff_w = 0).
+  __shared__ int remesh;
+  if (t == 0) {
+    ScDev* sc = a.sc;
+    const uint32_t good_cnt = st->num_good;
+    const double sigphi = b.n < 2 ? 0.0 : st->phi_sig_good;
+    const double sigx = b.n < 2 ? 0.0 : st->x_sig_good;
+    const double sigy = b.n < 2 ? 0.0 : st->y_sig_good;
+    const double ke = st->w_avg_good;
+    const double gamma = (double)ke / b.mass + 1.0;
+    const double beta = ::sqrt(1.0 - 1.0 / (gamma * gamma));
+    const double lmbd = 299.792458 / a.freq;
+    const double sigz = sigphi * 0.5 / 3.14159265 * beta * lmbd;
+    const double sigr = ::sqrt(sigx * sigx + sigy * sigy);
+    const double cratior = sigr * sigz;
+    const double cratioz = sigz * sigz * gamma * gamma;
+    hps_spch_state& s = sc->st;
+    const double cratior_old = s.sigr_old * s.sigz_old;
+    const double cratioz_old = s.sigz_old * s.sigz_old * s.gm_old * s.gm_old;
+    sc->freq = a.freq;
+    sc->beta = beta;
+    sc->frac = a.frac;
+    sc->current = a.current;
+    s.kick_count += 1;
+    if (cratior_old == 0.0 || cratioz_old == 0.0 ||
+        __builtin_fabs(cratior - cratior_old) / cratior_old > s.remesh_val ||
+        __builtin_fabs(cratioz - cratioz_old) / cratioz_old > s.remesh_val) {
+      s.sigr_old = sigr;
+      s.sigz_old = sigz;
+      s.gm_old = gamma;
+      s.good_cnt_old = good_cnt;
+      s.remesh_count += 1;
+      // UpdateScheffMeshSizeKernel via UpdateMeshKernelCall: the reference
+      // compares the device pointers x_sig_good > y_sig_good (space_charge.cu:131);
+      // x_sig_good is allocated first (beam.cu:51,54), so sigma_y is used.
+      const double bl = beta * CLIGHT / a.freq;
+      s.dr = 3.0 * st->y_sig_good / s.nx;
+      s.dz = 3.0 * bl * st->phi_sig_good / (s.nz * PI);
+      s.num_on_mesh = 0;
+      sc->ratio_r = sc->ratio_z = sc->ratio_q = sc->ratio_g = 1.0;
+      sc->remesh = 1;
+      sc->valid = 1;
+    } else {
+      sc->ratio_r = sigr / s.sigr_old;
+      sc->ratio_z = sigz / s.sigz_old;
+      sc->ratio_q = s.good_cnt_old ? (double)(good_cnt / s.good_cnt_old) : 0.0;
+      sc->ratio_g = gamma / s.gm_old;
+      sc->remesh = 0;
+    }
+    remesh = sc->remesh;
+  }
+  __syncthreads();
+  if (remesh) {
+    const uint32_t nb = a.sc->st.nx * a.sc->st.nz;
+    for (uint32_t i = t; i < nb; i += kBlock) a.bins[i] = 0;
+  }
+}
+
+// --- SCHEFF rebuild ----------------------------------------------------------------
+
+// Ring Green's table, UpdateFldTbl1Kernel (space_charge_kernel.cu:38-100),
+// stored [rp][dz][rs] so that the field sum reads consecutive rs.
+__global__ __launch_bounds__(kBlock) void k_greens(const ScDev* sc, double* fld1, uint32_t nr,
+                                                   uint32_t nz, uint32_t r_num_part, double mass)
+{
+  if (!sc->remesh) return;
+  const uint32_t total = (nr + 1) * nz * nr;
+  const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= total) return;
+  const uint32_t k = idx % nr;            // source ring index rs
+  const uint32_t t = (idx / nr) % nz;     // dz
+  const uint32_t rpi = idx / (nr * nz);
+  const double r_beta = sc->beta, r_freq = sc->freq;
+  const double r_current_ov_mc2 = sc->current * sc->frac / mass;
+  const double gm = 1.0 / ::sqrt(1.0 - r_beta * r_beta);
+  const double dr = sc->st.dr;
+  const double dzg = sc->st.dz * gm;
+  const double rs = ::sqrt(0.5 * (k * k + (k + 1) * (k + 1))) * dr;
+  const double zs = 0.5 * dzg;
+  const double zp = (t + 1) * dzg;
+  const double rp = rpi * dr;
+  const double d = zp - zs;
+  const double d2 = d * d;
+  const double c = (rp - rs) * (rp - rs);
+  const double b = (rp + rs) * (rp + rs);
+  const double rsrp = rs * rp;
+  double a = 4.0 * rsrp / (b + d2);
+  const double g = 1.0 - a;
+  const double h = hps::log(g);
+  const double ee = 1.0 + g * (0.4630106 - 0.2452740 * h + g * (0.1077857 - 0.04125321 * h));
+  const double ek = 1.38629436 - 0.5 * h + g * (0.1119697 - 0.1213486 * h + g *
+                    (0.07253230 - 0.02887472 * h));
+  double er1 = 0.0;
+  a = ::sqrt(b + d2);
+  const double rs2mrp2 = (rs * rs - rp * rp);
+  if (rp != 0.0) er1 = (ek - (rs2mrp2 + d2) * ee / (c + d2)) / (2.0 * rp * a);
+  const double ez1 = d * ee / (a * (c + d2));
+  const double c1 = 5721.67 * r_current_ov_mc2 / (r_freq * 1e6 * (double)r_num_part);
+  fld1[2 * (size_t)idx] = c1 * er1;
+  fld1[2 * (size_t)idx + 1] = c1 * ez1;
+}
+
+// Charge deposit, DistributeParticleKernel (space_charge_kernel.cu:122-224),
+// into an LDS-private int64 fixed-point histogram merged with integer atomics.
+__global__ __launch_bounds__(kBlock) void k_deposit(BeamArrays b, const hps_beam_stats* st,
+                                                    ScDev* sc, int64_t* bins)
+{
+  extern __shared__ unsigned long long hist[];
+  __shared__ uint32_t ured[kBlock];
+  if (!sc->remesh) return;
+  const int t = threadIdx.x;
+  const uint32_t nx = sc->st.nx, nz = sc->st.nz, nb = nx * nz;
+  for (uint32_t i = t; i < nb; i += kBlock) hist[i] = 0ull;
+  __syncthreads();
+  const double r_beta = sc->beta, r_freq = sc->freq;
+  const double phimc = st->phi_avg_r;
+  const double xbar = st->x_avg_good, ybar = st->y_avg_good;
+  const double epsq = st->x_sig_good / st->y_sig_good;
+  const double ref = st->phi_avg_good;
+  const double gm = 1.0 / ::sqrt(1.0 - r_beta * r_beta);
+  const double dr = sc->st.dr;
+  const double dzg = sc->st.dz * gm;
+  const double wave_len = CLIGHT / r_freq;
+  const double hl = nz * 0.5 * dzg;
+  const double c2 = r_beta * gm * wave_len / TWOPI;
+  const double drdr = dr * dr;
+  uint32_t count = 0;
+  for (uint32_t np = blockIdx.x * kBlock + t; np < b.n; np += gridDim.x * kBlock) {
+    if (!(b.loss[np] == 0 && b.lloss[np] == 0)) continue;
+    const double x = b.x[np], y = b.y[np];
+    double z = relative_phi(b.phi[np], ref);
+    const double rsq_sh = (x - xbar) * (x - xbar) / epsq + (y - ybar) * (y - ybar) * epsq;
+    const int i = to_int(::sqrt(rsq_sh) / dr + 1);
+    if (i < 1 || i > (int)nx) continue;
+    double ddz = z - phimc;
+    for (int guard = 0; (ddz > PI || ddz < -PI) && guard < 4096; ++guard) {
+      if (ddz > 0.0) ddz -= TWOPI;
+      else ddz += TWOPI;
+    }
+    z = -c2 * ddz;
+    if (!(__builtin_fabs(z) < hl)) continue;
+    const double zz = z + hl;
+    const int jm1 = to_int(zz / dzg + 1);
+    int i1 = i + 1;
+    if (rsq_sh < 0.5 * drdr * (double)(i * i + (i - 1) * (i - 1))) i1 = i - 1;
+    if (i1 < 1) i1 = 1;
+    if (i1 > (int)nx) i1 = (int)nx;
+    int j1 = jm1 + 1;
+    if (zz < (jm1 - 1 + 0.5) * dzg) j1 = jm1 - 1;
+    if (j1 < 1) j1 = 1;
+    if (j1 > (int)nz) j1 = (int)nz;
+    double a;
+    if (i1 == i) {
+      a = 1.0;
+    } else {
+      const double sqrtmp = ::sqrt(4.0 * rsq_sh / drdr - 1.0);
+      const double rminsq = 0.25 * drdr * (sqrtmp - 1.0) * (sqrtmp - 1.0);
+      const double rmaxsq = rminsq + sqrtmp * drdr;
+      if (i1 < i) a = (rmaxsq - (i - 1) * (i - 1) * drdr) / (rmaxsq - rminsq);
+      else a = ((i1 - 1) * (i1 - 1) * drdr - rminsq) / (rmaxsq - rminsq);
+    }
+    double cc = 1.0;
+    if (j1 != jm1) cc = (zz - (j1 - 1 + 0.5) * dzg) / ((jm1 - j1) * dzg);
+    const double bb = 1.0 - a;
+    const double d = 1.0 - cc;
+    if (jm1 < 1 || jm1 > (int)nz) continue;
+    int k = (jm1 - 1) * (int)nx + i;
+    atomicAdd(&hist[k - 1], (unsigned long long)(int64_t)(a * cc * 0x1p36));
+    k += i1 - i;
+    atomicAdd(&hist[k - 1], (unsigned long long)(int64_t)(bb * cc * 0x1p36));
+    k = (j1 - 1) * (int)nx + i;
+    atomicAdd(&hist[k - 1], (unsigned long long)(int64_t)(a * d * 0x1p36));
+    k += i1 - i;
+    atomicAdd(&hist[k - 1], (unsigned long long)(int64_t)(bb * d * 0x1p36));
+    ++count;
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < nb; i += kBlock) {
+    const unsigned long long v = hist[i];
+    if (v != 0ull) atomicAdd((unsigned long long*)&bins[i], v);
+  }
+  const uint32_t cs = block_count(count, ured, t);
+  if (t == 0 && cs) atomicAdd(&sc->st.num_on_mesh, cs);
+}
+
+// Field table E(zp, rp) = sum over bins of charge * Green's value,
+// UpdateFldTbl2Kernel (space_charge_kernel.cu:230-342).  One wavefront per
+// field point: lane l sums bins l, l+64, ... with fma, then a halving tree.
+__global__ __launch_bounds__(kBlock) void k_field(const ScDev* sc, const int64_t* bins,
+                                                  const double* fld1, double* fld2, uint32_t nr,
+                                                  uint32_t nz)
+{
+  if (!sc->remesh) return;
+  const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const uint32_t nout = (nz + 1) * (nr + 1);
+  if (wave >= nout) return;
+  const uint32_t zp = wave / (nr + 1), rp = wave % (nr + 1);
+  const uint32_t fld_sz = nr * nz;
+  const uint32_t dq = 64 / nr, dm = 64 % nr;
+  uint32_t zs = (uint32_t)lane / nr, rs = (uint32_t)lane % nr;
+  double tx = 0.0, ty = 0.0;
+  const double* g = fld1 + 2 * (size_t)rp * nz * nr;
+  for (uint32_t bi = lane; bi < fld_sz; bi += 64) {
+    const double bin = (double)bins[bi] * 0x1p-36;
+    double sign = 1.0;
+    uint32_t dz;
+    if (zs >= zp) { dz = zs - zp; sign = -1.0; }
+    else dz = zp - zs - 1;
+    const size_t fi = (size_t)dz * nr + rs;
+    tx = ::fma(bin, g[2 * fi], tx);
+    ty = ::fma(sign * bin, g[2 * fi + 1], ty);
+    rs += dm; zs += dq;
+    if (rs >= nr) { rs -= nr; zs += 1; }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double ox = __shfl_down(tx, o, 64);
+    const double oy = __shfl_down(ty, o, 64);
+    tx = tx + ox;
+    ty = ty + oy;
+  }
+  if (lane == 0) {
+    fld2[2 * (size_t)wave] = tx;
+    fld2[2 * (size_t)wave + 1] = ty;
+  }
+}
+
+// --- getter reductions ---------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_sums(BeamArrays b, const double* pa, const double* pb,
+                                                 int filter, Partials pr)
+{
+  __shared__ double red[5 * kBlock];
+  __shared__ uint32_t ured[kBlock];
+  const int t = threadIdx.x;
+  const uint32_t c = blockIdx.x;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  uint32_t nloss = 0, ngood = 0, nlloss = 0;
+#pragma unroll
+  for (int j = 0; j < (int)kPerLane; ++j) {
+    const uint32_t i = c * kChunk + j * kBlock + t;
+    if (i >= b.n) continue;
+    const uint32_t loss = b.loss[i], lloss = b.lloss[i];
+    nloss += loss != 0;
+    nlloss += lloss != 0;
+    ngood += is_good(loss, lloss);
+    const bool inc = filter == 0 ? loss == 0 : (filter == 1 ? is_good(loss, lloss) : true);
+    const double va = (inc && pa) ? pa[i] : 0.0;
+    const double vb = (inc && pb) ? pb[i] : 0.0;
+    acc[0] = acc[0] + va;
+    acc[1] = acc[1] + va * va;
+    acc[2] = acc[2] + vb;
+    acc[3] = acc[3] + vb * vb;
+    acc[4] = acc[4] + va * vb;
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) red[q * kBlock + t] = acc[q];
+  lane_tree<5>(red, t);
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) pr.sums[q * pr.mpad + c] = red[q * kBlock];
+  }
+  __syncthreads();
+  const uint32_t c0 = block_count(nloss, ured, t);
+  const uint32_t c1 = block_count(ngood, ured, t);
+  const uint32_t c2 = block_count(nlloss, ured, t);
+  if (t == 0) { pr.cnt[c] = c0; pr.cnt[pr.mpad + c] = c1; pr.cnt[2 * pr.mpad + c] = c2; }
+  if (!last_block(pr.ticket, t)) return;
+  double* scratch = pr.sums + kMaxQ * pr.mpad;
+  for (int q = 0; q < 5; ++q) {
+    const double s = chunk_tree(pr.sums + q * pr.mpad, scratch, pr.mpad, red, t);
+    if (t == 0) pr.out[q] = s;
+  }
+  for (int q = 0; q < 3; ++q) {
+    const uint32_t s = chunk_count(pr.cnt + q * pr.mpad, pr.nch, ured, t);
+    if (t == 0) pr.out_cnt[q] = s;
+  }
+  if (t == 0) *pr.ticket = 0;
+}
+
+__global__ void k_relative_phi(BeamArrays b, const double* ref)
+{
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < b.n) b.phi_r[i] = relative_phi(b.phi[i], ref[0]);
+}
+
+// UpdateLongitudinalLossCoordinateKernel, beam_kernel.cu:1549-1559
+__global__ void k_lloss_flag(BeamArrays b, const double* avg)
+{
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < b.n && b.phi[i] > avg[0] + TWOPI + PI) b.lloss[i] = 1;
+}
+
+// ShiftVariableKernel, beam_kernel.cu:1489-1496 (particle 0 untouched)
+__global__ void k_shift(double* a, uint32_t n, double v)
+{
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i > 0 && i < n) a[i] += v;
+}
+
+// ChangeFrequencyKernel, beam_kernel.cu:1531-1537
+__global__ void k_scale(double* a, uint32_t n, double v)
+{
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) a[i] *= v;
+}
+
+// CutBeamKernel, beam_kernel.cu:1464-1480
+__global__ void k_cut(const double* a, uint32_t* loss, uint32_t n, double lo, double hi)
+{
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i > 0 && i < n && loss[i] == 0) {
+    const double x = a[i];
+    if (x < hi && x > lo) loss[i] = 99999;
+  }
+}
+
+__global__ void k_probe(int fn, const double* x, const double* y, double* out, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double v = x[i];
+  double r = 0.0, s, c;
+  switch (fn) {
+    case 0: r = hps::exp(v); break;
+    case 1: r = hps::log(v); break;
+    case 2: hps::sincos(v, &s, &c); r = s; break;
+    case 3: hps::sincos(v, &s, &c); r = c; break;
+    case 4: r = hps::sinh(v); break;
+    case 5: r = hps::cosh(v); break;
+    case 6: r = ::sqrt(v); break;
+    case 7: r = hps::fmod_pos(v, TWOPI); break;
+    case 8: r = v / y[i]; break;
+    case 9: r = bessi0c(v); break;
+    case 10: r = bessi1c(v); break;
+    case 11: r = bessi1p(v); break;
+    default: break;
+  }
+  out[i] = r;
+}
+
+inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+hipError_t launch_segment(const SegArgs& a, hipStream_t s)
+{
+  const uint32_t nch = (a.b.n + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(k_segment, dim3(nch), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_phase(const PhaseArgs& a, hipStream_t s)
+{
+  const uint32_t nch = a.do_sums ? (a.b.n + kChunk - 1) / kChunk : 1;
+  hipLaunchKernelGGL(k_phase, dim3(nch), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_greens(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, uint32_t num,
+                         double mass, hipStream_t s)
+{
+  const uint32_t total = (nr + 1) * nz * nr;
+  hipLaunchKernelGGL(k_greens, dim3(blocks_for(total)), dim3(kBlock), 0, s, sc, fld1, nr, nz,
+                     num, mass);
+  return hipGetLastError();
+}
+
+hipError_t launch_deposit(BeamArrays b, const hps_beam_stats* st, ScDev* sc, int64_t* bins,
+                          uint32_t nr, uint32_t nz, hipStream_t s)
+{
+  const size_t lds = (size_t)nr * nz * sizeof(unsigned long long);
+  uint32_t grid = (b.n + 2047) / 2048;
+  if (grid < 1) grid = 1;
+  if (grid > 1024) grid = 1024;
+  hipLaunchKernelGGL(k_deposit, dim3(grid), dim3(kBlock), lds, s, b, st, sc, bins);
+  return hipGetLastError();
+}
+
+hipError_t launch_field(const ScDev* sc, const int64_t* bins, const double* fld1, double* fld2,
+                        uint32_t nr, uint32_t nz, hipStream_t s)
+{
+  const uint32_t nout = (nz + 1) * (nr + 1);
+  const uint32_t grid = (nout + 3) / 4;
+  hipLaunchKernelGGL(k_field, dim3(grid), dim3(kBlock), 0, s, sc, bins, fld1, fld2, nr, nz);
+  return hipGetLastError();
+}
+
+hipError_t launch_sums(BeamArrays b, const double* a, const double* bb, int filter, Partials pr,
+                       hipStream_t s)
+{
+  const uint32_t nch = (b.n + kChunk - 1) / kChunk;
+  hipLaunchKernelGGL(k_sums, dim3(nch), dim3(kBlock), 0, s, b, a, bb, filter, pr);
+  return hipGetLastError();
+}
+
+hipError_t launch_relative_phi(BeamArrays b, const double* ref, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_relative_phi, dim3(blocks_for(b.n)), dim3(kBlock), 0, s, b, ref);
+  return hipGetLastError();
+}
+
+hipError_t launch_lloss_flag(BeamArrays b, const double* avg, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_lloss_flag, dim3(blocks_for(b.n)), dim3(kBlock), 0, s, b, avg);
+  return hipGetLastError();
+}
+
+hipError_t launch_shift(double* a, uint32_t n, double v, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_shift, dim3(blocks_for(n)), dim3(kBlock), 0, s, a, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale(double* a, uint32_t n, double v, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_scale, dim3(blocks_for(n)), dim3(kBlock), 0, s, a, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_cut(const double* a, uint32_t* loss, uint32_t n, double lo, double hi,
+                      hipStream_t s)
+{
+  hipLaunchKernelGGL(k_cut, dim3(blocks_for(n)), dim3(kBlock), 0, s, a, loss, n, lo, hi);
+  return hipGetLastError();
+}
+
+hipError_t launch_probe(int fn, const double* x, const double* y, double* out, uint32_t n,
+                        hipStream_t s)
+{
+  hipLaunchKernelGGL(k_probe, dim3(blocks_for(n)), dim3(kBlock), 0, s, fn, x, y, out, n);
+  return hipGetLastError();
+}
+
+}  // namespace hps

@@ -1,0 +1,199 @@
+/*
+ * hpsim_amd.h -- C ABI of the MI355X HPSim tracking engine.
+ *
+ * The reference exposes its engine to Python through a hand-written CPython
+ * extension module "HPSim" (src/pywrapper/src/hpsim_module.cpp:398-416) whose
+ * methods call C++ objects Beam, BeamLine, SpaceCharge and SimulationEngine.
+ * This header is the language-neutral boundary that replaces those calls:
+ * plain handles, pointers and sizes, an int status on every call (0 = ok,
+ * negative = error, message from hps_last_error()).  hpsim_amd/HPSim.py binds
+ * it with ctypes and keeps the reference's Python types and method names.
+ *
+ * Units are the engine units of the reference: m, rad, MeV, MHz, A.
+ * All arrays are host pointers; the library owns device memory.
+ */
+#ifndef HPSIM_AMD_H
+#define HPSIM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hps_beam hps_beam;
+typedef struct hps_beamline hps_beamline;
+typedef struct hps_spch hps_spch;
+typedef struct hps_sim hps_sim;
+
+/* Element kinds: the reference element type strings
+ * (src/beamline/inc/beamline_element.h:46-263). */
+enum hps_element_kind {
+  HPS_DRIFT = 1,        /* "Drift"       */
+  HPS_QUAD = 2,         /* "Quad"        */
+  HPS_GAP_DTL = 3,      /* "RFGap-DTL"   */
+  HPS_GAP_CCL = 4,      /* "RFGap-CCL"   */
+  HPS_APERTURE_C = 5,   /* "ApertureC"   */
+  HPS_APERTURE_R = 6,   /* "ApertureR"   */
+  HPS_SPCH_COMP = 7,    /* "SpchComp"    */
+  HPS_DIAGNOSTICS = 8,  /* "Diagnostics" */
+  HPS_ROTATION = 9,     /* "Rotation"    */
+  HPS_STEERER = 10      /* "Steerer"     */
+};
+
+/* RF gap parameters, field for field RFGapParameter
+ * (src/beamline/inc/rfgap_parameter.h:4-32). */
+typedef struct {
+  double length, frequency, cell_length_over_beta_lambda, amplitude;
+  double phase_ref, phase_shift, energy_out, beta_center, dg, t, tp, sp;
+  double fit_beta_center, fit_beta_min;
+  double fit_t0, fit_t1, fit_t2, fit_t3, fit_t4, fit_t5;
+  double fit_s1, fit_s2, fit_s3, fit_s4, fit_s5;
+} hps_gap_param;
+
+/* One beamline element: the union of the element classes' host fields
+ * (beamline_element.h); a kind reads only its own fields. */
+typedef struct {
+  int32_t kind;            /* enum hps_element_kind */
+  int32_t aperture_in;     /* ApertureC/ApertureR::IsIn() */
+  double length;           /* m */
+  double aperture;         /* m, 0 = none */
+  double gradient;         /* Quad, T/m */
+  double angle;            /* Rotation, rad */
+  double bl_h, bl_v;       /* Steerer integrated fields, T*m */
+  double aper_xl, aper_xr, aper_yt, aper_yb;  /* ApertureR, m */
+  double fraction;         /* SpchComp */
+  hps_gap_param gap;       /* RFGap-DTL / RFGap-CCL */
+} hps_element;
+
+/* Scalar beam statistics: the device scalars of the reference Beam
+ * (src/beam/inc/beam.h:175-263). */
+typedef struct {
+  double x_avg, x_avg_good, xp_avg, y_avg, y_avg_good, yp_avg;
+  double phi_avg, phi_avg_r, phi_avg_good, w_avg, w_avg_good;
+  double x_sig, x_sig_good, xp_sig, y_sig, y_sig_good, yp_sig;
+  double phi_sig, phi_sig_r, phi_sig_good, w_sig, w_sig_good;
+  double x_emit, y_emit, z_emit;
+  uint32_t num_loss, num_lloss, num_good, pad_;
+} hps_beam_stats;
+
+/* Space-charge state: SpaceCharge members (src/simulation/inc/space_charge.h:83-124)
+ * plus the SCHEFF mesh spacing (space_charge_kernel.cu:12-13). */
+typedef struct {
+  uint32_t nx, nz, nx_init, nz_init, adj_bunch, good_cnt_old;
+  double fraction, interval, interval_init, adj_bunch_w, mesh_w;
+  double sigr_old, sigz_old, gm_old, remesh_val;
+  double dr, dz;
+  uint32_t num_on_mesh, remesh_count, kick_count, pad_;
+} hps_spch_state;
+
+const char* hps_last_error(void);
+
+/* Device selection; replaces set_gpu / SetGPU (src/beamline/src/init.cpp:12-18). */
+int hps_set_device(int device);
+
+/* ---- Beam (src/pywrapper/src/wrap_beam.cpp) ---- */
+/* Beam(mass=, charge=, current=, num=), wrap_beam.cpp:26-50 / beam.cpp:124-134 */
+int hps_beam_create(hps_beam** out, uint32_t num, double mass, double charge, double current);
+int hps_beam_destroy(hps_beam* b);
+/* set_distribution(x, xp, y, yp, phi, w[, loss[, lloss]]), wrap_beam.cpp:61-118,
+ * beam.cpp:317-368.  loss / lloss may be NULL (zeros). */
+int hps_beam_set_distribution(hps_beam* b, const double* x, const double* xp,
+    const double* y, const double* yp, const double* phi, const double* w,
+    const uint32_t* loss, const uint32_t* lloss);
+/* Copy coordinates to host; any pointer may be NULL.  Replaces the array
+ * part of get_x/get_xp/.../get_losses, wrap_beam.cpp:440-912. */
+int hps_beam_download(hps_beam* b, double* x, double* xp, double* y, double* yp,
+    double* phi, double* w, uint32_t* loss, uint32_t* lloss);
+/* Recompute the relative phase (reference particle, or average of good
+ * particles when use_good != 0) and copy it out; beam.cpp:1063-1066. */
+int hps_beam_relative_phi(hps_beam* b, int use_good, double* out);
+uint32_t hps_beam_size(const hps_beam* b);
+/* set_ref_w / set_ref_phi / set_frequency, wrap_beam.cpp:271-341 */
+int hps_beam_set_ref_w(hps_beam* b, double w);
+int hps_beam_set_ref_phi(hps_beam* b, double phi);
+int hps_beam_set_frequency(hps_beam* b, double freq);
+/* get_mass/get_charge/get_frequency/get_ref_w/get_ref_phi/get_current and the
+ * design energy (beam.h:140-151); wrap_beam.cpp:343-422,1286-1298.  which:
+ * 0 mass, 1 charge, 2 frequency, 3 ref_w (w[0]), 4 ref_phi (phi[0]),
+ * 5 current, 6 design_w, 7 current scaled by transmission. */
+int hps_beam_get_scalar(hps_beam* b, int which, double* out);
+/* get_loss_num('t'|'l'|'all'), wrap_beam.cpp:1245-1284 (which 0|1|2). */
+int hps_beam_get_loss_num(hps_beam* b, int which, uint32_t* out);
+/* get_avg_{x,y,phi,w}, wrap_beam.cpp:914-1049.  var: 0 x, 1 y, 2 phi,
+ * 3 w, 4 relative phi; good: 0 "transverse_good"/"absolute", 1 "longitudinal_good". */
+int hps_beam_get_avg(hps_beam* b, int var, int good, double* out);
+/* get_sig_{x,y,phi,w}, wrap_beam.cpp:1051-1186, same var/good codes. */
+int hps_beam_get_sig(hps_beam* b, int var, int good, double* out);
+/* get_emittance_{x,y,z}, wrap_beam.cpp:1188-1243 (axis 0|1|2). */
+int hps_beam_get_emittance(hps_beam* b, int axis, double* out);
+/* Full scalar state (for diagnostics and tests). */
+int hps_beam_get_stats(hps_beam* b, hps_beam_stats* out);
+/* save/restore_{initial,intermediate}_beam, wrap_beam.cpp:163-233 (slot 0|1). */
+int hps_beam_save(hps_beam* b, int slot);
+int hps_beam_restore(hps_beam* b, int slot);
+/* translate(axis, val), wrap_beam.cpp:1317-1351; axis 0 x,1 xp,2 y,3 yp,4 phi,5 w. */
+int hps_beam_translate(hps_beam* b, int axis, double val);
+/* apply_cut(coord, min, max), wrap_beam.cpp:1353-1371; coord 'x','y','w','p'. */
+int hps_beam_apply_cut(hps_beam* b, char coord, double lo, double hi);
+
+/* ---- BeamLine (src/beamline/inc/beamline.h:11-34) ---- */
+int hps_beamline_create(hps_beamline** out);
+int hps_beamline_destroy(hps_beamline* bl);
+/* BeamLine::AddElement, beamline.h:53-57 (the reference fills it from the
+ * database, src/beamline/src/init.cpp:20-429). */
+int hps_beamline_add(hps_beamline* bl, const char* name, const hps_element* e);
+/* Replace element i (model-parameter update, the PV path of the reference). */
+int hps_beamline_set(hps_beamline* bl, uint32_t index, const hps_element* e);
+uint32_t hps_beamline_size(const hps_beamline* bl);
+/* BeamLine::GetElementModelIndex (case-insensitive), beamline.cpp:70-77. */
+int hps_beamline_index(const hps_beamline* bl, const char* name, uint32_t* out);
+
+/* ---- SpaceCharge (src/pywrapper/src/wrap_spacecharge.cpp) ---- */
+/* SpaceCharge(nr, nz, interval, adj_bunch), wrap_spacecharge.cpp:31-56 */
+int hps_spch_create(hps_spch** out, uint32_t nr, uint32_t nz, double interval, uint32_t adj_bunch);
+int hps_spch_destroy(hps_spch* s);
+/* set_{interval,remesh_threshold,adj_bunch,adj_bunch_cutoff_w,mesh_size_cutoff_w}
+ * (what 0..4), space_charge.h:60-109. */
+int hps_spch_set(hps_spch* s, int what, double value);
+/* set_mesh_size(nr, nz), space_charge.cpp:61-73 */
+int hps_spch_set_mesh_size(hps_spch* s, uint32_t nr, uint32_t nz);
+int hps_spch_get_state(hps_spch* s, hps_spch_state* out);
+/* Tables of the last remesh: Green's table in reference layout
+ * [rp][rs][dz] ((nr+1)*nr*nz double2), field table [zp][rp]
+ * ((nz+1)*(nr+1) double2), fixed-point charge histogram (nr*nz int64, units
+ * of 2^-36 particle).  Any pointer may be NULL. */
+int hps_spch_get_tables(hps_spch* s, double* fld1, double* fld2, int64_t* bins);
+
+/* ---- Simulator (src/pywrapper/src/wrap_simulator.cpp) ---- */
+/* Simulator(beam, beamline, spch=None), wrap_simulator.cpp:27-54 */
+int hps_sim_create(hps_sim** out, hps_beam* b, hps_beamline* bl, hps_spch* s);
+int hps_sim_destroy(hps_sim* m);
+/* simulate() / simulate(start, end): inclusive model indices, -1 = first /
+ * last element; SimulationEngine::Simulate, simulation_engine.cpp:73-95. */
+int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end);
+/* reset(), wrap_simulator.cpp:67-72 */
+int hps_sim_reset(hps_sim* m);
+/* set_space_charge('on'|'off'), wrap_simulator.cpp:101-118 */
+int hps_sim_set_space_charge(hps_sim* m, int on);
+/* Per-kernel-family device timing with HIP events around every launch of
+ * later simulate() calls (families: 0 fused element maps + kick + moments,
+ * 1 relative phase + SCHEFF control, 2 Green's table, 3 charge deposit,
+ * 4 field sum).  Enabling resets the counters.  No reference counterpart:
+ * the reference only times whole simulate() calls (timer.cpp:4-19). */
+int hps_sim_set_profiling(hps_sim* m, int on);
+int hps_sim_get_profile(hps_sim* m, double* ms, uint32_t* launches);
+/* Device time of the last simulate() in ms (the reference prints it,
+ * simulation_engine.cpp:84-92). */
+int hps_sim_last_time_ms(hps_sim* m, double* out);
+
+/* ---- elementary-function probes (host-callable, run on the device) ---- */
+/* Evaluates fn over n inputs on the GPU: 0 exp, 1 log, 2 sin, 3 cos, 4 sinh,
+ * 5 cosh, 6 sqrt, 7 fmod(x, 2pi), 8 x/y, 9 Bessel I0, 10 Bessel I1,
+ * 11 Bessel I1'. */
+int hps_probe_math(int fn, const double* x, const double* y, double* out, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

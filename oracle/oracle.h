@@ -1,0 +1,159 @@
+/*
+ * oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Serial FP64 CPU restatement of the reference HPSim tracking hot path
+ * (element maps, beam statistics, SCHEFF space charge, simulation engine).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library, and only as the checker / CPU baseline; the product
+ * (hpsim_amd) never links, imports or calls it.
+ */
+#ifndef HPSIM_ORACLE_H
+#define HPSIM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element kinds (reference element type strings, beamline_element.h). */
+enum {
+  ORC_EL_DRIFT = 1,        /* "Drift"       */
+  ORC_EL_QUAD = 2,         /* "Quad"        */
+  ORC_EL_GAP_DTL = 3,      /* "RFGap-DTL"   */
+  ORC_EL_GAP_CCL = 4,      /* "RFGap-CCL"   */
+  ORC_EL_APERTURE_C = 5,   /* "ApertureC"   */
+  ORC_EL_APERTURE_R = 6,   /* "ApertureR"   */
+  ORC_EL_SPCH_COMP = 7,    /* "SpchComp"    */
+  ORC_EL_DIAGNOSTICS = 8,  /* "Diagnostics" */
+  ORC_EL_ROTATION = 9,     /* "Rotation"    */
+  ORC_EL_STEERER = 10      /* "Steerer"     */
+};
+
+/* RFGapParameter, reference rfgap_parameter.h:4-32 (same field order). */
+typedef struct {
+  double length, frequency, cell_length_over_beta_lambda, amplitude;
+  double phase_ref, phase_shift, energy_out, beta_center, dg, t, tp, sp;
+  double fit_beta_center, fit_beta_min;
+  double fit_t0, fit_t1, fit_t2, fit_t3, fit_t4, fit_t5;
+  double fit_s1, fit_s2, fit_s3, fit_s4, fit_s5;
+} orc_gap_param;
+
+typedef struct {
+  int32_t type;
+  int32_t aperture_in;     /* ApertureC/ApertureR IsIn() */
+  double length;
+  double aperture;
+  double gradient;         /* Quad, T/m */
+  double angle;            /* Rotation, rad */
+  double bl_h, bl_v;       /* Steerer integrated fields */
+  double aper_xl, aper_xr, aper_yt, aper_yb;  /* ApertureR */
+  double fraction;         /* SpchComp */
+  orc_gap_param gap;
+} orc_element;
+
+/* Scalar beam statistics (device scalars of reference beam.h:175-263). */
+typedef struct {
+  double x_avg, x_avg_good, xp_avg, y_avg, y_avg_good, yp_avg;
+  double phi_avg, phi_avg_r, phi_avg_good, w_avg, w_avg_good;
+  double x_sig, x_sig_good, xp_sig, y_sig, y_sig_good, yp_sig;
+  double phi_sig, phi_sig_r, phi_sig_good, w_sig, w_sig_good;
+  double x_emit, y_emit, z_emit;
+  uint32_t num_loss, num_lloss, num_good, pad_;
+} orc_beam_stats;
+
+/* Space-charge (SCHEFF) state, reference space_charge.h:83-124. */
+typedef struct {
+  uint32_t nx, nz, nx_init, nz_init, adj_bunch, good_cnt_old;
+  double fraction, interval, interval_init, adj_bunch_w, mesh_w;
+  double sigr_old, sigz_old, gm_old, remesh_val;
+  double dr, dz;           /* scheff_sz, space_charge_kernel.cu:12 */
+  uint32_t num_on_mesh, remesh_count, kick_count, pad_;
+} orc_spch_state;
+
+typedef struct orc_beam orc_beam;
+typedef struct orc_spch orc_spch;
+typedef struct orc_sim orc_sim;
+
+/* Canonical reduction order shared with the GPU path. */
+#define ORC_CANON_LANES 256
+#define ORC_CANON_PER_LANE 4
+#define ORC_CANON_CHUNK (ORC_CANON_LANES * ORC_CANON_PER_LANE)
+/* Fixed-point deposit scale: weights are truncated to multiples of 2^-36. */
+#define ORC_DEPOSIT_SHIFT 36
+
+const char* orc_last_error(void);
+
+/* --- beam --- */
+orc_beam* orc_beam_create(uint32_t num, double mass, double charge, double current);
+void orc_beam_destroy(orc_beam* b);
+int orc_beam_set_distribution(orc_beam* b, const double* x, const double* xp,
+    const double* y, const double* yp, const double* phi, const double* w,
+    const uint32_t* loss, const uint32_t* lloss);
+void orc_beam_set_ref_w(orc_beam* b, double w);
+void orc_beam_set_ref_phi(orc_beam* b, double phi);
+void orc_beam_set_frequency(orc_beam* b, double f);
+double orc_beam_get_frequency(const orc_beam* b);
+double orc_beam_get_design_w(const orc_beam* b);
+uint32_t orc_beam_size(const orc_beam* b);
+void orc_beam_get_arrays(const orc_beam* b, double* x, double* xp, double* y,
+    double* yp, double* phi, double* phi_r, double* w, uint32_t* loss, uint32_t* lloss);
+void orc_beam_get_stats(const orc_beam* b, orc_beam_stats* s);
+void orc_beam_set_num_loss(orc_beam* b, uint32_t n);
+/* Getter semantics of wrap_beam.cpp (they update device state, Q1). */
+uint32_t orc_beam_loss_num(orc_beam* b, int which);        /* 0 't', 1 'l', 2 'all' */
+double orc_beam_emittance(orc_beam* b, int axis);          /* 0 x, 1 y, 2 z */
+double orc_beam_avg(orc_beam* b, int var, int good);       /* var 0 x,1 y,2 phi,3 w,4 phi-relative */
+double orc_beam_sig(orc_beam* b, int var, int good);       /* var 0 x,1 y,2 phi,3 w,4 phi-relative */
+void orc_beam_update_loss(orc_beam* b);
+void orc_beam_update_stat_for_space_charge(orc_beam* b);
+void orc_beam_save_initial(orc_beam* b);
+void orc_beam_restore_initial(orc_beam* b);
+void orc_beam_save_intermediate(orc_beam* b);
+void orc_beam_restore_intermediate(orc_beam* b);
+void orc_beam_translate(orc_beam* b, int axis, double val);  /* 0 x,1 xp,2 y,3 yp,4 phi,5 w */
+void orc_beam_apply_cut(orc_beam* b, char coord, double lo, double hi);
+
+/* --- space charge --- */
+orc_spch* orc_spch_create(uint32_t nr, uint32_t nz, int adj_bunch);
+void orc_spch_destroy(orc_spch* s);
+void orc_spch_set_interval(orc_spch* s, double v);
+void orc_spch_set_remesh_threshold(orc_spch* s, double v);
+void orc_spch_set_adj_bunch(orc_spch* s, uint32_t v);
+void orc_spch_set_adj_bunch_cutoff_w(orc_spch* s, double v);
+void orc_spch_set_mesh_size_cutoff_w(orc_spch* s, double v);
+void orc_spch_set_mesh_size(orc_spch* s, uint32_t nr, uint32_t nz);
+void orc_spch_get_state(const orc_spch* s, orc_spch_state* st);
+/* Tables of the last remesh: fld1 (nr+1)*nr*nz double2 in reference layout
+ * [rp][rs][dz]; fld2 (nz+1)*(nr+1) double2 [zp][rp]; bins nr*nz int64. */
+void orc_spch_get_tables(const orc_spch* s, double* fld1, double* fld2, int64_t* bins);
+int orc_spch_start(orc_spch* s, orc_beam* b, double length);
+
+/* --- simulator --- */
+orc_sim* orc_sim_create(orc_beam* b, const orc_element* elems, uint32_t n, orc_spch* s);
+void orc_sim_destroy(orc_sim* m);
+int orc_sim_simulate(orc_sim* m, int32_t start, int32_t end);
+void orc_sim_reset(orc_sim* m);
+void orc_sim_set_space_charge(orc_sim* m, int on);
+
+/* --- pieces exposed for pinning tests --- */
+double orc_t_exp(double x);
+double orc_t_log(double x);
+void orc_t_sincos(double x, double* s, double* c);
+double orc_t_sinh(double x);
+double orc_t_cosh(double x);
+double orc_t_fmod_pos(double x, double y);
+double orc_t_bessi0(double x);
+double orc_t_bessi1(double x);
+double orc_t_bessi1p(double x);
+void orc_t_ttf(double beta, const orc_gap_param* g, double* t, double* tp, double* sp);
+double orc_t_canon_sum(const double* v, uint32_t n);
+void orc_t_green_table(uint32_t nr, uint32_t nz, double dr, double dz, double freq,
+    double beta, uint32_t num, double current_ov_mc2, double* fld1);
+void orc_t_field(uint32_t nr, uint32_t nz, const int64_t* bins, const double* fld1, double* fld2);
+uint32_t orc_t_nextpow2(uint32_t v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

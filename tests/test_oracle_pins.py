@@ -1,0 +1,391 @@
+"""Pins of the CPU oracle against what the mathematics and independent
+libraries fix: closed forms, library functions (numpy / scipy / mpmath),
+brute force on small inputs, invariants and published known-answer vectors.
+The reference ships no tests or golden vectors for this path, so these are
+the anchors of every parity claim."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sp
+
+from hpsim_amd import configs as cfg
+
+ULP = np.finfo(np.float64).eps
+
+
+@pytest.fixture(scope="module")
+def L(oracle_lib):
+    return oracle_lib.lib()
+
+
+def vec(f, xs):
+    return np.array([f(float(v)) for v in xs])
+
+
+# --------------------------------------------------------------------------
+# elementary functions
+# --------------------------------------------------------------------------
+
+def test_exp_within_one_ulp(L):
+    x = np.random.default_rng(0).uniform(-700, 700, 20000)
+    got = vec(L.orc_t_exp, x)
+    assert np.max(np.abs(got - np.exp(x)) / np.exp(x)) <= 2 * ULP
+    assert L.orc_t_exp(0.0) == 1.0
+    assert L.orc_t_exp(800.0) == math.inf and L.orc_t_exp(-800.0) == 0.0
+    assert math.isnan(L.orc_t_exp(math.nan))
+
+
+def test_log_within_one_ulp(L):
+    rng = np.random.default_rng(1)
+    x = np.concatenate([10.0 ** rng.uniform(-300, 300, 10000), rng.uniform(0.5, 2.0, 10000),
+                        [5e-324, 2.2e-308]])
+    got = vec(L.orc_t_log, x)
+    ref = np.log(x)
+    assert np.all(np.abs(got - ref) <= 2 * ULP * np.maximum(np.abs(ref), 1e-300) + 1e-300)
+    assert L.orc_t_log(1.0) == 0.0
+    assert L.orc_t_log(0.0) == -math.inf
+    assert math.isnan(L.orc_t_log(-1.0))
+
+
+def test_sincos_against_mpmath(L):
+    import mpmath
+    mpmath.mp.prec = 120
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(-1e4, 1e4, 3000), rng.uniform(-4, 4, 2000),
+                        [0.0, 1e-300, np.pi / 2, np.pi, 1e6]])
+    s, c = C.c_double(), C.c_double()
+    for v in x:
+        L.orc_t_sincos(float(v), C.byref(s), C.byref(c))
+        es, ec = float(mpmath.sin(mpmath.mpf(float(v)))), float(mpmath.cos(mpmath.mpf(float(v))))
+        assert abs(s.value - es) <= 2 * ULP * max(abs(es), 1e-300) + 1e-300 or abs(s.value - es) <= 2e-16
+        assert abs(c.value - ec) <= 2 * ULP * max(abs(ec), 1e-300) + 1e-300 or abs(c.value - ec) <= 2e-16
+
+
+def test_sinh_cosh(L):
+    x = np.random.default_rng(3).uniform(-20, 20, 20000)
+    assert np.max(np.abs(vec(L.orc_t_sinh, x) - np.sinh(x)) / np.maximum(np.abs(np.sinh(x)), 1e-300)) <= 3 * ULP
+    assert np.max(np.abs(vec(L.orc_t_cosh, x) - np.cosh(x)) / np.cosh(x)) <= 3 * ULP
+    assert L.orc_t_sinh(0.0) == 0.0 and L.orc_t_cosh(0.0) == 1.0
+
+
+def test_fmod_is_exact(L):
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(0, 1e5, 20000), [0.0, 2 * np.pi, 4 * np.pi, 1e-30]])
+    got = vec(lambda v: L.orc_t_fmod_pos(v, 2 * np.pi), x)
+    assert np.array_equal(got.view(np.uint64), np.fmod(x, 2 * np.pi).view(np.uint64))
+
+
+# --------------------------------------------------------------------------
+# Bessel and transit-time factor (simulation_kernel.cu:15-73, 290-308)
+# --------------------------------------------------------------------------
+
+def test_bessel_polynomials_within_abramowitz_stegun_bounds(L):
+    x = np.concatenate([np.linspace(-3.74, 3.74, 801), np.linspace(3.76, 30, 400)])
+    i0 = vec(L.orc_t_bessi0, x)
+    i1 = vec(L.orc_t_bessi1, x)
+    # A&S 9.8.1-9.8.4: relative errors below 1.6e-7 / 1.9e-7 / 8e-9 / 2.2e-7
+    assert np.max(np.abs(i0 / sp.i0(x) - 1)) < 3e-7
+    nz = np.abs(x) > 1e-3
+    assert np.max(np.abs(i1[nz] / sp.i1(x[nz]) - 1)) < 3e-7
+    # I1' = I0 - I1/x; limit 0.5 at 0
+    xp = x[nz]
+    assert np.max(np.abs(vec(L.orc_t_bessi1p, xp) - sp.ivp(1, xp)) / np.abs(sp.ivp(1, xp))) < 1e-6
+    assert L.orc_t_bessi1p(0.0) == 0.5
+
+
+def _gap(**kw):
+    from oracle.oracle import GapParam
+    g = GapParam()
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+def test_transit_time_factor_polynomial_and_clamp(L):
+    t = [0.8, 0.05, -0.2, 0.3, 0.01, -0.002]
+    s = [0.1, 0.02, -0.03, 0.004, 0.0005]
+    g = _gap(fit_beta_center=0.1, fit_beta_min=0.05, **{f"fit_t{i}": v for i, v in enumerate(t)},
+             **{f"fit_s{i + 1}": v for i, v in enumerate(s)})
+    T, Tp, Sp = C.c_double(), C.c_double(), C.c_double()
+    for beta in (0.06, 0.08, 0.1, 0.109):
+        L.orc_t_ttf(beta, C.byref(g), C.byref(T), C.byref(Tp), C.byref(Sp))
+        poly = np.polynomial.Polynomial(t)
+        assert abs(T.value - poly(beta)) < 1e-15
+        # T' and S' are beta^2 / (2 pi beta_g) times the beta-derivatives
+        coef = beta * beta / (2 * np.pi * 0.1)
+        assert abs(Tp.value - coef * poly.deriv()(beta)) < 1e-15
+        spoly = np.polynomial.Polynomial([0.0] + s)
+        assert abs(Sp.value + coef * spoly.deriv()(beta)) < 1e-15
+    # clamp: above 1.1 beta_g the factor is evaluated at beta_g; below beta_min at beta_min
+    L.orc_t_ttf(0.2, C.byref(g), C.byref(T), C.byref(Tp), C.byref(Sp))
+    assert T.value == pytest.approx(np.polynomial.Polynomial(t)(0.1), abs=1e-15)
+    L.orc_t_ttf(0.01, C.byref(g), C.byref(T), C.byref(Tp), C.byref(Sp))
+    assert T.value == pytest.approx(np.polynomial.Polynomial(t)(0.05), abs=1e-15)
+
+
+# --------------------------------------------------------------------------
+# canonical reduction
+# --------------------------------------------------------------------------
+
+def test_canonical_sum_exact_on_integers_and_close_to_fsum(L):
+    rng = np.random.default_rng(5)
+    for n in (1, 255, 1024, 1025, 70001):
+        v = rng.integers(-1000, 1000, n).astype(np.float64)
+        assert L.orc_t_canon_sum(v.ctypes.data_as(C.POINTER(C.c_double)), n) == float(v.sum())
+        w = rng.standard_normal(n)
+        s = L.orc_t_canon_sum(w.ctypes.data_as(C.POINTER(C.c_double)), n)
+        assert abs(s - math.fsum(w)) <= 4 * ULP * math.fsum(np.abs(w))
+
+
+# --------------------------------------------------------------------------
+# SCHEFF: ring Green's function and field sum (space_charge_kernel.cu)
+# --------------------------------------------------------------------------
+
+def test_green_table_matches_exact_ring_field(L):
+    """fld1 = c1 * (E_r, E_z) of a charged ring; compare the Hastings
+    polynomials of the reference with scipy's complete elliptic integrals."""
+    nr, nz, dr, dz, freq, beta, num, cur = 8, 16, 1e-4, 2e-4, 201.25, 0.05, 1000, 0.01 / 939.294
+    f1 = np.empty((nr + 1) * nr * nz * 2)
+    L.orc_t_green_table(nr, nz, dr, dz, freq, beta, num, cur, f1.ctypes.data_as(C.POINTER(C.c_double)))
+    f1 = f1.reshape(nr + 1, nr, nz, 2)
+    gm = 1 / math.sqrt(1 - beta * beta)
+    dzg = dz * gm
+    c1 = 5721.67 * cur / (freq * 1e6 * num)
+    # A&S 17.3.34 / 17.3.36: |error| of the polynomial K, E below 3e-5 / 4e-5
+    eK, eE = 3e-5, 4e-5
+    for rp_i in range(nr + 1):
+        for k in range(nr):
+            for t in range(nz):
+                rs = math.sqrt(0.5 * (k * k + (k + 1) ** 2)) * dr
+                rp = rp_i * dr
+                d = (t + 0.5) * dzg
+                b = (rp + rs) ** 2
+                c = (rp - rs) ** 2
+                m = 4 * rs * rp / (b + d * d)
+                K, E = sp.ellipk(m), sp.ellipe(m)
+                a = math.sqrt(b + d * d)
+                ez = d * E / (a * (c + d * d))
+                assert abs(f1[rp_i, k, t, 1] / c1 - ez) <= eE * d / (a * (c + d * d)) * 1.01
+                if rp_i == 0:
+                    assert f1[rp_i, k, t, 0] == 0.0
+                    # on axis: E_z of a ring is (pi/2) d / (rs^2 + d^2)^1.5
+                    assert abs(ez - (math.pi / 2) * d / (rs * rs + d * d) ** 1.5) < 1e-12 * ez
+                else:
+                    R = (rs * rs - rp * rp + d * d) / (c + d * d)
+                    er = (K - R * E) / (2 * rp * a)
+                    bound = (eK + eE * abs(R)) / (2 * rp * a) * 1.01
+                    assert abs(f1[rp_i, k, t, 0] / c1 - er) <= bound
+
+
+def test_field_sum_brute_force(L):
+    """Canonical field sum vs a plain double loop of charge x Green's value."""
+    nr, nz = 4, 8
+    rng = np.random.default_rng(6)
+    bins = rng.integers(0, 2 ** 40, nr * nz).astype(np.int64)
+    bins[rng.random(nr * nz) < 0.3] = 0
+    f1 = rng.standard_normal((nr + 1) * nr * nz * 2)
+    f2 = np.empty((nz + 1) * (nr + 1) * 2)
+    L.orc_t_field(nr, nz, bins.ctypes.data_as(C.POINTER(C.c_int64)),
+                  f1.ctypes.data_as(C.POINTER(C.c_double)), f2.ctypes.data_as(C.POINTER(C.c_double)))
+    G = f1.reshape(nr + 1, nr, nz, 2)
+    q = bins.astype(np.float64) * 2.0 ** -36
+    for zp in range(nz + 1):
+        for rp in range(nr + 1):
+            ex, ez = [], []
+            for zs in range(nz):
+                for rs in range(nr):
+                    # field point z index zp sits (zp - zs - 1/2) bins from source zs
+                    off = zp - zs - 0.5
+                    dzi = int(abs(off) - 0.5)
+                    sgn = 1.0 if off > 0 else -1.0
+                    ex.append(q[zs * nr + rs] * G[rp, rs, dzi, 0])
+                    ez.append(sgn * q[zs * nr + rs] * G[rp, rs, dzi, 1])
+            i = zp * (nr + 1) + rp
+            assert f2[2 * i] == pytest.approx(math.fsum(ex), rel=1e-13, abs=1e-300)
+            assert f2[2 * i + 1] == pytest.approx(math.fsum(ez), rel=1e-12, abs=1e-12 * max(map(abs, ez)))
+
+
+# --------------------------------------------------------------------------
+# element maps through the oracle engine
+# --------------------------------------------------------------------------
+
+def _single(O, elements, x=0.0, xp=0.0, y=0.0, yp=0.0, phi=0.0, w=10.0, n=2, sc=None):
+    """Beam of n identical particles (index 0 is forced on axis)."""
+    b = O.Beam(n, 939.294, -1.0, 0.0)
+    b.set_distribution(*[np.full(n, v) for v in (x, xp, y, yp, phi, w)])
+    b.set_frequency(201.25)
+    b.set_ref_w(w)
+    sim = O.Simulator(b, elements, sc)
+    sim.simulate()
+    return b.arrays()
+
+
+def test_drift_closed_form(oracle_lib):
+    O = oracle_lib
+    L_ = 0.37
+    a = _single(O, [cfg.Element("D", "Drift", length=L_)], x=1e-3, xp=2e-3, y=-1e-3, yp=5e-4)
+    g = 10.0 / 939.294
+    beta = math.sqrt(g * (g + 2)) / (g + 1)
+    lam = 299.792458 / 201.25
+    assert a["x"][1] == pytest.approx(1e-3 + 2e-3 * L_, rel=1e-15)
+    assert a["y"][1] == pytest.approx(-1e-3 + 5e-4 * L_, rel=1e-15)
+    assert a["xp"][1] == 2e-3 and a["w"][1] == 10.0
+    assert a["phi"][1] == pytest.approx(2 * math.pi * L_ / (lam * beta), rel=1e-14)
+
+
+def test_drift_aperture_loss_code_is_element_index(oracle_lib):
+    O = oracle_lib
+    el = [cfg.Element("D0", "Drift", length=0.1, aperture=0.01),
+          cfg.Element("D1", "Drift", length=1.0, aperture=0.01)]
+    a = _single(O, el, x=5e-3, xp=9e-3, n=3)
+    # first half of D0 ends at 5e-3 + 9e-3 * 0.05 < 1 cm; D1 pushes it out: code 1;
+    # a particle lost in element 0 would carry code 0 (= not lost, reference quirk)
+    assert list(a["loss"]) == [0, 1, 1]
+    a0 = _single(O, el[:1], x=9.9e-3, xp=1e-1, n=3)
+    assert list(a0["loss"]) == [0, 0, 0] and a0["x"][1] == 9.9e-3
+
+
+def test_quad_thick_lens_matrices(oracle_lib):
+    """Two quarter-length steps per half equal the textbook thick-lens
+    matrix; det = 1 (symplectic) in both planes."""
+    O = oracle_lib
+    G, Lq, w = 12.0, 0.08, 5.0
+    el = [cfg.Element("Q", "Quad", length=Lq, gradient=G)]
+    rays = {}
+    for name, (x, xp) in {"a": (1e-6, 0.0), "b": (0.0, 1e-9)}.items():
+        a = _single(O, el, x=x, xp=xp, y=x, yp=xp, w=w)
+        rays[name] = (a["x"][1], a["xp"][1], a["y"][1], a["yp"][1])
+    g1 = w / 939.294
+    k = math.sqrt(abs(G * -1.0) / (math.sqrt(g1 * (g1 + 2)) * 939.294) * 299.792458)
+    # charge -1 and G > 0: qG < 0 -> defocusing in x, focusing in y
+    Mx = np.array([[math.cosh(k * Lq), math.sinh(k * Lq) / k], [k * math.sinh(k * Lq), math.cosh(k * Lq)]])
+    My = np.array([[math.cos(k * Lq), math.sin(k * Lq) / k], [-k * math.sin(k * Lq), math.cos(k * Lq)]])
+    gx = np.array([[rays["a"][0] / 1e-6, rays["b"][0] / 1e-9], [rays["a"][1] / 1e-6, rays["b"][1] / 1e-9]])
+    gy = np.array([[rays["a"][2] / 1e-6, rays["b"][2] / 1e-9], [rays["a"][3] / 1e-6, rays["b"][3] / 1e-9]])
+    assert np.allclose(gx, Mx, rtol=1e-9, atol=1e-12)
+    assert np.allclose(gy, My, rtol=1e-9, atol=1e-12)
+    assert abs(np.linalg.det(gx) - 1) < 1e-9 and abs(np.linalg.det(gy) - 1) < 1e-9
+    # G = 0 is a drift
+    a = _single(O, [cfg.Element("Q", "Quad", length=Lq, gradient=0.0)], x=1e-3, xp=1e-3)
+    assert a["x"][1] == pytest.approx(1e-3 + 1e-3 * Lq, rel=1e-15)
+
+
+def test_synchronous_particle_follows_design(oracle_lib):
+    """Particle 0 (on axis, never kicked) gains V*T*cos(phi_s) per gap
+    (Panofsky), so its energy equals the lattice's design output energy."""
+    wl = cfg.config(0, num=64, periods=6)
+    elems = cfg.build_lattice(wl.lattice)
+    from tests.helpers import run_oracle
+    beam, _, _ = run_oracle(wl, elems=elems)
+    a = beam.arrays()
+    assert a["w"][0] == pytest.approx(elems[-1].gap["energy_out"], rel=1e-12)
+    gaps = [e for e in elems if e.kind == "RFGap-DTL"]
+    g0 = gaps[0].gap
+    assert g0["energy_out"] - 0.75 == pytest.approx(
+        0.3 * cfg._ttf(g0["beta_center"], g0) * math.cos(math.radians(-30)), rel=1e-12)
+
+
+def test_space_charge_kick_skips_every_quarter_of_nextpow2(oracle_lib):
+    """Q3: ApplySpchKickKernel's thread 0 exits, so particle indices that are
+    multiples of NextPow2(N)/4 are never kicked (space_charge_kernel.cu:586-588)."""
+    O = oracle_lib
+    n = 3000
+    wl = cfg.config(0, num=n, periods=1)
+    dist = cfg.make_distribution(wl.beam)
+    b = O.Beam(n, 939.294, -1.0, 0.05)
+    b.set_distribution(*dist)
+    b.set_frequency(201.25)
+    b.set_ref_w(0.75)
+    before = b.arrays()
+    sc = O.SpaceCharge(32, 128, 0.05)
+    sc.start(b, 0.05)
+    after = b.arrays()
+    stride = 4096 // 4
+    changed = after["xp"] != before["xp"]
+    skip = np.arange(n) % stride == 0
+    assert not changed[skip].any()
+    assert changed[~skip].mean() > 0.5
+
+
+def test_zero_current_kick_is_near_identity(oracle_lib):
+    """Q9: with zero current the field is zero, but the kick recomputes
+    gamma* non-relativistically, a relative change of order (beta gamma*)^4."""
+    O = oracle_lib
+    n = 2048
+    wl = cfg.config(0, num=n, periods=1)
+    b = O.Beam(n, 939.294, -1.0, 0.0)
+    b.set_distribution(*cfg.make_distribution(wl.beam))
+    b.set_frequency(201.25)
+    b.set_ref_w(0.75)
+    before = b.arrays()
+    O.SpaceCharge(32, 128, 0.05).start(b, 0.05)
+    after = b.arrays()
+    assert np.max(np.abs(after["xp"] - before["xp"]) / np.maximum(np.abs(before["xp"]), 1e-30)) < 1e-9
+    assert np.max(np.abs(after["w"] / before["w"] - 1)) < 1e-12
+
+
+def test_deposit_conserves_charge(oracle_lib):
+    """The fixed-point histogram holds one unit of weight per particle on
+    the mesh (each particle's four weights sum to 1) up to truncation."""
+    O = oracle_lib
+    n = 4096
+    wl = cfg.config(0, num=n, periods=1)
+    b = O.Beam(n, 939.294, -1.0, 0.01)
+    b.set_distribution(*cfg.make_distribution(wl.beam))
+    b.set_frequency(201.25)
+    b.set_ref_w(0.75)
+    sc = O.SpaceCharge(32, 128, 0.05)
+    sc.start(b, 0.05)
+    _, _, bins = sc.tables()
+    on = sc.state()["num_on_mesh"]
+    assert 0 < on <= n
+    assert abs(int(bins.sum()) - on * 2 ** 36) <= 4 * on
+
+
+def test_stale_loss_denominator(oracle_lib):
+    """Q1: the SCHEFF moments divide by N - num_loss, where num_loss only
+    changes in UpdateLoss (getters, apertures); calling a getter between two
+    simulate() calls therefore changes the later result."""
+    O = oracle_lib
+    wl = cfg.config(0, num=2048, periods=4)
+    elems = cfg.build_lattice(wl.lattice)
+    dist = cfg.make_distribution(wl.beam)
+    out = []
+    for call_getter in (False, True):
+        b = O.Beam(2048, 939.294, -1.0, 0.01)
+        b.set_distribution(*dist)
+        b.set_frequency(201.25)
+        b.set_ref_w(0.75)
+        sim = O.Simulator(b, elems, O.SpaceCharge(32, 128, 0.05))
+        sim.simulate(0, 7)
+        if call_getter:
+            assert b.get_loss_num("t") > 0
+        sim.simulate(8, 15)
+        out.append(b.arrays()["x"])
+    assert not np.array_equal(out[0], out[1])
+
+
+# --------------------------------------------------------------------------
+# synthetic workload generator
+# --------------------------------------------------------------------------
+
+def test_philox_known_answer_vectors():
+    """Random123 philox4x32-10 known-answer vectors."""
+    cases = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+             ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+             ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+              (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for c, k, want in cases:
+        r = cfg.philox4x32(*[np.array([v], np.uint32) for v in c], *k)
+        assert tuple(int(v[0]) for v in r) == want
+
+
+def test_truncated_gaussian_moments():
+    z = cfg.truncated_normal(200000, 3, seed=11)
+    assert np.max(np.abs(z)) <= 3.0
+    # variance of a unit normal truncated at 3 sigma: 1 - 6 phi(3) / (2 Phi(3) - 1)
+    var = 1 - 6 * math.exp(-4.5) / math.sqrt(2 * math.pi) / math.erf(3 / math.sqrt(2))
+    assert abs(z.mean()) < 0.01 and abs(z.var() - var) < 0.01
+    assert np.array_equal(z, cfg.truncated_normal(200000, 3, seed=11))
