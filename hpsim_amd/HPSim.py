@@ -435,6 +435,15 @@ class Simulator:
     def set_space_charge(self, on_off):
         check(_lib().hps_sim_set_space_charge(self._h, 1 if str(on_off).lower() == "on" else 0))
 
+    def set_graph(self, on=True):
+        """Launch simulate() as a cached HIP graph (default) or eagerly."""
+        check(_lib().hps_sim_set_graph(self._h, 1 if on else 0))
+
+    def graph_replays(self):
+        v = C.c_uint32()
+        check(_lib().hps_sim_graph_replays(self._h, C.byref(v)))
+        return v.value
+
     def set_profiling(self, on=True):
         """Time every kernel launch of later simulate() calls with HIP events."""
         check(_lib().hps_sim_set_profiling(self._h, 1 if on else 0))

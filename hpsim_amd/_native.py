@@ -68,7 +68,7 @@ EXPORTS = (
     "hps_spch_get_state", "hps_spch_get_tables",
     "hps_sim_create", "hps_sim_destroy", "hps_sim_simulate", "hps_sim_reset",
     "hps_sim_set_space_charge", "hps_sim_last_time_ms", "hps_sim_set_profiling",
-    "hps_sim_get_profile", "hps_probe_math",
+    "hps_sim_get_profile", "hps_sim_set_graph", "hps_sim_graph_replays", "hps_probe_math",
 )
 
 _lib = None
@@ -124,6 +124,8 @@ def _declare(L):
         "hps_sim_last_time_ms": (C.c_int, [P, dp]),
         "hps_sim_set_profiling": (C.c_int, [P, C.c_int]),
         "hps_sim_get_profile": (C.c_int, [P, dp, up]),
+        "hps_sim_set_graph": (C.c_int, [P, C.c_int]),
+        "hps_sim_graph_replays": (C.c_int, [P, up]),
         "hps_probe_math": (C.c_int, [C.c_int, dp, dp, dp, u32]),
     }
     for name, (res, args) in sig.items():

@@ -220,10 +220,13 @@ def build_lattice(spec: LatticeSpec):
         # quad
         g = spec.gradient * (1 if p % 2 == 0 else -1)
         if spec.scale_gradient_with_bg:
+            # thin-lens FODO: sin(mu/2) = L Lq q G c / (2 beta gamma m c^2), so
+            # the phase advance holds for G proportional to beta*gamma / L
             bg = math.sqrt((w / spec.mass) * (w / spec.mass + 2.0))
+            period = 2 * spec.drift_len + spec.quad_len + beta * lam
             if bg0 is None:
-                bg0 = bg
-            g *= bg / bg0
+                bg0 = (bg, period)
+            g *= (bg / bg0[0]) * (bg0[1] / period)
         phi += 2 * (twopi * spec.quad_len * 0.5 / (lam * beta))
         elems.append(Element(f"Q{p}", "Quad", length=spec.quad_len, aperture=spec.aperture,
                              gradient=g))

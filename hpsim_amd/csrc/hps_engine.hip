@@ -293,6 +293,13 @@ struct hps_sim {
   std::vector<hipEvent_t> evpool;
   double prof_ms[5] = {0, 0, 0, 0, 0};
   uint32_t prof_n[5] = {0, 0, 0, 0, 0};
+  // HIP graph of the last launch sequence, reused while the sequence
+  // (every kernel argument and the map table) is byte-identical
+  bool use_graph = true;
+  std::vector<uint8_t> graph_sig;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint32_t graph_replays = 0;
 };
 
 extern "C" {
@@ -329,6 +336,7 @@ int hps_beam_create(hps_beam** out, uint32_t num, double mass, double charge, do
        hipMemset(b->st, 0, sizeof(hps_beam_stats)) == hipSuccess;
   b->pr.nch = (num + kChunk - 1) / kChunk;
   b->pr.mpad = next_pow2(b->pr.nch);
+  if (b->pr.mpad > 4096) { delete b; return fail("a beam holds at most 4,194,304 particles"); }
   const size_t nsum = (size_t)kMaxQ * b->pr.mpad + b->pr.mpad / 2 + 256;
   ok = ok && hipMalloc(&b->pr.sums, nsum * sizeof(double)) == hipSuccess &&
        hipMemset(b->pr.sums, 0, nsum * sizeof(double)) == hipSuccess;
@@ -819,6 +827,8 @@ int hps_sim_destroy(hps_sim* m)
   if (!m) return 0;
   (void)hipStreamSynchronize(m->b->stream);
   for (hipEvent_t e : m->evpool) (void)hipEventDestroy(e);
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
   (void)hipFree(m->d_ops);
   (void)hipEventDestroy(m->ev0);
   (void)hipEventDestroy(m->ev1);
@@ -838,6 +848,20 @@ int hps_sim_set_profiling(hps_sim* m, int on)
   if (!m) return fail("null simulator");
   m->profile = on != 0;
   for (int k = 0; k < 5; ++k) { m->prof_ms[k] = 0; m->prof_n[k] = 0; }
+  return 0;
+}
+
+int hps_sim_set_graph(hps_sim* m, int on)
+{
+  if (!m) return fail("null simulator");
+  m->use_graph = on != 0;
+  return 0;
+}
+
+int hps_sim_graph_replays(hps_sim* m, uint32_t* out)
+{
+  if (!m || !out) return fail("null argument");
+  *out = m->graph_replays;
   return 0;
 }
 
@@ -1094,34 +1118,29 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   const uint32_t kstride = b->grid_size * (b->blck_size / 4);
   const bool kick_ok = kstride != 0;
   const bool deposit_ok = b->blck_size / 4 != 0;
-  // (family, start event, stop event) of the profiled launches
-  std::vector<std::pair<int, size_t>> marks;
-  size_t nev = 0;
-  auto mark = [&](int fam) -> hipError_t {
-    if (!m->profile) return hipSuccess;
-    while (m->evpool.size() < nev + 2) {
-      hipEvent_t e;
-      hipError_t r = hipEventCreate(&e);
-      if (r != hipSuccess) return r;
-      m->evpool.push_back(e);
-    }
-    marks.push_back({fam, nev});
-    hipError_t r = hipEventRecord(m->evpool[nev], s);
-    nev += 2;
-    return r;
+
+  // The launch list of this call.
+  struct Launch {
+    int kind;          // 0 segment, 1 phase, 2 greens, 3 deposit, 4 field, 5 host step
+    SegArgs seg;
+    PhaseArgs ph;
+    double new_freq;
   };
-  auto mark_end = [&]() -> hipError_t {
-    if (!m->profile) return hipSuccess;
-    return hipEventRecord(m->evpool[marks.back().second + 1], s);
-  };
-  HPS_CHECK(hipEventRecord(m->ev0, s));
+  std::vector<Launch> launches;
+  bool host_steps = false;
   for (const Step& st : c.steps) {
     if (st.kind == 1) {
-      if (change_frequency(b, st.new_freq)) return -1;
+      Launch l{};
+      l.kind = 5;
+      l.new_freq = st.new_freq;
+      launches.push_back(l);
+      host_steps = true;
       continue;
     }
     if (st.op_end > st.op_begin || st.kick || st.stats != STAT_NONE) {
-      SegArgs a{};
+      Launch l{};
+      l.kind = 0;
+      SegArgs& a = l.seg;
       a.b = b->d;
       a.st = b->st;
       a.pr = b->pr;
@@ -1134,31 +1153,98 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       a.sc = m->sc ? m->sc->dev : nullptr;
       a.fld2 = m->sc ? m->sc->fld2 : nullptr;
       a.stats = st.stats;
-      HPS_CHECK(mark(0));
-      HPS_CHECK(launch_segment(a, s));
-      HPS_CHECK(mark_end());
+      launches.push_back(l);
     }
     if (st.kick_point) {
       hps_spch* sc = m->sc;
-      PhaseArgs p{};
+      Launch l{};
+      l.kind = 1;
+      PhaseArgs& p = l.ph;
       p.b = b->d; p.st = b->st; p.pr = b->pr; p.sc = sc->dev; p.bins = sc->bins;
       p.do_sums = b->n > 2 ? 1 : 0;
       p.frac = st.frac; p.current = b->current; p.freq = b->freq;
-      HPS_CHECK(mark(1));
-      HPS_CHECK(launch_phase(p, s));
-      HPS_CHECK(mark_end());
-      HPS_CHECK(mark(2));
-      HPS_CHECK(launch_greens(sc->dev, sc->fld1, sc->cfg.nx, sc->cfg.nz, b->n, b->mass, s));
-      HPS_CHECK(mark_end());
-      if (deposit_ok) {
-        HPS_CHECK(mark(3));
-        HPS_CHECK(launch_deposit(b->d, b->st, sc->dev, sc->bins, sc->cfg.nx, sc->cfg.nz, s));
-        HPS_CHECK(mark_end());
-      }
-      HPS_CHECK(mark(4));
-      HPS_CHECK(launch_field(sc->dev, sc->bins, sc->fld1, sc->fld2, sc->cfg.nx, sc->cfg.nz, s));
-      HPS_CHECK(mark_end());
+      launches.push_back(l);
+      Launch g = l;
+      g.kind = 2;
+      launches.push_back(g);
+      if (deposit_ok) { g.kind = 3; launches.push_back(g); }
+      g.kind = 4;
+      launches.push_back(g);
     }
+  }
+  auto issue = [&](const Launch& l, hipStream_t st) -> hipError_t {
+    hps_spch* sc = m->sc;
+    switch (l.kind) {
+      case 0: return launch_segment(l.seg, st);
+      case 1: return launch_phase(l.ph, st);
+      case 2: return launch_greens(sc->dev, sc->fld1, sc->cfg.nx, sc->cfg.nz, b->n, b->mass, st);
+      case 3: return launch_deposit(b->d, b->st, sc->dev, sc->bins, sc->cfg.nx, sc->cfg.nz, st);
+      default: return launch_field(sc->dev, sc->bins, sc->fld1, sc->fld2, sc->cfg.nx, sc->cfg.nz, st);
+    }
+  };
+
+  // profiling: HIP events around every launch (eager launches only)
+  std::vector<std::pair<int, size_t>> marks;
+  size_t nev = 0;
+  const bool eager = m->profile || host_steps || !m->use_graph;
+  HPS_CHECK(hipEventRecord(m->ev0, s));
+  if (eager) {
+    for (const Launch& l : launches) {
+      if (l.kind == 5) {
+        // the phases of a frequency change are not known at compile time of
+        // later launches' arguments: rebuild them after the host step
+        if (change_frequency(b, l.new_freq)) return -1;
+        continue;
+      }
+      if (m->profile) {
+        while (m->evpool.size() < nev + 2) {
+          hipEvent_t e;
+          HPS_CHECK(hipEventCreate(&e));
+          m->evpool.push_back(e);
+        }
+        marks.push_back({l.kind, nev});
+        HPS_CHECK(hipEventRecord(m->evpool[nev], s));
+      }
+      Launch lc = l;
+      if (l.kind == 1) { lc.ph.current = b->current; lc.ph.freq = b->freq; }
+      HPS_CHECK(issue(lc, s));
+      if (m->profile) {
+        HPS_CHECK(hipEventRecord(m->evpool[nev + 1], s));
+        nev += 2;
+      }
+    }
+  } else {
+    std::vector<uint8_t> sig;
+    const auto put = [&](const void* p, size_t nb) {
+      const uint8_t* q = (const uint8_t*)p;
+      sig.insert(sig.end(), q, q + nb);
+    };
+    for (const Launch& l : launches) {
+      put(&l.kind, sizeof l.kind);
+      if (l.kind == 0) put(&l.seg, sizeof l.seg);
+      else put(&l.ph, sizeof l.ph);
+    }
+    put(c.ops.data(), c.ops.size() * sizeof(Op));
+    if (!m->graph_exec || sig != m->graph_sig) {
+      if (m->graph_exec) { (void)hipGraphExecDestroy(m->graph_exec); m->graph_exec = nullptr; }
+      if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+      HPS_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
+      hipError_t err = hipSuccess;
+      for (const Launch& l : launches) {
+        err = issue(l, s);
+        if (err != hipSuccess) break;
+      }
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(s, &g);
+      if (err != hipSuccess) return fail(std::string("graph capture: ") + hipGetErrorString(err));
+      HPS_CHECK(e2);
+      m->graph = g;
+      HPS_CHECK(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0));
+      m->graph_sig.swap(sig);
+    } else {
+      m->graph_replays += 1;
+    }
+    HPS_CHECK(hipGraphLaunch(m->graph_exec, s));
   }
   HPS_CHECK(hipEventRecord(m->ev1, s));
   HPS_CHECK(hipEventSynchronize(m->ev1));
@@ -1167,6 +1253,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   m->last_ms = ms;
   for (const auto& mk : marks) {
     float t = 0;
+    (void)nev;
     HPS_CHECK(hipEventElapsedTime(&t, m->evpool[mk.second], m->evpool[mk.second + 1]));
     m->prof_ms[mk.first] += t;
     m->prof_n[mk.first] += 1;

@@ -25,69 +25,95 @@ constexpr int kBlock = 256;
 
 // --- canonical reductions -------------------------------------------------
 
-// Halving tree over the 256 lane values of Q quantities stored in red[q*256+t].
+// Canonical halving tree over the 256 lane values of Q sums (levels 128 and
+// 64 through LDS, 32..1 with wavefront shuffles: lane i adds lane i + o at
+// every level, the same pairing as the serial definition), plus an integer
+// count and a max carried alongside.  The results are valid in thread 0;
+// red must hold Q*256 doubles, ured 256 words.
 template <int Q>
-__device__ __forceinline__ void lane_tree(double* red, int t)
+__device__ __forceinline__ void block_reduce(double (&v)[Q], uint32_t& cnt, double& mx, double* red,
+                                             uint32_t* ured, double* mred, int t)
 {
 #pragma unroll
-  for (int o = kBlock / 2; o >= 1; o >>= 1) {
-    __syncthreads();
-    if (t < o) {
+  for (int q = 0; q < Q; ++q) red[q * kBlock + t] = v[q];
+  ured[t] = cnt;
+  mred[t] = mx;
+  __syncthreads();
+  if (t < 128) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) red[q * kBlock + t] = red[q * kBlock + t] + red[q * kBlock + t + o];
+    for (int q = 0; q < Q; ++q) red[q * kBlock + t] = red[q * kBlock + t] + red[q * kBlock + t + 128];
+    ured[t] += ured[t + 128];
+    mred[t] = ::fmax(mred[t], mred[t + 128]);
+  }
+  __syncthreads();
+  if (t < 64) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = red[q * kBlock + t] + red[q * kBlock + t + 64];
+    cnt = ured[t] + ured[t + 64];
+    mx = ::fmax(mred[t], mred[t + 64]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) v[q] = v[q] + __shfl_down(v[q], o, 64);
+      cnt += __shfl_down(cnt, o, 64);
+      mx = ::fmax(mx, __shfl_down(mx, o, 64));
     }
   }
   __syncthreads();
 }
 
-__device__ __forceinline__ uint32_t block_count(uint32_t v, uint32_t* ured, int t)
+// Cross-chunk trees of Q quantities stored P[q * mpad + c] (zero padded to
+// mpad, a power of two) and the sum of the integer counts C[0, nch).  Thread
+// t first folds its entries t + 256k in registers (the upper levels of the
+// halving tree), then one block tree finishes.  Every thread gets the
+// results.  mpad <= 4096 (16 entries per thread).
+template <int Q>
+__device__ void final_reduce(const double* P, const uint32_t* C, uint32_t mpad, uint32_t nch,
+                             double* red, uint32_t* ured, double* mred, int t, double (&out)[Q],
+                             uint32_t& cnt_out)
 {
-  ured[t] = v;
-  for (int o = kBlock / 2; o >= 1; o >>= 1) {
-    __syncthreads();
-    if (t < o) ured[t] += ured[t + o];
+  const uint32_t K = mpad > (uint32_t)kBlock ? mpad / kBlock : 1;
+  double v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    double r[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t c = t + kBlock * k;
+      r[k] = ((uint32_t)k < K && c < mpad) ? P[q * mpad + c] : 0.0;
+    }
+    if (K >= 16) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) r[k] = r[k] + r[k + 8];
+    }
+    if (K >= 8) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) r[k] = r[k] + r[k + 4];
+    }
+    if (K >= 4) {
+      r[0] = r[0] + r[2];
+      r[1] = r[1] + r[3];
+    }
+    if (K >= 2) r[0] = r[0] + r[1];
+    v[q] = r[0];
+  }
+  uint32_t cnt = 0;
+  if (C)
+    for (uint32_t c = t; c < nch; c += kBlock) cnt += C[c];
+  double mx = 0.0;
+  block_reduce<Q>(v, cnt, mx, red, ured, mred, t);
+  __shared__ double bcast[kMaxQ];
+  __shared__ uint32_t bcnt;
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) bcast[q] = v[q];
+    bcnt = cnt;
   }
   __syncthreads();
-  uint32_t r = ured[0];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) out[q] = bcast[q];
+  cnt_out = bcnt;
   __syncthreads();
-  return r;
-}
-
-__device__ __forceinline__ double block_max(double v, double* red, int t)
-{
-  red[t] = v;
-  for (int o = kBlock / 2; o >= 1; o >>= 1) {
-    __syncthreads();
-    if (t < o) red[t] = ::fmax(red[t], red[t + o]);
-  }
-  __syncthreads();
-  double r = red[0];
-  __syncthreads();
-  return r;
-}
-
-// Halving tree over P[0, mpad) (zero padded), done by one workgroup.  The
-// levels above 256 go through scratch so that P itself is left intact.
-__device__ double chunk_tree(const double* P, double* scratch, uint32_t mpad, double* red, int t)
-{
-  const double* src = P;
-  for (uint32_t o = mpad / 2; o >= (uint32_t)kBlock; o >>= 1) {
-    for (uint32_t c = t; c < o; c += kBlock) scratch[c] = src[c] + src[c + o];
-    __syncthreads();
-    src = scratch;
-  }
-  red[t] = (uint32_t)t < mpad ? src[t] : 0.0;
-  lane_tree<1>(red, t);
-  double r = red[0];
-  __syncthreads();
-  return r;
-}
-
-__device__ uint32_t chunk_count(const uint32_t* C, uint32_t nch, uint32_t* ured, int t)
-{
-  uint32_t s = 0;
-  for (uint32_t c = t; c < nch; c += kBlock) s += C[c];
-  return block_count(s, ured, t);
 }
 
 // Returns true in every thread of the last workgroup to finish.
@@ -214,11 +240,18 @@ __device__ __forceinline__ void kick(Particle& p, const KickConst& k, const doub
 
 // Per-chunk moments of the SCHEFF statistics (quantity order in partials):
 //   0 phi, 1 x, 2 x^2, 3 y, 4 y^2, 5 w   over (loss + lloss) == 0
-// plus the good count and max(phi) over all particles.
+// plus the good count and the largest phi among particles not yet flagged
+// as longitudinally lost (the chunks a new lloss flag can appear in).
 constexpr int kScQ = 6;
 
-__device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, double* red,
-                               uint32_t* ured, const Partials& pr, const Particle* p, bool fresh)
+struct RedSmem {
+  double red[kMaxQ * kBlock];
+  double mred[kBlock];
+  uint32_t ured[kBlock];
+};
+
+__device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem& sm,
+                               const Partials& pr, const Particle* p, bool fresh)
 {
   double acc[kScQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   uint32_t cnt = 0;
@@ -231,7 +264,7 @@ __device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, double* r
     if (i < b.n) {
       if (fresh) { phi = b.phi[i]; x = b.x[i]; y = b.y[i]; w = b.w[i]; loss = b.loss[i]; lloss = b.lloss[i]; }
       else { phi = p[j].phi; x = p[j].x; y = p[j].y; w = p[j].w; loss = p[j].loss; lloss = p[j].lloss; }
-      mx = ::fmax(mx, phi);
+      if (lloss == 0) mx = ::fmax(mx, phi);
     }
     const bool g = (i < b.n) && is_good(loss, lloss);
     const double vphi = g ? phi : 0.0, vx = g ? x : 0.0, vy = g ? y : 0.0, vw = g ? w : 0.0;
@@ -243,48 +276,39 @@ __device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, double* r
     acc[5] = acc[5] + vw;
     cnt += g ? 1u : 0u;
   }
-#pragma unroll
-  for (int q = 0; q < kScQ; ++q) red[q * kBlock + t] = acc[q];
-  lane_tree<kScQ>(red, t);
+  block_reduce<kScQ>(acc, cnt, mx, sm.red, sm.ured, sm.mred, t);
   if (t == 0) {
 #pragma unroll
-    for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = red[q * kBlock];
+    for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = acc[q];
+    pr.cnt[c] = cnt;
+    pr.pmax[c] = mx;
   }
-  __syncthreads();
-  const uint32_t cs = block_count(cnt, ured, t);
-  const double ms = block_max(mx, red, t);
-  if (t == 0) { pr.cnt[c] = cs; pr.pmax[c] = ms; }
 }
 
 // Aperture statistics (UpdateLoss + UpdateAvgXY, simulation_engine.cu:311-312):
 // count of loss != 0, sums of x and y over loss == 0.
-__device__ void aper_chunk_stats(const BeamArrays& b, uint32_t c, int t, double* red,
-                                 uint32_t* ured, const Partials& pr, const Particle* p)
+__device__ void aper_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem& sm,
+                                 const Partials& pr, const Particle* p)
 {
-  double sx = 0.0, sy = 0.0;
+  double acc[2] = {0.0, 0.0};
   uint32_t nl = 0;
 #pragma unroll
   for (int j = 0; j < (int)kPerLane; ++j) {
     const uint32_t i = c * kChunk + j * kBlock + t;
     const bool v = i < b.n;
     const bool ok = v && p[j].loss == 0;
-    sx = sx + (ok ? p[j].x : 0.0);
-    sy = sy + (ok ? p[j].y : 0.0);
+    acc[0] = acc[0] + (ok ? p[j].x : 0.0);
+    acc[1] = acc[1] + (ok ? p[j].y : 0.0);
     nl += (v && p[j].loss != 0) ? 1u : 0u;
   }
-  red[t] = sx;
-  red[kBlock + t] = sy;
-  lane_tree<2>(red, t);
-  if (t == 0) { pr.sums[c] = red[0]; pr.sums[pr.mpad + c] = red[kBlock]; }
-  __syncthreads();
-  const uint32_t cs = block_count(nl, ured, t);
-  if (t == 0) pr.cnt[c] = cs;
+  double mx = 0.0;
+  block_reduce<2>(acc, nl, mx, sm.red, sm.ured, sm.mred, t);
+  if (t == 0) { pr.sums[c] = acc[0]; pr.sums[pr.mpad + c] = acc[1]; pr.cnt[c] = nl; }
 }
 
 __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
 {
-  __shared__ double red[kScQ * kBlock];
-  __shared__ uint32_t ured[kBlock];
+  __shared__ RedSmem sm;
   __shared__ uint32_t rescan_list[64];
   __shared__ uint32_t rescan_n;
   const int t = threadIdx.x;
@@ -336,19 +360,18 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
   if (a.stats == STAT_NONE) return;
 
   const Partials& pr = a.pr;
-  double* scratch = pr.sums + kMaxQ * pr.mpad;
   if (a.stats == STAT_APERTURE) {
-    aper_chunk_stats(b, c, t, red, ured, pr, p);
+    aper_chunk_stats(b, c, t, sm, pr, p);
     if (!last_block(pr.ticket, t)) return;
-    const uint32_t nloss = chunk_count(pr.cnt, pr.nch, ured, t);
-    const double sx = chunk_tree(pr.sums, scratch, pr.mpad, red, t);
-    const double sy = chunk_tree(pr.sums + pr.mpad, scratch, pr.mpad, red, t);
+    double s[2];
+    uint32_t nloss;
+    final_reduce<2>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, nloss);
     if (t == 0) {
       hps_beam_stats* st = a.st;
       st->num_loss = nloss;
       const double surv = (double)(uint32_t)(b.n - nloss);
-      st->x_avg = sx / surv;
-      st->y_avg = sy / surv;
+      st->x_avg = s[0] / surv;
+      st->y_avg = s[1] / surv;
       *pr.ticket = 0;
     }
     return;
@@ -357,16 +380,18 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
   // STAT_SC: moments after the maps, then (last block) the longitudinal-loss
   // update of Beam::UpdateLongitudinalLoss (beam.cpp:776-782) and the good
   // moments of beam.cpp:1008-1014.
-  sc_chunk_stats(b, c, t, red, ured, pr, p, false);
+  sc_chunk_stats(b, c, t, sm, pr, p, false);
   if (!last_block(pr.ticket, t)) return;
   hps_beam_stats* st = a.st;
-  uint32_t g1 = chunk_count(pr.cnt, pr.nch, ured, t);
-  const double sphi1 = chunk_tree(pr.sums, scratch, pr.mpad, red, t);
-  const double avg1 = g1 > 0 ? sphi1 / (double)g1 : -99999;
+  double s[kScQ];
+  uint32_t g1;
+  final_reduce<1>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t,
+                  *reinterpret_cast<double(*)[1]>(s), g1);
+  const double avg1 = g1 > 0 ? s[0] / (double)g1 : -99999;
   const double thr = avg1 + TWOPI + PI;
-  // Chunks holding a particle beyond the threshold get their lloss flags set
-  // and their partials recomputed (usually there are none).  A rescanned
-  // chunk's max is cleared so that the scan terminates.
+  // Chunks where a not-yet-flagged particle is beyond the threshold get their
+  // lloss flags set and their partials recomputed (rare: once per particle).
+  bool any = false;
   for (uint32_t it = 0; it < pr.nch / 64 + 2; ++it) {
     if (t == 0) rescan_n = 0;
     __syncthreads();
@@ -379,6 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
     __syncthreads();
     const uint32_t nfound = rescan_n;
     if (nfound == 0) break;
+    any = true;
     const uint32_t take = nfound < 64 ? nfound : 64;
     for (uint32_t r = 0; r < take; ++r) {
       const uint32_t cc = rescan_list[r];
@@ -388,15 +414,21 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
         if (i < b.n && b.phi[i] > thr) b.lloss[i] = 1;
       }
       __syncthreads();
-      sc_chunk_stats(b, cc, t, red, ured, pr, p, true);
-      if (t == 0) pr.pmax[cc] = -__builtin_huge_val();
+      sc_chunk_stats(b, cc, t, sm, pr, p, true);
       __syncthreads();
     }
   }
-  const uint32_t g2 = chunk_count(pr.cnt, pr.nch, ured, t);
-  double s[kScQ];
+  uint32_t g2 = g1;
+  if (any) {
+    final_reduce<kScQ>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, g2);
+  } else {
+    double r5[kScQ - 1];
+    uint32_t dummy;
+    final_reduce<kScQ - 1>(pr.sums + pr.mpad, nullptr, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred,
+                           t, r5, dummy);
 #pragma unroll
-  for (int q = 0; q < kScQ; ++q) s[q] = chunk_tree(pr.sums + q * pr.mpad, scratch, pr.mpad, red, t);
+    for (int q = 1; q < kScQ; ++q) s[q] = r5[q - 1];
+  }
   if (t == 0) {
     st->num_good = g2;
     st->phi_avg_good = g2 > 0 ? s[0] / (double)g2 : -99999;
@@ -415,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
 {
-  __shared__ double red[3 * kBlock];
+  __shared__ RedSmem sm;
   const int t = threadIdx.x;
   const uint32_t c = blockIdx.x;
   const BeamArrays& b = a.b;
@@ -423,36 +455,36 @@ __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
   hps_beam_stats* st = a.st;
   if (a.do_sums) {
     const double ref = st->phi_avg_good;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < (int)kPerLane; ++j) {
       const uint32_t i = c * kChunk + j * kBlock + t;
       double pr_ = 0.0;
       uint32_t loss = 1, lloss = 0;
-      if (i < b.n) { pr_ = relative_phi(b.phi[i], ref); loss = b.loss[i]; lloss = b.lloss[i]; }
+      if (i < b.n) { loss = b.loss[i]; lloss = b.lloss[i]; pr_ = relative_phi(b.phi[i], ref); }
       const bool tg = (i < b.n) && loss == 0;
       const bool g = (i < b.n) && is_good(loss, lloss);
-      s0 = s0 + (tg ? pr_ : 0.0);
+      acc[0] = acc[0] + (tg ? pr_ : 0.0);
       const double vg = g ? pr_ : 0.0;
-      s1 = s1 + vg;
-      s2 = s2 + vg * vg;
+      acc[1] = acc[1] + vg;
+      acc[2] = acc[2] + vg * vg;
     }
-    red[t] = s0; red[kBlock + t] = s1; red[2 * kBlock + t] = s2;
-    lane_tree<3>(red, t);
+    uint32_t cnt = 0;
+    double mx = 0.0;
+    block_reduce<3>(acc, cnt, mx, sm.red, sm.ured, sm.mred, t);
     if (t == 0) {
-      pr.sums[c] = red[0];
-      pr.sums[pr.mpad + c] = red[kBlock];
-      pr.sums[2 * pr.mpad + c] = red[2 * kBlock];
+      pr.sums[c] = acc[0];
+      pr.sums[pr.mpad + c] = acc[1];
+      pr.sums[2 * pr.mpad + c] = acc[2];
     }
     if (!last_block(pr.ticket, t)) return;
-    double* scratch = pr.sums + kMaxQ * pr.mpad;
-    const double sr = chunk_tree(pr.sums, scratch, pr.mpad, red, t);
-    const double sg = chunk_tree(pr.sums + pr.mpad, scratch, pr.mpad, red, t);
-    const double sg2 = chunk_tree(pr.sums + 2 * pr.mpad, scratch, pr.mpad, red, t);
+    double s[3];
+    uint32_t dummy;
+    final_reduce<3>(pr.sums, nullptr, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, dummy);
     if (t == 0) {
-      st->phi_avg_r = sr / (double)(uint32_t)(b.n - st->num_loss);
+      st->phi_avg_r = s[0] / (double)(uint32_t)(b.n - st->num_loss);
       const double gd = (double)st->num_good;
-      const double avg = sg / gd, avg2 = sg2 / gd;
+      const double avg = s[1] / gd, avg2 = s[2] / gd;
       st->phi_sig_good = ::sqrt(avg2 - avg * avg);
       *pr.ticket = 0;
     }
@@ -474,37 +506,37 @@ __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
     const double sigr = ::sqrt(sigx * sigx + sigy * sigy);
     const double cratior = sigr * sigz;
     const double cratioz = sigz * sigz * gamma * gamma;
-    hps_spch_state& s = sc->st;
-    const double cratior_old = s.sigr_old * s.sigz_old;
-    const double cratioz_old = s.sigz_old * s.sigz_old * s.gm_old * s.gm_old;
+    hps_spch_state& ss = sc->st;
+    const double cratior_old = ss.sigr_old * ss.sigz_old;
+    const double cratioz_old = ss.sigz_old * ss.sigz_old * ss.gm_old * ss.gm_old;
     sc->freq = a.freq;
     sc->beta = beta;
     sc->frac = a.frac;
     sc->current = a.current;
-    s.kick_count += 1;
+    ss.kick_count += 1;
     if (cratior_old == 0.0 || cratioz_old == 0.0 ||
-        __builtin_fabs(cratior - cratior_old) / cratior_old > s.remesh_val ||
-        __builtin_fabs(cratioz - cratioz_old) / cratioz_old > s.remesh_val) {
-      s.sigr_old = sigr;
-      s.sigz_old = sigz;
-      s.gm_old = gamma;
-      s.good_cnt_old = good_cnt;
-      s.remesh_count += 1;
+        __builtin_fabs(cratior - cratior_old) / cratior_old > ss.remesh_val ||
+        __builtin_fabs(cratioz - cratioz_old) / cratioz_old > ss.remesh_val) {
+      ss.sigr_old = sigr;
+      ss.sigz_old = sigz;
+      ss.gm_old = gamma;
+      ss.good_cnt_old = good_cnt;
+      ss.remesh_count += 1;
       // UpdateScheffMeshSizeKernel via UpdateMeshKernelCall: the reference
       // compares the device pointers x_sig_good > y_sig_good (space_charge.cu:131);
       // x_sig_good is allocated first (beam.cu:51,54), so sigma_y is used.
       const double bl = beta * CLIGHT / a.freq;
-      s.dr = 3.0 * st->y_sig_good / s.nx;
-      s.dz = 3.0 * bl * st->phi_sig_good / (s.nz * PI);
-      s.num_on_mesh = 0;
+      ss.dr = 3.0 * st->y_sig_good / ss.nx;
+      ss.dz = 3.0 * bl * st->phi_sig_good / (ss.nz * PI);
+      ss.num_on_mesh = 0;
       sc->ratio_r = sc->ratio_z = sc->ratio_q = sc->ratio_g = 1.0;
       sc->remesh = 1;
       sc->valid = 1;
     } else {
-      sc->ratio_r = sigr / s.sigr_old;
-      sc->ratio_z = sigz / s.sigz_old;
-      sc->ratio_q = s.good_cnt_old ? (double)(good_cnt / s.good_cnt_old) : 0.0;
-      sc->ratio_g = gamma / s.gm_old;
+      sc->ratio_r = sigr / ss.sigr_old;
+      sc->ratio_z = sigz / ss.sigz_old;
+      sc->ratio_q = ss.good_cnt_old ? (double)(good_cnt / ss.good_cnt_old) : 0.0;
+      sc->ratio_g = gamma / ss.gm_old;
       sc->remesh = 0;
     }
     remesh = sc->remesh;
@@ -639,8 +671,13 @@ __global__ __launch_bounds__(kBlock) void k_deposit(BeamArrays b, const hps_beam
     const unsigned long long v = hist[i];
     if (v != 0ull) atomicAdd((unsigned long long*)&bins[i], v);
   }
-  const uint32_t cs = block_count(count, ured, t);
-  if (t == 0 && cs) atomicAdd(&sc->st.num_on_mesh, cs);
+  ured[t] = count;
+  __syncthreads();
+  for (int o = kBlock / 2; o >= 1; o >>= 1) {
+    if (t < o) ured[t] += ured[t + o];
+    __syncthreads();
+  }
+  if (t == 0 && ured[0]) atomicAdd(&sc->st.num_on_mesh, ured[0]);
 }
 
 // Field table E(zp, rp) = sum over bins of charge * Green's value,
@@ -691,8 +728,7 @@ __global__ __launch_bounds__(kBlock) void k_field(const ScDev* sc, const int64_t
 __global__ __launch_bounds__(kBlock) void k_sums(BeamArrays b, const double* pa, const double* pb,
                                                  int filter, Partials pr)
 {
-  __shared__ double red[5 * kBlock];
-  __shared__ uint32_t ured[kBlock];
+  __shared__ RedSmem sm;
   const int t = threadIdx.x;
   const uint32_t c = blockIdx.x;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -714,29 +750,32 @@ __global__ __launch_bounds__(kBlock) void k_sums(BeamArrays b, const double* pa,
     acc[3] = acc[3] + vb * vb;
     acc[4] = acc[4] + va * vb;
   }
-#pragma unroll
-  for (int q = 0; q < 5; ++q) red[q * kBlock + t] = acc[q];
-  lane_tree<5>(red, t);
+  // counts are packed 3 x 11 bits (at most 1024 per chunk each)
+  uint32_t packed = nloss | (ngood << 11) | (nlloss << 22);
+  double mx = 0.0;
+  block_reduce<5>(acc, packed, mx, sm.red, sm.ured, sm.mred, t);
   if (t == 0) {
 #pragma unroll
-    for (int q = 0; q < 5; ++q) pr.sums[q * pr.mpad + c] = red[q * kBlock];
+    for (int q = 0; q < 5; ++q) pr.sums[q * pr.mpad + c] = acc[q];
+    pr.cnt[c] = packed & 0x7ffu;
+    pr.cnt[pr.mpad + c] = (packed >> 11) & 0x7ffu;
+    pr.cnt[2 * pr.mpad + c] = packed >> 22;
   }
-  __syncthreads();
-  const uint32_t c0 = block_count(nloss, ured, t);
-  const uint32_t c1 = block_count(ngood, ured, t);
-  const uint32_t c2 = block_count(nlloss, ured, t);
-  if (t == 0) { pr.cnt[c] = c0; pr.cnt[pr.mpad + c] = c1; pr.cnt[2 * pr.mpad + c] = c2; }
   if (!last_block(pr.ticket, t)) return;
-  double* scratch = pr.sums + kMaxQ * pr.mpad;
-  for (int q = 0; q < 5; ++q) {
-    const double s = chunk_tree(pr.sums + q * pr.mpad, scratch, pr.mpad, red, t);
-    if (t == 0) pr.out[q] = s;
+  double s[5];
+  uint32_t c0, c1, c2;
+  final_reduce<5>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, c0);
+  double d1[1], d2[1];
+  final_reduce<1>(pr.sums, pr.cnt + pr.mpad, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, d1, c1);
+  final_reduce<1>(pr.sums, pr.cnt + 2 * pr.mpad, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, d2, c2);
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) pr.out[q] = s[q];
+    pr.out_cnt[0] = c0;
+    pr.out_cnt[1] = c1;
+    pr.out_cnt[2] = c2;
+    *pr.ticket = 0;
   }
-  for (int q = 0; q < 3; ++q) {
-    const uint32_t s = chunk_count(pr.cnt + q * pr.mpad, pr.nch, ured, t);
-    if (t == 0) pr.out_cnt[q] = s;
-  }
-  if (t == 0) *pr.ticket = 0;
 }
 
 __global__ void k_relative_phi(BeamArrays b, const double* ref)
@@ -831,9 +870,9 @@ hipError_t launch_deposit(BeamArrays b, const hps_beam_stats* st, ScDev* sc, int
                           uint32_t nr, uint32_t nz, hipStream_t s)
 {
   const size_t lds = (size_t)nr * nz * sizeof(unsigned long long);
-  uint32_t grid = (b.n + 2047) / 2048;
+  uint32_t grid = (b.n + 4095) / 4096;
   if (grid < 1) grid = 1;
-  if (grid > 1024) grid = 1024;
+  if (grid > 256) grid = 256;
   hipLaunchKernelGGL(k_deposit, dim3(grid), dim3(kBlock), lds, s, b, st, sc, bins);
   return hipGetLastError();
 }

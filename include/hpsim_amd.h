@@ -183,6 +183,13 @@ int hps_sim_set_space_charge(hps_sim* m, int on);
  * the reference only times whole simulate() calls (timer.cpp:4-19). */
 int hps_sim_set_profiling(hps_sim* m, int on);
 int hps_sim_get_profile(hps_sim* m, double* ms, uint32_t* launches);
+/* Launch the kernel sequence of simulate() as a HIP graph (default on),
+ * re-captured whenever any kernel argument or map differs from the cached
+ * one; profiling and beam-frequency changes use eager launches.  No
+ * reference counterpart (the reference launches every kernel from the host
+ * loop, simulation_engine.cu:125-373). */
+int hps_sim_set_graph(hps_sim* m, int on);
+int hps_sim_graph_replays(hps_sim* m, uint32_t* out);
 /* Device time of the last simulate() in ms (the reference prints it,
  * simulation_engine.cpp:84-92). */
 int hps_sim_last_time_ms(hps_sim* m, double* out);

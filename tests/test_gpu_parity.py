@@ -66,3 +66,143 @@ def test_tracking_space_charge_bit_exact(num, periods):
     assert err is None, err + f"\nspch gpu {gs}\nspch oracle {os_}"
     for k in ("remesh_count", "kick_count", "num_on_mesh", "dr", "dz", "sigr_old", "sigz_old"):
         assert gs[k] == os_[k], (k, gs[k], os_[k])
+
+
+def test_graph_replays_match_eager_and_oracle():
+    """restore + simulate three times on one simulator (the space-charge
+    state carries over between calls): eager launches, then the captured
+    HIP graph replayed with eager work in between; every call must equal
+    the oracle's same sequence."""
+    from hpsim_amd import HPSim, configs as cfg
+    from oracle import oracle as O
+    wl = H.workload(0, num=3000, periods=3, sc=True)
+    elems = cfg.build_lattice(wl.lattice)
+    dist = cfg.make_distribution(wl.beam)
+    b = wl.beam
+    ob = O.Beam(b.num, b.mass, b.charge, b.current)
+    ob.set_distribution(*dist)
+    ob.set_frequency(b.freq)
+    ob.set_ref_w(b.w0)
+    ob.save_initial_beam()
+    osim = O.Simulator(ob, elems, O.SpaceCharge(wl.nr, wl.nz, wl.interval))
+    refs = []
+    for _ in range(3):
+        ob.restore_initial_beam()
+        osim.simulate()
+        refs.append(ob.arrays())
+    for graph in (False, True):
+        beam = HPSim.Beam(mass=b.mass, charge=b.charge, current=b.current, num=b.num)
+        beam.set_distribution(*dist)
+        beam.set_frequency(b.freq)
+        beam.set_ref_w(b.w0)
+        beam.save_initial_beam()
+        sim = HPSim.Simulator(beam, HPSim.BeamLine(elements=elems),
+                              HPSim.SpaceCharge(nr=wl.nr, nz=wl.nz, interval=wl.interval))
+        sim.set_graph(graph)
+        for rep in range(3):
+            beam.restore_initial_beam()
+            sim.simulate()
+            HPSim.probe_math("exp", np.linspace(0, 1, 1000))   # eager work between replays
+            err = H.compare_arrays(beam.arrays(), refs[rep], f"graph={graph} rep={rep} ")
+            assert err is None, err
+        if graph:
+            assert sim.graph_replays() == 2
+
+
+def test_graph_replay_same_simulator():
+    """Same simulator, restore + simulate three times: the second and third
+    calls replay the captured graph."""
+    from hpsim_amd import HPSim, configs as cfg
+    wl = H.workload(0, num=2048, periods=2, sc=False)
+    elems = cfg.build_lattice(wl.lattice)
+    dist = cfg.make_distribution(wl.beam)
+    ob, _, _ = H.run_oracle(wl, elems=elems, dist=dist)
+    ref = ob.arrays()
+    b = wl.beam
+    beam = HPSim.Beam(mass=b.mass, charge=b.charge, current=b.current, num=b.num)
+    beam.set_distribution(*dist)
+    beam.set_frequency(b.freq)
+    beam.set_ref_w(b.w0)
+    beam.save_initial_beam()
+    sim = HPSim.Simulator(beam, HPSim.BeamLine(elements=elems))
+    for k in range(3):
+        beam.restore_initial_beam()
+        sim.reset()
+        sim.simulate()
+        err = H.compare_arrays(beam.arrays(), ref, f"call {k} ")
+        assert err is None, err
+    assert sim.graph_replays() == 2
+
+
+def test_longitudinal_loss_rescan_bit_exact():
+    """A 70-degree phase spread pushes particles out of the bucket, so the
+    lloss flags and the chunk rescan path are exercised."""
+    import math
+    wl = H.workload(0, num=5000, periods=6, sc=True)
+    wl.beam.sig_phi = math.radians(70.0)
+    ob, osc, _ = H.run_oracle(wl)
+    gb, gsc, _ = H.run_gpu(wl)
+    oa = ob.arrays()
+    assert (oa["lloss"] != 0).sum() > 10
+    err = H.compare_arrays(gb.arrays(), oa)
+    assert err is None, err
+
+
+def test_aperture_elements_and_rotation_steerer():
+    from hpsim_amd import configs as cfg
+    wl = H.workload(0, num=4096, periods=3, sc=True)
+    elems = cfg.build_lattice(wl.lattice)
+    extra = [cfg.Element("AC", "ApertureC", aperture=0.003, aperture_in=True),
+             cfg.Element("ROT", "Rotation", angle=0.3),
+             cfg.Element("ST", "Steerer", bl_h=2e-4, bl_v=1e-4),
+             cfg.Element("AR", "ApertureR", aper_xl=0.004, aper_xr=0.002, aper_yt=0.003,
+                         aper_yb=0.0025, aperture_in=True),
+             cfg.Element("DG", "Diagnostics"),
+             cfg.Element("SC", "SpchComp", fraction=0.5)]
+    elems = elems[:5] + extra + elems[5:]
+    dist = cfg.make_distribution(wl.beam)
+    ob, _, _ = H.run_oracle(wl, elems=elems, dist=dist)
+    gb, _, _ = H.run_gpu(wl, elems=elems, dist=dist)
+    oa = ob.arrays()
+    assert (oa["loss"] == 5).sum() > 0 and (oa["loss"] == 8).sum() > 0
+    err = H.compare_arrays(gb.arrays(), oa)
+    assert err is None, err
+
+
+def test_getters_match_oracle():
+    wl = H.workload(0, num=3000, periods=3, sc=True)
+    ob, _, _ = H.run_oracle(wl)
+    gb, _, _ = H.run_gpu(wl)
+    pairs = [(lambda b: b.get_loss_num("t"), lambda o: o.get_loss_num("t")),
+             (lambda b: b.get_emittance_x(), lambda o: o.get_emittance("x")),
+             (lambda b: b.get_emittance_y(), lambda o: o.get_emittance("y")),
+             (lambda b: b.get_emittance_z(), lambda o: o.get_emittance("z")),
+             (lambda b: b.get_avg_x(), lambda o: o.get_avg("x")),
+             (lambda b: b.get_avg_y("longitudinal_good"), lambda o: o.get_avg("y", True)),
+             (lambda b: b.get_avg_phi("relative"), lambda o: o.get_avg("phi_r")),
+             (lambda b: b.get_avg_w("longitudinal_good"), lambda o: o.get_avg("w", True)),
+             (lambda b: b.get_sig_x(), lambda o: o.get_sig("x")),
+             (lambda b: b.get_sig_phi("longitudinal_good"), lambda o: o.get_sig("phi", True)),
+             (lambda b: b.get_sig_w(), lambda o: o.get_sig("w")),
+             (lambda b: b.get_loss_num("l"), lambda o: o.get_loss_num("l")),
+             (lambda b: b.get_loss_num("all"), lambda o: o.get_loss_num("all"))]
+    for k, (g, o) in enumerate(pairs):
+        gv, ov = g(gb), o(ob)
+        assert gv == ov or (gv != gv and ov != ov), (k, gv, ov)
+    gs, os_ = gb.stats(), ob.stats()
+    for key, v in os_.items():
+        assert gs[key] == v or (v != v and gs[key] != gs[key]), (key, gs[key], v)
+
+
+@pytest.mark.parametrize("num", [1, 2, 3, 5, 257])
+def test_tiny_beams_bit_exact(num):
+    """Edge cases of the launch geometry: N = 1 skips quad/gap maps, N <= 2
+    skips SCHEFF statistics and kicks, N = 3..4 kicks nobody (Q3)."""
+    wl = H.workload(0, num=num, periods=2, sc=True)
+    ob, osc, _ = H.run_oracle(wl)
+    gb, gsc, _ = H.run_gpu(wl)
+    err = H.compare_arrays(gb.arrays(), ob.arrays())
+    assert err is None, err
+    gs, os_ = gsc.state(), osc.state()
+    for k in ("remesh_count", "kick_count", "num_on_mesh"):
+        assert gs[k] == os_[k], (k, gs[k], os_[k])
