@@ -10,11 +10,12 @@ elements x steps / wall time of the timed steps.
 The JSON line also carries
   roofline:      HBM roofline of the fused element-map kernel (k_segment):
                  algorithmic bytes per launch / average launch time, the
-                 average measured with HIP events around every launch in the
-                 timed steps;
+                 average measured with HIP events (on the engine's stream)
+                 around every launch of one extra eager step run after the
+                 timed steps, which replay a captured HIP graph;
   cpu_baseline:  the serial FP64 oracle (oracle/liboracle.so, one core) on a
                  bounded sample of the same workload: the full beam through
-                 the first lattice period(s), timed on this host.
+                 the first 40 lattice periods (about 20 s), timed on this host.
 """
 from __future__ import annotations
 
@@ -39,7 +40,7 @@ def parse():
     p.add_argument("--num", type=int, default=None, help="override particle count")
     p.add_argument("--periods", type=int, default=None, help="override lattice periods")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-periods", type=int, default=1,
+    p.add_argument("--cpu-periods", type=int, default=40,
                    help="lattice periods of the CPU-baseline sample")
     return p.parse_args()
 
@@ -100,16 +101,22 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    sim.set_profiling(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     dt = time.perf_counter() - t0
+    replays = sim.graph_replays()
+    # one more step launched eagerly with HIP events around every kernel
+    # launch (on the engine's stream) for the per-kernel times
+    sim.set_profiling(True)
+    step()
     prof = sim.profile()
+    sim.set_profiling(False)
 
     pushes = b.num * nel * args.steps
     seg_ms, seg_n = prof["segment"]
     avg_s = seg_ms / 1e3 / max(seg_n, 1)
+    kernel_ms = {k: v[0] for k, v in prof.items()}
     achieved = SEG_BYTES_PER_PARTICLE * b.num / avg_s / 1e9
     lost = beam.get_loss_num("t")
     out = {
@@ -126,13 +133,16 @@ def main():
         "dtype": "f64",
         "data": "synthetic (Philox-seeded truncated Gaussian beam, generated lattice)",
         "config": {"workload": wl.name, "particles": b.num, "elements": nel,
-                   "mesh": f"{wl.nr}x{wl.nz}", "sc_kicks_per_step": prof["phase"][1] // args.steps,
+                   "mesh": f"{wl.nr}x{wl.nz}", "sc_kicks_per_step": prof["phase"][1],
                    "transmitted": b.num - lost},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_segment", "avg_launch_us": avg_s * 1e6,
-                     "bytes_per_launch": SEG_BYTES_PER_PARTICLE * b.num},
-        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
+                     "launches": seg_n, "bytes_per_launch": SEG_BYTES_PER_PARTICLE * b.num,
+                     "timing": "HIP events around every launch of one eager step after "
+                               "the timed steps"},
+        "kernel_ms_per_step": kernel_ms,
+        "graph_replays": replays,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_periods)

@@ -90,13 +90,18 @@ class Beam:
         ll = None if lloss is None else np.ascontiguousarray(lloss, dtype=np.uint32)
         check(_lib().hps_beam_set_distribution(self._h, *[dptr(a) for a in arrs], uptr(lo), uptr(ll)))
 
-    def set_waterbag(self, *args, **kw):
-        raise NotImplementedError("set_waterbag (std::rand sampling, beam.cpp:391-540) is not "
-                                  "provided; build distributions with set_distribution")
+    def set_waterbag(self, ax, bx, ex, ay, by, ey, az, bz, ez, phi, w, freq, seed=0):
+        """set_waterbag(alpha_x, beta_x, emittance_x, ..., phi, w, frequency[, seed]),
+        wrap_beam.cpp:125-144: PARMILA Twiss input (beta in cm, emittance in
+        cm*rad), sampled with the C library's srand/rand (seed 0 = time)."""
+        check(_lib().hps_beam_set_waterbag(self._h, *[float(v) for v in (
+            ax, bx, ex, ay, by, ey, az, bz, ez, phi, w, freq)], int(seed)))
 
-    def set_dc(self, *args, **kw):
-        raise NotImplementedError("set_dc (std::rand sampling, beam.cpp:542-) is not provided; "
-                                  "build distributions with set_distribution")
+    def set_dc(self, ax, bx, ex, ay, by, ey, dphi, sync_phi, sync_w, seed=0):
+        """set_dc(alpha_x, beta_x, emittance_x, alpha_y, beta_y, emittance_y,
+        width_phi, synchronous_phi, synchronous_w[, seed]), wrap_beam.cpp:146-165."""
+        check(_lib().hps_beam_set_dc(self._h, *[float(v) for v in (
+            ax, bx, ex, ay, by, ey, dphi, sync_phi, sync_w)], int(seed)))
 
     # -- save / restore / output ---------------------------------------------
     def save_initial_beam(self): check(_lib().hps_beam_save(self._h, 0))

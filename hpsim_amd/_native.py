@@ -58,6 +58,7 @@ class SpchState(C.Structure):
 EXPORTS = (
     "hps_last_error", "hps_set_device",
     "hps_beam_create", "hps_beam_destroy", "hps_beam_set_distribution", "hps_beam_download",
+    "hps_beam_set_waterbag", "hps_beam_set_dc",
     "hps_beam_relative_phi", "hps_beam_size", "hps_beam_set_ref_w", "hps_beam_set_ref_phi",
     "hps_beam_set_frequency", "hps_beam_get_scalar", "hps_beam_get_loss_num",
     "hps_beam_get_avg", "hps_beam_get_sig", "hps_beam_get_emittance", "hps_beam_get_stats",
@@ -89,6 +90,8 @@ def _declare(L):
         "hps_beam_destroy": (C.c_int, [P]),
         "hps_beam_set_distribution": (C.c_int, [P, dp, dp, dp, dp, dp, dp, up, up]),
         "hps_beam_download": (C.c_int, [P, dp, dp, dp, dp, dp, dp, up, up]),
+        "hps_beam_set_waterbag": (C.c_int, [P] + [d] * 12 + [u32]),
+        "hps_beam_set_dc": (C.c_int, [P] + [d] * 9 + [u32]),
         "hps_beam_relative_phi": (C.c_int, [P, C.c_int, dp]),
         "hps_beam_size": (u32, [P]),
         "hps_beam_set_ref_w": (C.c_int, [P, d]),

@@ -12,7 +12,9 @@
 #include <cctype>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -396,6 +398,97 @@ int hps_beam_set_distribution(hps_beam* b, const double* x, const double* xp, co
   }
   HPS_CHECK(hipDeviceSynchronize());
   return init_phi_avg_good(b);
+}
+
+// Initial-beam samplers of the reference Beam (beam.cpp:391-537).  They run
+// on the host like the reference's and draw from the C library's
+// srand/rand, so one seed gives the reference's particles on the same libc.
+namespace {
+
+double rand_pm1() { return 2.0 * std::rand() / RAND_MAX - 1.0; }   // beam.cpp:424-425
+
+// PARMILA ellipse half-widths from Twiss (a, b, e): g = (1 + a^2)/b,
+// umax = sqrt(e/g), upmax = sqrt(e*g), shear du = -a/g (beam.cpp:409-418).
+struct Ellipse {
+  double umax, upmax, du;
+  Ellipse(double a, double bt, double e)
+  {
+    const double g = (1.0 + a * a) / bt;
+    umax = std::sqrt(e / g);
+    upmax = std::sqrt(e * g);
+    du = -a / g;
+  }
+};
+
+int upload_sampled(hps_beam* b, std::vector<double>& v)
+{
+  const size_t n = b->n;
+  return hps_beam_set_distribution(b, v.data(), v.data() + n, v.data() + 2 * n,
+                                    v.data() + 3 * n, v.data() + 4 * n, v.data() + 5 * n,
+                                    nullptr, nullptr);
+}
+
+}  // namespace
+
+int hps_beam_set_waterbag(hps_beam* b, double ax, double bx, double ex, double ay, double by,
+    double ey, double az, double bz, double ez, double sync_phi, double sync_w, double freq,
+    uint32_t seed)
+{
+  if (!b) return fail("null beam");
+  if (!(bx > 0 && by > 0 && bz > 0 && freq > 0 && sync_w > 0))
+    return fail("set_waterbag needs positive beta functions, frequency and energy");
+  std::srand(seed == 0 ? (unsigned)std::time(nullptr) : seed);
+  const Ellipse ex_(ax, bx, ex), ey_(ay, by, ey), ez_(az, bz, ez);
+  const double zpmx = ez / ez_.umax;               // beam.cpp:418
+  const double wave_len = 29979.2458 / freq;       // cm
+  const double gm = sync_w / b->mass + 1.0;
+  const double beta = std::sqrt(1.0 - 1.0 / (gm * gm));
+  const size_t n = b->n;
+  std::vector<double> v(6 * n);
+  for (size_t i = 0; i < n; ++i) {
+    double r1, r2, r3, r4, r5, r6;
+    // a point of the unit 6-ball: three discs by rejection, then the joint radius
+    do {
+      do { r1 = rand_pm1(); r2 = rand_pm1(); } while (r1 * r1 + r2 * r2 > 1.0);
+      do { r3 = rand_pm1(); r4 = rand_pm1(); } while (r3 * r3 + r4 * r4 > 1.0);
+      do { r5 = rand_pm1(); r6 = rand_pm1(); } while (r5 * r5 + r6 * r6 > 1.0);
+    } while (r1 * r1 + r2 * r2 + r3 * r3 + r4 * r4 + r5 * r5 + r6 * r6 > 1.0);
+    v[i] = 0.01 * (r1 * ex_.umax + ex_.du * r2 * ex_.upmax);
+    v[n + i] = r2 * ex_.upmax;
+    v[2 * n + i] = 0.01 * (r3 * ey_.umax + ey_.du * r4 * ey_.upmax);
+    v[3 * n + i] = r4 * ey_.upmax;
+    v[4 * n + i] = -(r5 * ez_.umax + ez_.du * zpmx * r6) * 2.0 * hps::PI / (beta * wave_len) + sync_phi;
+    v[5 * n + i] = 2.0 * sync_w * zpmx * r6 + sync_w;
+  }
+  b->freq = freq;                                  // beam.cpp:421
+  return upload_sampled(b, v);
+}
+
+int hps_beam_set_dc(hps_beam* b, double ax, double bx, double ex, double ay, double by,
+    double ey, double dphi, double sync_phi, double sync_w, uint32_t seed)
+{
+  if (!b) return fail("null beam");
+  if (!(bx > 0 && by > 0)) return fail("set_dc needs positive beta functions");
+  std::srand(seed == 0 ? (unsigned)std::time(nullptr) : seed);
+  const Ellipse ex_(ax, bx, ex), ey_(ay, by, ey);
+  const size_t n = b->n;
+  std::vector<double> v(6 * n);
+  for (size_t i = 0; i < n; ++i) {
+    double r1, r2, r3, r4;
+    // a point of the unit 4-ball
+    do {
+      do { r1 = rand_pm1(); r2 = rand_pm1(); } while (r1 * r1 + r2 * r2 > 1.0);
+      do { r3 = rand_pm1(); r4 = rand_pm1(); } while (r3 * r3 + r4 * r4 > 1.0);
+    } while (r1 * r1 + r2 * r2 + r3 * r3 + r4 * r4 > 1.0);
+    const double r5 = 2.0 * ((double)i + 1.0) / (double)n - 1.0;   // phase ramp by index
+    v[i] = 0.01 * (r1 * ex_.umax + ex_.du * r2 * ex_.upmax);
+    v[n + i] = r2 * ex_.upmax;
+    v[2 * n + i] = 0.01 * (r3 * ey_.umax + ey_.du * r4 * ey_.upmax);
+    v[3 * n + i] = r4 * ey_.upmax;
+    v[4 * n + i] = r5 * dphi + sync_phi;
+    v[5 * n + i] = sync_w;
+  }
+  return upload_sampled(b, v);
 }
 
 int hps_beam_download(hps_beam* b, double* x, double* xp, double* y, double* yp, double* phi,

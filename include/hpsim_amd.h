@@ -101,6 +101,18 @@ int hps_beam_destroy(hps_beam* b);
 int hps_beam_set_distribution(hps_beam* b, const double* x, const double* xp,
     const double* y, const double* yp, const double* phi, const double* w,
     const uint32_t* loss, const uint32_t* lloss);
+/* set_waterbag(ax, bx, ex, ay, by, ey, az, bz, ez, phi, w, freq[, seed]),
+ * wrap_beam.cpp:125-144 / Beam::InitWaterbagBeam beam.cpp:391-463: PARMILA
+ * Twiss input (beta in cm, emittance in cm*rad), 6D waterbag drawn with the C
+ * library's srand(seed)/rand (seed 0 = time), sets the beam frequency. */
+int hps_beam_set_waterbag(hps_beam* b, double ax, double bx, double ex, double ay, double by,
+    double ey, double az, double bz, double ez, double sync_phi, double sync_w, double freq,
+    uint32_t seed);
+/* set_dc(ax, bx, ex, ay, by, ey, dphi, sync_phi, sync_w[, seed]),
+ * wrap_beam.cpp:146-165 / Beam::InitDCBeam beam.cpp:480-537: 4D transverse
+ * waterbag, phase a linear ramp over [-dphi, dphi], w = sync_w. */
+int hps_beam_set_dc(hps_beam* b, double ax, double bx, double ex, double ay, double by,
+    double ey, double dphi, double sync_phi, double sync_w, uint32_t seed);
 /* Copy coordinates to host; any pointer may be NULL.  Replaces the array
  * part of get_x/get_xp/.../get_losses, wrap_beam.cpp:440-912. */
 int hps_beam_download(hps_beam* b, double* x, double* xp, double* y, double* yp,

@@ -31,6 +31,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define CLIGHT 299.792458                       /* constant.h:7 */
 #define PI 3.14159265358979323846              /* constant.h:8 */
@@ -313,6 +314,94 @@ int orc_beam_set_distribution(orc_beam* b, const double* x, const double* xp,
   else memset(b->lloss, 0, (size_t)b->n * 4);
   init_phi_avg_good(b);
   return 0;
+}
+
+/* Uniform deviate on [-1, 1] from the C library generator, as the
+ * reference draws it (beam.cpp:424-425). */
+static double rand_pm1(void) { return 2.0 * rand() / RAND_MAX - 1.0; }
+
+/* Twiss half-widths of a PARMILA phase-space ellipse (beam.cpp:409-418):
+ * g = (1 + a^2)/b, umax = sqrt(e/g), upmax = sqrt(e*g), du = -a/g. */
+static void twiss_widths(double a, double bt, double e, double* umax, double* upmax,
+                         double* du)
+{
+  double g = (1.0 + a * a) / bt;
+  *umax = sqrt(e / g);
+  *upmax = sqrt(e * g);
+  *du = -a / g;
+}
+
+/* Beam::InitWaterbagBeam, beam.cpp:391-463.  Points uniform in the 6-ball
+ * (three disc draws by rejection, then rejection on the joint radius),
+ * mapped onto the three Twiss ellipses; x, y in m from cm, phase from the
+ * longitudinal ellipse in cm at beta*lambda of sync_w. */
+int orc_beam_set_waterbag(orc_beam* b, double ax, double bx, double ex, double ay,
+    double by, double ey, double az, double bz, double ez, double sync_phi,
+    double sync_w, double freq, uint32_t seed)
+{
+  srand(seed == 0 ? (unsigned)time(NULL) : seed);
+  double xmx, xpmx, dx, ymx, ypmx, dy, zmx, zpmx_unused, dz;
+  twiss_widths(ax, bx, ex, &xmx, &xpmx, &dx);
+  twiss_widths(ay, by, ey, &ymx, &ypmx, &dy);
+  twiss_widths(az, bz, ez, &zmx, &zpmx_unused, &dz);
+  double zpmx = ez / zmx;   /* beam.cpp:418 uses e/zmax, not sqrt(e*g) */
+  double wave_len = 29979.2458 / freq;   /* cm */
+  b->freq = freq;
+  double gm = sync_w / b->mass + 1.0;
+  double beta = sqrt(1.0 - 1.0 / (gm * gm));
+  uint32_t n = b->n;
+  double* v = (double*)malloc(sizeof(double) * 6 * (size_t)n);
+  if (!v) return -1;
+  for (uint32_t i = 0; i < n; ++i) {
+    double r1, r2, r3, r4, r5, r6;
+    do {
+      do { r1 = rand_pm1(); r2 = rand_pm1(); } while (r1 * r1 + r2 * r2 > 1.0);
+      do { r3 = rand_pm1(); r4 = rand_pm1(); } while (r3 * r3 + r4 * r4 > 1.0);
+      do { r5 = rand_pm1(); r6 = rand_pm1(); } while (r5 * r5 + r6 * r6 > 1.0);
+    } while (r1 * r1 + r2 * r2 + r3 * r3 + r4 * r4 + r5 * r5 + r6 * r6 > 1.0);
+    v[i] = 0.01 * (r1 * xmx + dx * r2 * xpmx);
+    v[n + i] = r2 * xpmx;
+    v[2 * (size_t)n + i] = 0.01 * (r3 * ymx + dy * r4 * ypmx);
+    v[3 * (size_t)n + i] = r4 * ypmx;
+    v[4 * (size_t)n + i] = -(r5 * zmx + dz * zpmx * r6) * 2.0 * PI / (beta * wave_len) + sync_phi;
+    v[5 * (size_t)n + i] = 2.0 * sync_w * zpmx * r6 + sync_w;
+  }
+  int rc = orc_beam_set_distribution(b, v, v + n, v + 2 * (size_t)n, v + 3 * (size_t)n,
+                                     v + 4 * (size_t)n, v + 5 * (size_t)n, NULL, NULL);
+  free(v);
+  return rc;
+}
+
+/* Beam::InitDCBeam, beam.cpp:480-537: transverse points uniform in the
+ * 4-ball, phase a uniform ramp over [-dphi, dphi) by index, w = sync_w. */
+int orc_beam_set_dc(orc_beam* b, double ax, double bx, double ex, double ay,
+    double by, double ey, double dphi, double sync_phi, double sync_w, uint32_t seed)
+{
+  srand(seed == 0 ? (unsigned)time(NULL) : seed);
+  double xmx, xpmx, dx, ymx, ypmx, dy;
+  twiss_widths(ax, bx, ex, &xmx, &xpmx, &dx);
+  twiss_widths(ay, by, ey, &ymx, &ypmx, &dy);
+  uint32_t n = b->n;
+  double* v = (double*)malloc(sizeof(double) * 6 * (size_t)n);
+  if (!v) return -1;
+  for (uint32_t i = 0; i < n; ++i) {
+    double r1, r2, r3, r4;
+    do {
+      do { r1 = rand_pm1(); r2 = rand_pm1(); } while (r1 * r1 + r2 * r2 > 1.0);
+      do { r3 = rand_pm1(); r4 = rand_pm1(); } while (r3 * r3 + r4 * r4 > 1.0);
+    } while (r1 * r1 + r2 * r2 + r3 * r3 + r4 * r4 > 1.0);
+    double r5 = 2.0 * ((double)i + 1.0) / (double)n - 1.0;
+    v[i] = 0.01 * (r1 * xmx + dx * r2 * xpmx);
+    v[n + i] = r2 * xpmx;
+    v[2 * (size_t)n + i] = 0.01 * (r3 * ymx + dy * r4 * ypmx);
+    v[3 * (size_t)n + i] = r4 * ypmx;
+    v[4 * (size_t)n + i] = r5 * dphi + sync_phi;
+    v[5 * (size_t)n + i] = sync_w;
+  }
+  int rc = orc_beam_set_distribution(b, v, v + n, v + 2 * (size_t)n, v + 3 * (size_t)n,
+                                     v + 4 * (size_t)n, v + 5 * (size_t)n, NULL, NULL);
+  free(v);
+  return rc;
 }
 
 /* Beam::SetRefEnergy / SetRefPhase, beam.cpp:1074-1089 */

@@ -90,6 +90,13 @@ void orc_beam_destroy(orc_beam* b);
 int orc_beam_set_distribution(orc_beam* b, const double* x, const double* xp,
     const double* y, const double* yp, const double* phi, const double* w,
     const uint32_t* loss, const uint32_t* lloss);
+/* PARMILA-style waterbag / DC initial beams drawn from the C library's
+ * srand/rand (seed 0 = time-seeded). */
+int orc_beam_set_waterbag(orc_beam* b, double ax, double bx, double ex, double ay,
+    double by, double ey, double az, double bz, double ez, double sync_phi,
+    double sync_w, double freq, uint32_t seed);
+int orc_beam_set_dc(orc_beam* b, double ax, double bx, double ex, double ay,
+    double by, double ey, double dphi, double sync_phi, double sync_w, uint32_t seed);
 void orc_beam_set_ref_w(orc_beam* b, double w);
 void orc_beam_set_ref_phi(orc_beam* b, double phi);
 void orc_beam_set_frequency(orc_beam* b, double f);

@@ -101,6 +101,10 @@ def lib():
                   "orc_beam_save_intermediate", "orc_beam_restore_intermediate"):
             getattr(L, f).argtypes = [P]
         L.orc_beam_translate.argtypes = [P, C.c_int, C.c_double]
+        L.orc_beam_set_waterbag.restype = C.c_int
+        L.orc_beam_set_waterbag.argtypes = [P] + [C.c_double] * 12 + [C.c_uint32]
+        L.orc_beam_set_dc.restype = C.c_int
+        L.orc_beam_set_dc.argtypes = [P] + [C.c_double] * 9 + [C.c_uint32]
         L.orc_beam_apply_cut.argtypes = [P, C.c_char, C.c_double, C.c_double]
         L.orc_spch_create.restype = P
         L.orc_spch_create.argtypes = [C.c_uint32, C.c_uint32, C.c_int]
@@ -184,6 +188,15 @@ class Beam:
         lo = None if loss is None else np.ascontiguousarray(loss, dtype=np.uint32)
         ll = None if lloss is None else np.ascontiguousarray(lloss, dtype=np.uint32)
         lib().orc_beam_set_distribution(self.h, *[_dp(a) for a in arrs], _up(lo), _up(ll))
+
+    def set_waterbag(self, ax, bx, ex, ay, by, ey, az, bz, ez, phi, w, freq, seed=0):
+        if lib().orc_beam_set_waterbag(self.h, ax, bx, ex, ay, by, ey, az, bz, ez, phi, w,
+                                       freq, seed):
+            raise MemoryError("set_waterbag")
+
+    def set_dc(self, ax, bx, ex, ay, by, ey, dphi, sync_phi, sync_w, seed=0):
+        if lib().orc_beam_set_dc(self.h, ax, bx, ex, ay, by, ey, dphi, sync_phi, sync_w, seed):
+            raise MemoryError("set_dc")
 
     def set_ref_w(self, v): lib().orc_beam_set_ref_w(self.h, v)
     def set_ref_phi(self, v): lib().orc_beam_set_ref_phi(self.h, v)

@@ -206,3 +206,31 @@ def test_tiny_beams_bit_exact(num):
     gs, os_ = gsc.state(), osc.state()
     for k in ("remesh_count", "kick_count", "num_on_mesh"):
         assert gs[k] == os_[k], (k, gs[k], os_[k])
+
+
+def test_waterbag_and_dc_beams_match_oracle_then_track(oracle_lib):
+    """set_waterbag / set_dc draw the same particles as the oracle (host
+    sampling with the C library's rand), and the sampled beam then tracks
+    bit-exactly through a lattice with space charge."""
+    from hpsim_amd import HPSim
+    from hpsim_amd import configs as cfg
+    O = oracle_lib
+    n = 4096
+    wb = (-0.3, 20.0, 0.003, 0.4, 18.0, 0.003, 0.1, 30.0, 0.01, -0.52, 0.75, 201.25, 11)
+    dc = (0.2, 15.0, 0.002, -0.2, 15.0, 0.002, 1.0, -0.5, 0.75, 5)
+    for kind, args in (("waterbag", wb), ("dc", dc)):
+        g = HPSim.Beam(mass=939.294, charge=-1.0, current=0.01, num=n)
+        o = O.Beam(n, 939.294, -1.0, 0.01)
+        getattr(g, "set_" + kind)(*args)
+        getattr(o, "set_" + kind)(*args)
+        if kind == "dc":
+            g.set_frequency(201.25)
+            o.set_frequency(201.25)
+        assert H.compare_arrays(g.arrays(), o.arrays(), kind) is None
+        assert g.get_frequency() == o.get_frequency()
+    wl = cfg.config(0, num=n, periods=2)
+    elems = cfg.build_lattice(wl.lattice)
+    HPSim.Simulator(g, HPSim.BeamLine(elements=elems),
+                    HPSim.SpaceCharge(nr=wl.nr, nz=wl.nz, interval=wl.interval)).simulate()
+    O.Simulator(o, elems, O.SpaceCharge(wl.nr, wl.nz, wl.interval)).simulate()
+    assert H.compare_arrays(g.arrays(), o.arrays(), "dc beam tracked") is None

@@ -389,3 +389,78 @@ def test_truncated_gaussian_moments():
     var = 1 - 6 * math.exp(-4.5) / math.sqrt(2 * math.pi) / math.erf(3 / math.sqrt(2))
     assert abs(z.mean()) < 0.01 and abs(z.var() - var) < 0.01
     assert np.array_equal(z, cfg.truncated_normal(200000, 3, seed=11))
+
+
+# --------------------------------------------------------------------------
+# waterbag / DC initial beams (beam.cpp:391-537)
+# --------------------------------------------------------------------------
+
+def _rms_twiss(u, v):
+    cuu = np.mean(u * u) - np.mean(u) ** 2
+    cvv = np.mean(v * v) - np.mean(v) ** 2
+    cuv = np.mean(u * v) - np.mean(u) * np.mean(v)
+    e = math.sqrt(cuu * cvv - cuv * cuv)
+    return e, -cuv / e, cuu / e
+
+
+def test_libc_rand_is_the_glibc_generator():
+    """The samplers draw from the C library's rand(); glibc's srand(1) stream
+    starts 1804289383, 846930886, 1681692777 (the reference's Linux build
+    draws the same stream)."""
+    libc = C.CDLL(None)
+    libc.srand(1)
+    assert [libc.rand() for _ in range(3)] == [1804289383, 846930886, 1681692777]
+
+
+def test_waterbag_moments_match_twiss_closed_form(oracle_lib):
+    O = oracle_lib
+    """A uniform 6-ball has <r_i^2> = 1/8, so each plane's rms emittance is
+    e/8 with the input alpha and beta (x in cm, phase mapped back through
+    beta*lambda, W through 2 W_s)."""
+    n = 200001
+    ax, bx, ex, ay, by, ey, az, bz, ez = -1.2, 20.0, 0.003, 0.8, 15.0, 0.004, 0.3, 30.0, 0.02
+    phis, ws, f, m = -0.5, 0.75, 201.25, 939.294
+    b = O.Beam(n, m, -1.0, 0.0)
+    b.set_waterbag(ax, bx, ex, ay, by, ey, az, bz, ez, phis, ws, f, 7)
+    a = b.arrays()
+    assert a["x"][0] == 0 and a["xp"][0] == 0 and a["y"][0] == 0 and a["yp"][0] == 0
+    assert b.get_frequency() == f
+    gm = ws / m + 1
+    k = 2 * math.pi / (math.sqrt(1 - 1 / gm ** 2) * 29979.2458 / f)
+    planes = [(a["x"][1:] * 100, a["xp"][1:], ex, ax, bx),
+              (a["y"][1:] * 100, a["yp"][1:], ey, ay, by),
+              (-(a["phi"][1:] - phis) / k, (a["w"][1:] - ws) / (2 * ws), ez, az, bz)]
+    for u, v, e, al, bt in planes:
+        er, alr, btr = _rms_twiss(u, v)
+        assert abs(er * 8 / e - 1) < 0.01
+        assert abs(alr - al) < 0.02 and abs(btr / bt - 1) < 0.01
+    # every point lies inside the joint 6-ball: sum of the normalised
+    # Courant-Snyder invariants (gamma u^2 + 2 alpha u v + beta v^2) / e <= 1
+    inv = sum(((1 + al * al) / bt * u * u + 2 * al * u * v + bt * v * v) / e
+              for u, v, e, al, bt in planes)
+    assert np.all(inv <= 1 + 1e-9) and inv.max() > 0.99
+    # same seed, same beam; another seed, another beam
+    b2 = O.Beam(n, m, -1.0, 0.0)
+    b2.set_waterbag(ax, bx, ex, ay, by, ey, az, bz, ez, phis, ws, f, 7)
+    assert np.array_equal(b2.arrays()["phi"], a["phi"])
+    b2.set_waterbag(ax, bx, ex, ay, by, ey, az, bz, ez, phis, ws, f, 8)
+    assert not np.array_equal(b2.arrays()["phi"], a["phi"])
+
+
+def test_dc_beam_ramp_and_4ball_moments(oracle_lib):
+    O = oracle_lib
+    n = 100000
+    ax, bx, ex, ay, by, ey = 0.5, 12.0, 0.002, -0.4, 18.0, 0.0025
+    dphi, phis, ws = 1.5, 0.2, 0.75
+    b = O.Beam(n, 939.294, -1.0, 0.0)
+    b.set_dc(ax, bx, ex, ay, by, ey, dphi, phis, ws, 3)
+    a = b.arrays()
+    i = np.arange(n, dtype=np.float64)
+    assert np.array_equal(a["phi"], (2.0 * (i + 1.0) / n - 1.0) * dphi + phis)
+    assert np.all(a["w"] == ws)
+    for u, v, e, al, bt in ((a["x"][1:] * 100, a["xp"][1:], ex, ax, bx),
+                            (a["y"][1:] * 100, a["yp"][1:], ey, ay, by)):
+        er, alr, btr = _rms_twiss(u, v)
+        # uniform 4-ball: <r_i^2> = 1/6
+        assert abs(er * 6 / e - 1) < 0.01
+        assert abs(alr - al) < 0.02 and abs(btr / bt - 1) < 0.01
