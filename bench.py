@@ -40,6 +40,9 @@ def parse():
     p.add_argument("--num", type=int, default=None, help="override particle count")
     p.add_argument("--periods", type=int, default=None, help="override lattice periods")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--eager", action="store_true",
+                   help="launch every kernel from the host instead of replaying a HIP graph "
+                        "(for profilers that do not follow graph launches)")
     p.add_argument("--cpu-periods", type=int, default=40,
                    help="lattice periods of the CPU-baseline sample")
     return p.parse_args()
@@ -93,6 +96,7 @@ def main():
     bl = HPSim.BeamLine(elements=elems)
     sc = HPSim.SpaceCharge(nr=wl.nr, nz=wl.nz, interval=wl.interval)
     sim = HPSim.Simulator(beam, bl, sc)
+    sim.set_graph(not args.eager)
     nel = len(elems)
 
     def step():
@@ -133,7 +137,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (Philox-seeded truncated Gaussian beam, generated lattice)",
         "config": {"workload": wl.name, "particles": b.num, "elements": nel,
-                   "mesh": f"{wl.nr}x{wl.nz}", "sc_kicks_per_step": prof["phase"][1],
+                   "mesh": f"{wl.nr}x{wl.nz}", "sc_kicks_per_step": prof["phase_final"][1],
                    "transmitted": b.num - lost},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -143,6 +147,7 @@ def main():
                                "the timed steps"},
         "kernel_ms_per_step": kernel_ms,
         "graph_replays": replays,
+        "launch": "eager" if args.eager else "hip-graph",
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_periods)

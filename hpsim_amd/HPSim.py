@@ -455,11 +455,12 @@ class Simulator:
 
     def profile(self):
         """{family: (total ms, launches)} since profiling was enabled."""
-        ms = (C.c_double * 5)()
-        n = (C.c_uint32 * 5)()
+        names = ("segment", "phase", "greens", "deposit", "field", "segment_final",
+                 "phase_final")
+        ms = (C.c_double * len(names))()
+        n = (C.c_uint32 * len(names))()
         check(_lib().hps_sim_get_profile(self._h, ms, n))
-        names = ("segment", "phase", "greens", "deposit", "field")
-        return {names[k]: (ms[k], n[k]) for k in range(5)}
+        return {names[k]: (ms[k], n[k]) for k in range(len(names))}
 
     def last_time_ms(self):
         v = C.c_double()

@@ -248,7 +248,6 @@ void free_beam(hps_beam* b)
   (void)hipFree(b->pr.sums);
   (void)hipFree(b->pr.cnt);
   (void)hipFree(b->pr.pmax);
-  (void)hipFree(b->pr.ticket);
   (void)hipFree(b->pr.out);
   (void)hipFree(b->pr.out_cnt);
   for (int k = 0; k < 2; ++k) {
@@ -293,8 +292,8 @@ struct hps_sim {
   // per-kernel-family device time (HIP events around each launch)
   bool profile = false;
   std::vector<hipEvent_t> evpool;
-  double prof_ms[5] = {0, 0, 0, 0, 0};
-  uint32_t prof_n[5] = {0, 0, 0, 0, 0};
+  double prof_ms[HPS_PROFILE_FAMILIES] = {};
+  uint32_t prof_n[HPS_PROFILE_FAMILIES] = {};
   // HIP graph of the last launch sequence, reused while the sequence
   // (every kernel argument and the map table) is byte-identical
   bool use_graph = true;
@@ -346,7 +345,6 @@ int hps_beam_create(hps_beam** out, uint32_t num, double mass, double charge, do
        hipMemset(b->pr.cnt, 0, 3 * (size_t)b->pr.mpad * 4) == hipSuccess;
   ok = ok && hipMalloc(&b->pr.pmax, (size_t)b->pr.mpad * 8) == hipSuccess &&
        hipMemset(b->pr.pmax, 0, (size_t)b->pr.mpad * 8) == hipSuccess;
-  ok = ok && hipMalloc(&b->pr.ticket, 16) == hipSuccess && hipMemset(b->pr.ticket, 0, 16) == hipSuccess;
   ok = ok && hipMalloc(&b->pr.out, kMaxQ * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc(&b->pr.out_cnt, 4 * sizeof(uint32_t)) == hipSuccess;
   if (!ok) { free_beam(b); delete b; return fail("device allocation failed for the beam"); }
@@ -940,7 +938,7 @@ int hps_sim_set_profiling(hps_sim* m, int on)
 {
   if (!m) return fail("null simulator");
   m->profile = on != 0;
-  for (int k = 0; k < 5; ++k) { m->prof_ms[k] = 0; m->prof_n[k] = 0; }
+  for (int k = 0; k < HPS_PROFILE_FAMILIES; ++k) { m->prof_ms[k] = 0; m->prof_n[k] = 0; }
   return 0;
 }
 
@@ -961,7 +959,7 @@ int hps_sim_graph_replays(hps_sim* m, uint32_t* out)
 int hps_sim_get_profile(hps_sim* m, double* ms, uint32_t* launches)
 {
   if (!m || !ms || !launches) return fail("null argument");
-  for (int k = 0; k < 5; ++k) { ms[k] = m->prof_ms[k]; launches[k] = m->prof_n[k]; }
+  for (int k = 0; k < HPS_PROFILE_FAMILIES; ++k) { ms[k] = m->prof_ms[k]; launches[k] = m->prof_n[k]; }
   return 0;
 }
 
@@ -1213,8 +1211,10 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   const bool deposit_ok = b->blck_size / 4 != 0;
 
   // The launch list of this call.
+  // kinds 0..6 are the profiling families of the header; 7 is a host step
+  enum { L_SEG = 0, L_PHASE, L_GREENS, L_DEPOSIT, L_FIELD, L_SEG_FINAL, L_PHASE_FINAL, L_HOST };
   struct Launch {
-    int kind;          // 0 segment, 1 phase, 2 greens, 3 deposit, 4 field, 5 host step
+    int kind;
     SegArgs seg;
     PhaseArgs ph;
     double new_freq;
@@ -1224,7 +1224,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   for (const Step& st : c.steps) {
     if (st.kind == 1) {
       Launch l{};
-      l.kind = 5;
+      l.kind = L_HOST;
       l.new_freq = st.new_freq;
       launches.push_back(l);
       host_steps = true;
@@ -1232,7 +1232,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
     }
     if (st.op_end > st.op_begin || st.kick || st.stats != STAT_NONE) {
       Launch l{};
-      l.kind = 0;
+      l.kind = L_SEG;
       SegArgs& a = l.seg;
       a.b = b->d;
       a.st = b->st;
@@ -1247,31 +1247,39 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       a.fld2 = m->sc ? m->sc->fld2 : nullptr;
       a.stats = st.stats;
       launches.push_back(l);
+      if (a.stats != STAT_NONE) {
+        l.kind = L_SEG_FINAL;
+        launches.push_back(l);
+      }
     }
     if (st.kick_point) {
       hps_spch* sc = m->sc;
       Launch l{};
-      l.kind = 1;
+      l.kind = L_PHASE;
       PhaseArgs& p = l.ph;
       p.b = b->d; p.st = b->st; p.pr = b->pr; p.sc = sc->dev; p.bins = sc->bins;
       p.do_sums = b->n > 2 ? 1 : 0;
       p.frac = st.frac; p.current = b->current; p.freq = b->freq;
-      launches.push_back(l);
+      if (p.do_sums) launches.push_back(l);
       Launch g = l;
-      g.kind = 2;
+      g.kind = L_PHASE_FINAL;
       launches.push_back(g);
-      if (deposit_ok) { g.kind = 3; launches.push_back(g); }
-      g.kind = 4;
+      g.kind = L_GREENS;
+      launches.push_back(g);
+      if (deposit_ok) { g.kind = L_DEPOSIT; launches.push_back(g); }
+      g.kind = L_FIELD;
       launches.push_back(g);
     }
   }
   auto issue = [&](const Launch& l, hipStream_t st) -> hipError_t {
     hps_spch* sc = m->sc;
     switch (l.kind) {
-      case 0: return launch_segment(l.seg, st);
-      case 1: return launch_phase(l.ph, st);
-      case 2: return launch_greens(sc->dev, sc->fld1, sc->cfg.nx, sc->cfg.nz, b->n, b->mass, st);
-      case 3: return launch_deposit(b->d, b->st, sc->dev, sc->bins, sc->cfg.nx, sc->cfg.nz, st);
+      case L_SEG: return launch_segment(l.seg, st);
+      case L_SEG_FINAL: return launch_seg_final(l.seg, st);
+      case L_PHASE: return launch_phase(l.ph, st);
+      case L_PHASE_FINAL: return launch_phase_final(l.ph, st);
+      case L_GREENS: return launch_greens(sc->dev, sc->fld1, sc->cfg.nx, sc->cfg.nz, b->n, b->mass, st);
+      case L_DEPOSIT: return launch_deposit(b->d, b->st, sc->dev, sc->bins, sc->cfg.nx, sc->cfg.nz, st);
       default: return launch_field(sc->dev, sc->bins, sc->fld1, sc->fld2, sc->cfg.nx, sc->cfg.nz, st);
     }
   };
@@ -1283,7 +1291,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   HPS_CHECK(hipEventRecord(m->ev0, s));
   if (eager) {
     for (const Launch& l : launches) {
-      if (l.kind == 5) {
+      if (l.kind == L_HOST) {
         // the phases of a frequency change are not known at compile time of
         // later launches' arguments: rebuild them after the host step
         if (change_frequency(b, l.new_freq)) return -1;
@@ -1299,7 +1307,10 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
         HPS_CHECK(hipEventRecord(m->evpool[nev], s));
       }
       Launch lc = l;
-      if (l.kind == 1) { lc.ph.current = b->current; lc.ph.freq = b->freq; }
+      if (l.kind == L_PHASE || l.kind == L_PHASE_FINAL) {
+        lc.ph.current = b->current;
+        lc.ph.freq = b->freq;
+      }
       HPS_CHECK(issue(lc, s));
       if (m->profile) {
         HPS_CHECK(hipEventRecord(m->evpool[nev + 1], s));
@@ -1314,7 +1325,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
     };
     for (const Launch& l : launches) {
       put(&l.kind, sizeof l.kind);
-      if (l.kind == 0) put(&l.seg, sizeof l.seg);
+      if (l.kind == L_SEG || l.kind == L_SEG_FINAL) put(&l.seg, sizeof l.seg);
       else put(&l.ph, sizeof l.ph);
     }
     put(c.ops.data(), c.ops.size() * sizeof(Op));

@@ -30,12 +30,11 @@ struct ScDev {
 };
 
 // Scratch for the canonical reductions: per-chunk partial sums, padded to a
-// power of two with zeros, plus the grid ticket of the last-block pattern.
+// power of two with zeros.
 struct Partials {
   double* sums;        // [kMaxQ][mpad]
-  uint32_t* cnt;       // [mpad]
+  uint32_t* cnt;       // [3][mpad]
   double* pmax;        // [mpad]
-  uint32_t* ticket;    // one counter per kernel family
   double* out;         // [kMaxQ] final sums for host-side getters
   uint32_t* out_cnt;   // [1]
   uint32_t mpad;       // NextPow2(nchunks)
@@ -73,7 +72,9 @@ struct PhaseArgs {
 
 // Launchers (hps_kernels.hip).
 hipError_t launch_segment(const SegArgs& a, hipStream_t s);
+hipError_t launch_seg_final(const SegArgs& a, hipStream_t s);
 hipError_t launch_phase(const PhaseArgs& a, hipStream_t s);
+hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s);
 hipError_t launch_greens(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, uint32_t num,
                          double mass, hipStream_t s);
 hipError_t launch_deposit(BeamArrays b, const hps_beam_stats* st, ScDev* sc, int64_t* bins,

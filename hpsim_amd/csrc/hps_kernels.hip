@@ -2,19 +2,22 @@
 //
 // Kernel families (one HBM sweep each):
 //   k_segment  SCHEFF kick of the previous kick point + every element map up
-//              to the next kick point, fused in registers, + the beam moments
-//              the next kick needs (reference: SimulatePartialDrift/HalfQuad/
-//              RFGap kernels, ApplySpchKickKernel, and the reductions of
-//              Beam::UpdateStatForSpaceCharge up to the x/y moments).
-//   k_phase    relative phase and its moments, then the Scheff::Start control
-//              logic on the device (no host round trip per kick).
+//              to the next kick point, fused in registers, + per-chunk beam
+//              moments the next kick needs (reference: SimulatePartialDrift/
+//              HalfQuad/RFGap kernels, ApplySpchKickKernel, and the
+//              reductions of Beam::UpdateStatForSpaceCharge up to the x/y
+//              moments); k_seg_final finishes them in one workgroup.
+//   k_phase    per-chunk relative-phase moments; k_phase_final finishes them
+//              and runs the Scheff::Start control logic on the device (no
+//              host round trip per kick).
 //   k_greens / k_deposit / k_field   the SCHEFF rebuild, run only when the
 //              control step asks for a remesh (flag in device memory).
 // Reductions follow one canonical order (see kChunk): lane sums of 4
 // particles strided by 256, halving tree over 256 lanes, halving tree over
-// the chunk partials zero-padded to a power of two.  The last workgroup of a
-// launch (ticket counter) performs the cross-chunk tree, so each reduction
-// costs one launch.
+// the chunk partials zero-padded to a power of two.  The grid kernels write
+// one partial per chunk; a one-workgroup kernel launched after them performs
+// the cross-chunk tree (kernel order makes the partials visible, with no
+// device-wide fence per workgroup).
 #include "hps_internal.hpp"
 
 namespace hps {
@@ -114,22 +117,6 @@ __device__ void final_reduce(const double* P, const uint32_t* C, uint32_t mpad, 
   for (int q = 0; q < Q; ++q) out[q] = bcast[q];
   cnt_out = bcnt;
   __syncthreads();
-}
-
-// Returns true in every thread of the last workgroup to finish.
-__device__ __forceinline__ bool last_block(uint32_t* ticket, int t)
-{
-  __shared__ int am_last;
-  __syncthreads();
-  if (t == 0) {
-    __threadfence();
-    const uint32_t prev = atomicAdd(ticket, 1u);
-    am_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  const bool r = am_last != 0;
-  if (r) __threadfence();
-  return r;
 }
 
 __device__ __forceinline__ bool is_good(uint32_t loss, uint32_t lloss)
@@ -309,8 +296,6 @@ __device__ void aper_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem
 __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
 {
   __shared__ RedSmem sm;
-  __shared__ uint32_t rescan_list[64];
-  __shared__ uint32_t rescan_n;
   const int t = threadIdx.x;
   const uint32_t c = blockIdx.x;
   const BeamArrays& b = a.b;
@@ -358,11 +343,24 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
     }
   }
   if (a.stats == STAT_NONE) return;
+  if (a.stats == STAT_APERTURE) aper_chunk_stats(b, c, t, sm, a.pr, p);
+  else sc_chunk_stats(b, c, t, sm, a.pr, p, false);
+}
 
+// Cross-chunk step of the k_segment statistics in one workgroup, launched
+// after k_segment so that kernel order, not a device-wide fence per
+// workgroup, makes the partials visible: the aperture loss count and
+// centroid (UpdateLoss + UpdateAvgXY), or the longitudinal-loss update and
+// good moments of the space-charge statistics.
+__global__ __launch_bounds__(kBlock) void k_seg_final(SegArgs a)
+{
+  __shared__ RedSmem sm;
+  __shared__ uint32_t rescan_list[64];
+  __shared__ uint32_t rescan_n;
+  const int t = threadIdx.x;
+  const BeamArrays& b = a.b;
   const Partials& pr = a.pr;
   if (a.stats == STAT_APERTURE) {
-    aper_chunk_stats(b, c, t, sm, pr, p);
-    if (!last_block(pr.ticket, t)) return;
     double s[2];
     uint32_t nloss;
     final_reduce<2>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, nloss);
@@ -372,16 +370,12 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
       const double surv = (double)(uint32_t)(b.n - nloss);
       st->x_avg = s[0] / surv;
       st->y_avg = s[1] / surv;
-      *pr.ticket = 0;
     }
     return;
   }
 
-  // STAT_SC: moments after the maps, then (last block) the longitudinal-loss
-  // update of Beam::UpdateLongitudinalLoss (beam.cpp:776-782) and the good
-  // moments of beam.cpp:1008-1014.
-  sc_chunk_stats(b, c, t, sm, pr, p, false);
-  if (!last_block(pr.ticket, t)) return;
+  // STAT_SC: the longitudinal-loss update of Beam::UpdateLongitudinalLoss
+  // (beam.cpp:776-782), then the good moments of beam.cpp:1008-1014.
   hps_beam_stats* st = a.st;
   double s[kScQ];
   uint32_t g1;
@@ -414,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
         if (i < b.n && b.phi[i] > thr) b.lloss[i] = 1;
       }
       __syncthreads();
-      sc_chunk_stats(b, cc, t, sm, pr, p, true);
+      sc_chunk_stats(b, cc, t, sm, pr, nullptr, true);
       __syncthreads();
     }
   }
@@ -439,12 +433,15 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
     st->y_avg_good = yavg;
     st->y_sig_good = ::sqrt(y2avg - yavg * yavg);
     st->w_avg_good = g2 > 0 ? s[5] / (double)g2 : -99999;
-    *pr.ticket = 0;
   }
 }
 
+
 // --- k_phase -------------------------------------------------------------------
 
+// Per-chunk sums of the relative phase (UpdateRelativePhi(true) then the
+// phi_r moments of Beam::UpdateStatForSpaceCharge, beam.cpp:1003-1016):
+//   0 phi_r over loss == 0, 1 phi_r and 2 phi_r^2 over good particles.
 __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
 {
   __shared__ RedSmem sm;
@@ -452,53 +449,67 @@ __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
   const uint32_t c = blockIdx.x;
   const BeamArrays& b = a.b;
   const Partials& pr = a.pr;
-  hps_beam_stats* st = a.st;
-  if (a.do_sums) {
-    const double ref = st->phi_avg_good;
-    double acc[3] = {0.0, 0.0, 0.0};
+  const double ref = a.st->phi_avg_good;
+  double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int j = 0; j < (int)kPerLane; ++j) {
-      const uint32_t i = c * kChunk + j * kBlock + t;
-      double pr_ = 0.0;
-      uint32_t loss = 1, lloss = 0;
-      if (i < b.n) { loss = b.loss[i]; lloss = b.lloss[i]; pr_ = relative_phi(b.phi[i], ref); }
-      const bool tg = (i < b.n) && loss == 0;
-      const bool g = (i < b.n) && is_good(loss, lloss);
-      acc[0] = acc[0] + (tg ? pr_ : 0.0);
-      const double vg = g ? pr_ : 0.0;
-      acc[1] = acc[1] + vg;
-      acc[2] = acc[2] + vg * vg;
-    }
-    uint32_t cnt = 0;
-    double mx = 0.0;
-    block_reduce<3>(acc, cnt, mx, sm.red, sm.ured, sm.mred, t);
-    if (t == 0) {
-      pr.sums[c] = acc[0];
-      pr.sums[pr.mpad + c] = acc[1];
-      pr.sums[2 * pr.mpad + c] = acc[2];
-    }
-    if (!last_block(pr.ticket, t)) return;
-    double s[3];
-    uint32_t dummy;
-    final_reduce<3>(pr.sums, nullptr, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, dummy);
-    if (t == 0) {
-      st->phi_avg_r = s[0] / (double)(uint32_t)(b.n - st->num_loss);
-      const double gd = (double)st->num_good;
-      const double avg = s[1] / gd, avg2 = s[2] / gd;
-      st->phi_sig_good = ::sqrt(avg2 - avg * avg);
-      *pr.ticket = 0;
-    }
+  for (int j = 0; j < (int)kPerLane; ++j) {
+    const uint32_t i = c * kChunk + j * kBlock + t;
+    double pr_ = 0.0;
+    uint32_t loss = 1, lloss = 0;
+    if (i < b.n) { loss = b.loss[i]; lloss = b.lloss[i]; pr_ = relative_phi(b.phi[i], ref); }
+    const bool tg = (i < b.n) && loss == 0;
+    const bool g = (i < b.n) && is_good(loss, lloss);
+    acc[0] = acc[0] + (tg ? pr_ : 0.0);
+    const double vg = g ? pr_ : 0.0;
+    acc[1] = acc[1] + vg;
+    acc[2] = acc[2] + vg * vg;
   }
-  // Scheff::Start control logic, space_charge.cpp:84-138 (adj_bunch = 0,
-  // mesh_size_cutoff_w = 0).
-  __shared__ int remesh;
+  uint32_t cnt = 0;
+  double mx = 0.0;
+  block_reduce<3>(acc, cnt, mx, sm.red, sm.ured, sm.mred, t);
   if (t == 0) {
-    ScDev* sc = a.sc;
+    pr.sums[c] = acc[0];
+    pr.sums[pr.mpad + c] = acc[1];
+    pr.sums[2 * pr.mpad + c] = acc[2];
+  }
+}
+
+// One workgroup after k_phase: the phi_r moments, then the Scheff::Start
+// control logic of space_charge.cpp:84-138 (adj_bunch = 0,
+// mesh_size_cutoff_w = 0) on the device, and the cleared charge histogram
+// when it decides to remesh.  Thread 0 reads every input into registers
+// before it writes any output.
+__global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
+{
+  __shared__ RedSmem sm;
+  __shared__ int remesh;
+  const int t = threadIdx.x;
+  const BeamArrays& b = a.b;
+  hps_beam_stats* st = a.st;
+  ScDev* sc = a.sc;
+  double s[3] = {0.0, 0.0, 0.0};
+  if (a.do_sums) {
+    uint32_t dummy;
+    final_reduce<3>(a.pr.sums, nullptr, a.pr.mpad, a.pr.nch, sm.red, sm.ured, sm.mred, t, s,
+                    dummy);
+  }
+  if (t == 0) {
     const uint32_t good_cnt = st->num_good;
-    const double sigphi = b.n < 2 ? 0.0 : st->phi_sig_good;
-    const double sigx = b.n < 2 ? 0.0 : st->x_sig_good;
-    const double sigy = b.n < 2 ? 0.0 : st->y_sig_good;
+    const uint32_t num_loss = st->num_loss;
+    const double x_sig_good = st->x_sig_good, y_sig_good = st->y_sig_good;
     const double ke = st->w_avg_good;
+    double phi_sig_good = st->phi_sig_good;
+    hps_spch_state ss = sc->st;
+    if (a.do_sums) {
+      const double gd = (double)good_cnt;
+      const double avg = s[1] / gd, avg2 = s[2] / gd;
+      phi_sig_good = ::sqrt(avg2 - avg * avg);
+      st->phi_avg_r = s[0] / (double)(uint32_t)(b.n - num_loss);
+      st->phi_sig_good = phi_sig_good;
+    }
+    const double sigphi = b.n < 2 ? 0.0 : phi_sig_good;
+    const double sigx = b.n < 2 ? 0.0 : x_sig_good;
+    const double sigy = b.n < 2 ? 0.0 : y_sig_good;
     const double gamma = (double)ke / b.mass + 1.0;
     const double beta = ::sqrt(1.0 - 1.0 / (gamma * gamma));
     const double lmbd = 299.792458 / a.freq;
@@ -506,13 +517,10 @@ __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
     const double sigr = ::sqrt(sigx * sigx + sigy * sigy);
     const double cratior = sigr * sigz;
     const double cratioz = sigz * sigz * gamma * gamma;
-    hps_spch_state& ss = sc->st;
     const double cratior_old = ss.sigr_old * ss.sigz_old;
     const double cratioz_old = ss.sigz_old * ss.sigz_old * ss.gm_old * ss.gm_old;
-    sc->freq = a.freq;
-    sc->beta = beta;
-    sc->frac = a.frac;
-    sc->current = a.current;
+    double ratio_r = 1.0, ratio_z = 1.0, ratio_q = 1.0, ratio_g = 1.0;
+    int rm = 0;
     ss.kick_count += 1;
     if (cratior_old == 0.0 || cratioz_old == 0.0 ||
         __builtin_fabs(cratior - cratior_old) / cratior_old > ss.remesh_val ||
@@ -526,24 +534,33 @@ __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
       // compares the device pointers x_sig_good > y_sig_good (space_charge.cu:131);
       // x_sig_good is allocated first (beam.cu:51,54), so sigma_y is used.
       const double bl = beta * CLIGHT / a.freq;
-      ss.dr = 3.0 * st->y_sig_good / ss.nx;
-      ss.dz = 3.0 * bl * st->phi_sig_good / (ss.nz * PI);
+      ss.dr = 3.0 * y_sig_good / ss.nx;
+      ss.dz = 3.0 * bl * phi_sig_good / (ss.nz * PI);
       ss.num_on_mesh = 0;
-      sc->ratio_r = sc->ratio_z = sc->ratio_q = sc->ratio_g = 1.0;
-      sc->remesh = 1;
-      sc->valid = 1;
+      rm = 1;
     } else {
-      sc->ratio_r = sigr / ss.sigr_old;
-      sc->ratio_z = sigz / ss.sigz_old;
-      sc->ratio_q = ss.good_cnt_old ? (double)(good_cnt / ss.good_cnt_old) : 0.0;
-      sc->ratio_g = gamma / ss.gm_old;
-      sc->remesh = 0;
+      ratio_r = sigr / ss.sigr_old;
+      ratio_z = sigz / ss.sigz_old;
+      // uint / uint, space_charge.cpp:137-138
+      ratio_q = ss.good_cnt_old ? (double)(good_cnt / ss.good_cnt_old) : 0.0;
+      ratio_g = gamma / ss.gm_old;
     }
-    remesh = sc->remesh;
+    sc->st = ss;
+    sc->freq = a.freq;
+    sc->beta = beta;
+    sc->frac = a.frac;
+    sc->current = a.current;
+    sc->ratio_r = ratio_r;
+    sc->ratio_z = ratio_z;
+    sc->ratio_q = ratio_q;
+    sc->ratio_g = ratio_g;
+    sc->remesh = rm;
+    if (rm) sc->valid = 1;
+    remesh = rm;
   }
   __syncthreads();
   if (remesh) {
-    const uint32_t nb = a.sc->st.nx * a.sc->st.nz;
+    const uint32_t nb = sc->st.nx * sc->st.nz;
     for (uint32_t i = t; i < nb; i += kBlock) a.bins[i] = 0;
   }
 }
@@ -761,7 +778,13 @@ __global__ __launch_bounds__(kBlock) void k_sums(BeamArrays b, const double* pa,
     pr.cnt[pr.mpad + c] = (packed >> 11) & 0x7ffu;
     pr.cnt[2 * pr.mpad + c] = packed >> 22;
   }
-  if (!last_block(pr.ticket, t)) return;
+}
+
+// Cross-chunk trees of k_sums in one workgroup.
+__global__ __launch_bounds__(kBlock) void k_sums_final(Partials pr)
+{
+  __shared__ RedSmem sm;
+  const int t = threadIdx.x;
   double s[5];
   uint32_t c0, c1, c2;
   final_reduce<5>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, c0);
@@ -774,7 +797,6 @@ __global__ __launch_bounds__(kBlock) void k_sums(BeamArrays b, const double* pa,
     pr.out_cnt[0] = c0;
     pr.out_cnt[1] = c1;
     pr.out_cnt[2] = c2;
-    *pr.ticket = 0;
   }
 }
 
@@ -850,10 +872,22 @@ hipError_t launch_segment(const SegArgs& a, hipStream_t s)
   return hipGetLastError();
 }
 
+hipError_t launch_seg_final(const SegArgs& a, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_seg_final, dim3(1), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_phase(const PhaseArgs& a, hipStream_t s)
 {
-  const uint32_t nch = a.do_sums ? (a.b.n + kChunk - 1) / kChunk : 1;
+  const uint32_t nch = (a.b.n + kChunk - 1) / kChunk;
   hipLaunchKernelGGL(k_phase, dim3(nch), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_phase_final, dim3(1), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -891,6 +925,7 @@ hipError_t launch_sums(BeamArrays b, const double* a, const double* bb, int filt
 {
   const uint32_t nch = (b.n + kChunk - 1) / kChunk;
   hipLaunchKernelGGL(k_sums, dim3(nch), dim3(kBlock), 0, s, b, a, bb, filter, pr);
+  hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(kBlock), 0, s, pr);
   return hipGetLastError();
 }
 
