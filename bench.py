@@ -13,9 +13,11 @@ The JSON line also carries
                  average measured with HIP events (on the engine's stream)
                  around every launch of one extra eager step run after the
                  timed steps, which replay a captured HIP graph;
-  cpu_baseline:  the serial FP64 oracle (oracle/liboracle.so, one core) on a
-                 bounded sample of the same workload: the full beam through
-                 the first 40 lattice periods (about 20 s), timed on this host.
+  cpu_baseline:  the FP64 oracle built with OpenMP (oracle/liboracle_omp.so,
+                 bit-identical to the serial checker) on all host cores
+                 (OMP_NUM_THREADS) over the whole workload, timed on this
+                 host; "serial" inside it is the one-core oracle over the
+                 first 10 periods.
 """
 from __future__ import annotations
 
@@ -43,18 +45,23 @@ def parse():
     p.add_argument("--eager", action="store_true",
                    help="launch every kernel from the host instead of replaying a HIP graph "
                         "(for profilers that do not follow graph launches)")
-    p.add_argument("--cpu-periods", type=int, default=40,
-                   help="lattice periods of the CPU-baseline sample")
+    p.add_argument("--cpu-periods", type=int, default=200,
+                   help="lattice periods of the all-core CPU-baseline sample")
+    p.add_argument("--cpu-serial-periods", type=int, default=10,
+                   help="lattice periods of the one-core CPU-baseline sample")
     return p.parse_args()
 
 
-def cpu_baseline(wl, periods):
-    """Serial oracle on the first `periods` periods of the same workload."""
+def _oracle_run(wl, periods, openmp):
+    """The oracle on the first `periods` lattice periods of the workload;
+    returns (pushes/s, threads, seconds, elements)."""
     import copy
 
     from hpsim_amd import configs as cfg
     from oracle import oracle as O
 
+    if openmp:
+        O.use_openmp()
     w2 = copy.deepcopy(wl)
     elems = cfg.build_lattice(w2.lattice)[: 4 * periods]
     b = w2.beam
@@ -68,11 +75,30 @@ def cpu_baseline(wl, periods):
     t0 = time.perf_counter()
     sim.simulate()
     dt = time.perf_counter() - t0
-    pushes = b.num * len(elems)
-    return {"value": pushes / dt, "unit": "particle-element pushes/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{b.num} particles x first {len(elems)} elements "
-                      f"({periods} period(s)) of the same lattice, SCHEFF on, {dt:.1f} s"}
+    return b.num * len(elems) / dt, O.lib().orc_threads(), dt, len(elems)
+
+
+def cpu_baseline(wl, periods, serial_periods):
+    """OpenMP oracle on all host cores (OMP_NUM_THREADS) over `periods`
+    periods, plus the serial oracle on `serial_periods` periods in a child
+    process (one process holds one oracle build)."""
+    import subprocess
+
+    periods = min(periods, wl.lattice.periods)
+    rate, threads, dt, nel = _oracle_run(wl, periods, True)
+    out = {"value": rate, "unit": "particle-element pushes/s", "cores": threads,
+           "kind": "port",
+           "sample": f"{wl.beam.num} particles x first {nel} elements ({periods} periods) of "
+                     f"the same lattice, SCHEFF on, OpenMP oracle, {threads} threads, {dt:.1f} s"}
+    code = ("import json, sys; sys.path.insert(0, %r); import bench; "
+            "from hpsim_amd import configs as cfg; "
+            "print(json.dumps(bench._oracle_run(cfg.config(1, num=%d, periods=%d), %d, False)))"
+            % (ROOT, wl.beam.num, wl.lattice.periods, serial_periods))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True)
+    srate, _, sdt, snel = json.loads(r.stdout.strip().splitlines()[-1])
+    out["serial"] = {"value": srate, "cores": 1,
+                     "sample": f"first {snel} elements, serial oracle, {sdt:.1f} s"}
+    return out
 
 
 def main():
@@ -150,7 +176,7 @@ def main():
         "launch": "eager" if args.eager else "hip-graph",
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_periods)
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_periods, args.cpu_serial_periods)
     print(json.dumps(out), flush=True)
 
 

@@ -40,6 +40,22 @@ enum OpKind : int32_t {
   OP_APER_R = 8     // SimulateRectangularApertureKernel
 };
 
+// Map classes by register cost; a k_segment instantiation compiles only the
+// maps of its class and below, so segments without RF gaps (or quads) run
+// with fewer registers and more wavefronts in flight.
+enum OpClass : int32_t {
+  OPC_LIGHT = 0,    // drift, rotation, steerer, apertures
+  OPC_QUAD = 1,     // + half quad
+  OPC_GAP = 2       // + RF gap halves
+};
+
+__host__ __device__ inline int32_t op_class_of(int32_t kind)
+{
+  if (kind == OP_GAP1 || kind == OP_GAP2) return OPC_GAP;
+  if (kind == OP_QUAD) return OPC_QUAD;
+  return OPC_LIGHT;
+}
+
 // One map step.  a[] meaning per kind:
 //  DRIFT  h, aperture, wavelength
 //  QUAD   length, aperture, gradient, wavelength
@@ -366,18 +382,24 @@ __device__ __forceinline__ void steerer(Particle& p, double blh, double blv, dou
 }
 
 // Apply one map step to an alive particle.
+template <int kClass>
 __device__ __forceinline__ void apply_op(const Op& op, Particle& p, uint32_t index, double mass,
                                          double charge, double x_avg, double y_avg)
 {
   switch (op.kind) {
     case OP_DRIFT: drift(p, op.a[0], op.a[1], op.a[2], op.elem, mass); break;
-    case OP_QUAD: half_quad(p, op.a[0], op.a[1], op.a[2], op.a[3], op.elem, mass, charge); break;
+    case OP_QUAD:
+      if constexpr (kClass >= OPC_QUAD)
+        half_quad(p, op.a[0], op.a[1], op.a[2], op.a[3], op.elem, mass, charge);
+      break;
     case OP_GAP1:
-      gap_first_half(p, op.g, op.a[0], op.a[1], op.a[2], op.a[3], op.a[4], op.ccl != 0, mass);
+      if constexpr (kClass >= OPC_GAP)
+        gap_first_half(p, op.g, op.a[0], op.a[1], op.a[2], op.a[3], op.a[4], op.ccl != 0, mass);
       break;
     case OP_GAP2:
-      gap_second_half(p, op.g, op.a[0], op.a[1], op.a[2], op.a[3], op.a[4], op.a[5],
-                      op.ccl_cell_num, op.ccl != 0, mass, charge);
+      if constexpr (kClass >= OPC_GAP)
+        gap_second_half(p, op.g, op.a[0], op.a[1], op.a[2], op.a[3], op.a[4], op.a[5],
+                        op.ccl_cell_num, op.ccl != 0, mass, charge);
       break;
     case OP_ROT: rotation(p, op.a[0]); break;
     case OP_STEER: steerer(p, op.a[0], op.a[1], mass, charge); break;

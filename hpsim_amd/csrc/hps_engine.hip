@@ -275,6 +275,8 @@ struct hps_spch {
   double* fld1 = nullptr;
   double* fld2 = nullptr;
   int64_t* bins = nullptr;
+  int64_t* part = nullptr;   // [kDepositMaxRows][nr*nz] per-workgroup deposit rows
+  size_t max_lds = 0;        // LDS bytes per workgroup on this device
 };
 
 struct hps_sim {
@@ -772,9 +774,10 @@ int hps_beamline_index(const hps_beamline* bl, const char* name, uint32_t* out)
 // ---- SpaceCharge ----
 static int spch_alloc_tables(hps_spch* s)
 {
-  (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins);
-  s->fld1 = nullptr; s->fld2 = nullptr; s->bins = nullptr;
+  (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins); (void)hipFree(s->part);
+  s->fld1 = nullptr; s->fld2 = nullptr; s->bins = nullptr; s->part = nullptr;
   const uint32_t nr = s->cfg.nx, nz = s->cfg.nz;
+  HPS_CHECK(hipMalloc(&s->part, (size_t)kDepositMaxRows * nr * nz * 8));
   const size_t n1 = (size_t)(nr + 1) * nr * nz * 2, n2 = (size_t)(nz + 1) * (nr + 1) * 2;
   HPS_CHECK(hipMalloc(&s->fld1, n1 * 8));
   HPS_CHECK(hipMalloc(&s->fld2, n2 * 8));
@@ -812,6 +815,11 @@ int hps_spch_create(hps_spch** out, uint32_t nr, uint32_t nz, double interval, u
   s->cfg.adj_bunch = adj_bunch; s->cfg.fraction = 1.0;
   s->cfg.interval = interval; s->cfg.interval_init = interval;
   s->cfg.remesh_val = 0.05;
+  int lds = 0;
+  s->max_lds = 65536;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device) == hipSuccess &&
+      lds > 0)
+    s->max_lds = (size_t)lds;
   if (hipMalloc(&s->dev, sizeof(ScDev)) != hipSuccess ||
       hipMemset(s->dev, 0, sizeof(ScDev)) != hipSuccess || spch_alloc_tables(s) || spch_push_cfg(s)) {
     (void)hipFree(s->dev);
@@ -827,6 +835,7 @@ int hps_spch_destroy(hps_spch* s)
   if (!s) return 0;
   (void)hipStreamSynchronize(s->stream);
   (void)hipFree(s->dev); (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins);
+  (void)hipFree(s->part);
   delete s;
   return 0;
 }
@@ -1182,6 +1191,9 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   if (m->sc_on && m->sc) {
     if (m->sc->cfg.adj_bunch != 0) return fail("adjacent-bunch space charge (adj_bunch > 0) is not implemented");
     if (m->sc->cfg.mesh_w != 0.0) return fail("mesh_size_cutoff_w is not implemented");
+    // the deposit keeps one int64 histogram of nr * nz bins per workgroup in LDS
+    if ((size_t)m->sc->cfg.nx * m->sc->cfg.nz * 8 + kBlockLdsReserve > m->sc->max_lds)
+      return fail("the SCHEFF mesh has more bins than one workgroup's LDS histogram holds");
   }
   // SimulationEngine::Simulate, simulation_engine.cpp:73-95
   if (start_index < m->prev_end) hps_sim_reset(m);
@@ -1246,6 +1258,9 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       a.sc = m->sc ? m->sc->dev : nullptr;
       a.fld2 = m->sc ? m->sc->fld2 : nullptr;
       a.stats = st.stats;
+      a.op_class = OPC_LIGHT;
+      for (int32_t o = st.op_begin; o < st.op_end; ++o)
+        a.op_class = std::max(a.op_class, op_class_of(c.ops[o].kind));
       launches.push_back(l);
       if (a.stats != STAT_NONE) {
         l.kind = L_SEG_FINAL;
@@ -1279,8 +1294,11 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       case L_PHASE: return launch_phase(l.ph, st);
       case L_PHASE_FINAL: return launch_phase_final(l.ph, st);
       case L_GREENS: return launch_greens(sc->dev, sc->fld1, sc->cfg.nx, sc->cfg.nz, b->n, b->mass, st);
-      case L_DEPOSIT: return launch_deposit(b->d, b->st, sc->dev, sc->bins, sc->cfg.nx, sc->cfg.nz, st);
-      default: return launch_field(sc->dev, sc->bins, sc->fld1, sc->fld2, sc->cfg.nx, sc->cfg.nz, st);
+      case L_DEPOSIT:
+        return launch_deposit(b->d, b->st, sc->dev, sc->part, sc->bins, sc->cfg.nx, sc->cfg.nz, st);
+      default:
+        return launch_field(sc->dev, sc->bins, sc->fld1, sc->fld2, sc->cfg.nx, sc->cfg.nz,
+                            sc->max_lds, st);
     }
   };
 
@@ -1329,6 +1347,14 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       else put(&l.ph, sizeof l.ph);
     }
     put(c.ops.data(), c.ops.size() * sizeof(Op));
+    if (m->sc) {
+      const hps_spch* sc = m->sc;
+      const void* ptrs[5] = {sc->dev, sc->fld1, sc->fld2, sc->bins, sc->part};
+      put(ptrs, sizeof ptrs);
+      put(&sc->cfg.nx, sizeof sc->cfg.nx);
+      put(&sc->cfg.nz, sizeof sc->cfg.nz);
+      put(&sc->max_lds, sizeof sc->max_lds);
+    }
     if (!m->graph_exec || sig != m->graph_sig) {
       if (m->graph_exec) { (void)hipGraphExecDestroy(m->graph_exec); m->graph_exec = nullptr; }
       if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }

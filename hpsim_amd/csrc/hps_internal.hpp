@@ -56,6 +56,7 @@ struct SegArgs {
   const ScDev* sc;
   const double* fld2;      // (nz+1)*(nr+1) double2, [zp][rp]
   int32_t stats;           // StatKind computed after the maps
+  int32_t op_class;        // OpClass: heaviest map in [op_begin, op_end)
 };
 
 struct PhaseArgs {
@@ -77,10 +78,19 @@ hipError_t launch_phase(const PhaseArgs& a, hipStream_t s);
 hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s);
 hipError_t launch_greens(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, uint32_t num,
                          double mass, hipStream_t s);
-hipError_t launch_deposit(BeamArrays b, const hps_beam_stats* st, ScDev* sc, int64_t* bins,
-                          uint32_t nr, uint32_t nz, hipStream_t s);
+// k_deposit workgroup size (four particles per thread at full grid) and the
+// LDS it uses besides its histogram.
+constexpr uint32_t kDepositBlock = 1024;
+constexpr size_t kBlockLdsReserve = 4 * kDepositBlock;
+// k_deposit workgroups, each writing one row of partial histograms.
+constexpr uint32_t kDepositMaxRows = 256;
+uint32_t deposit_rows(uint32_t n);
+hipError_t launch_deposit(BeamArrays b, const hps_beam_stats* st, ScDev* sc, int64_t* part,
+                          int64_t* bins, uint32_t nr, uint32_t nz, hipStream_t s);
+// max_lds: LDS bytes a workgroup may use; the field kernel stages its
+// inputs there when they fit.
 hipError_t launch_field(const ScDev* sc, const int64_t* bins, const double* fld1, double* fld2,
-                        uint32_t nr, uint32_t nz, hipStream_t s);
+                        uint32_t nr, uint32_t nz, size_t max_lds, hipStream_t s);
 
 // Generic canonical sums for the getters: over the filtered particles
 // (filter 0: loss == 0, 1: loss + lloss == 0, 2: all) of a, a*a, b, b*b, a*b,

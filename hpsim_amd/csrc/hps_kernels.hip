@@ -237,8 +237,11 @@ struct RedSmem {
   uint32_t ured[kBlock];
 };
 
+// Recomputes the statistics partials of chunk c from memory (after k_seg_final
+// set new lloss flags in it); the same lane order as the accumulation in
+// k_segment.
 __device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem& sm,
-                               const Partials& pr, const Particle* p, bool fresh)
+                               const Partials& pr)
 {
   double acc[kScQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   uint32_t cnt = 0;
@@ -249,8 +252,7 @@ __device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem& 
     double phi = 0.0, x = 0.0, y = 0.0, w = 0.0;
     uint32_t loss = 1, lloss = 0;
     if (i < b.n) {
-      if (fresh) { phi = b.phi[i]; x = b.x[i]; y = b.y[i]; w = b.w[i]; loss = b.loss[i]; lloss = b.lloss[i]; }
-      else { phi = p[j].phi; x = p[j].x; y = p[j].y; w = p[j].w; loss = p[j].loss; lloss = p[j].lloss; }
+      phi = b.phi[i]; x = b.x[i]; y = b.y[i]; w = b.w[i]; loss = b.loss[i]; lloss = b.lloss[i];
       if (lloss == 0) mx = ::fmax(mx, phi);
     }
     const bool g = (i < b.n) && is_good(loss, lloss);
@@ -272,79 +274,86 @@ __device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem& 
   }
 }
 
-// Aperture statistics (UpdateLoss + UpdateAvgXY, simulation_engine.cu:311-312):
-// count of loss != 0, sums of x and y over loss == 0.
-__device__ void aper_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem& sm,
-                                 const Partials& pr, const Particle* p)
-{
-  double acc[2] = {0.0, 0.0};
-  uint32_t nl = 0;
-#pragma unroll
-  for (int j = 0; j < (int)kPerLane; ++j) {
-    const uint32_t i = c * kChunk + j * kBlock + t;
-    const bool v = i < b.n;
-    const bool ok = v && p[j].loss == 0;
-    acc[0] = acc[0] + (ok ? p[j].x : 0.0);
-    acc[1] = acc[1] + (ok ? p[j].y : 0.0);
-    nl += (v && p[j].loss != 0) ? 1u : 0u;
-  }
-  double mx = 0.0;
-  block_reduce<2>(acc, nl, mx, sm.red, sm.ured, sm.mred, t);
-  if (t == 0) { pr.sums[c] = acc[0]; pr.sums[pr.mpad + c] = acc[1]; pr.cnt[c] = nl; }
-}
-
+// One workgroup per 1024-particle chunk; each lane carries its four
+// particles (i = chunk*1024 + j*256 + lane) one after the other through the
+// kick, the maps and the store, and accumulates the chunk statistics in the
+// canonical lane order (j = 0..3, from +0.0):
+//   STAT_SC        0 phi, 1 x, 2 x^2, 3 y, 4 y^2, 5 w over (loss + lloss) == 0,
+//                  the good count and the largest phi among particles not yet
+//                  flagged as longitudinally lost;
+//   STAT_APERTURE  x and y over loss == 0 and the count of loss != 0
+//                  (UpdateLoss + UpdateAvgXY, simulation_engine.cu:311-312).
+// One particle in flight per lane keeps the register count low enough for
+// four wavefronts per SIMD.
+template <int kClass>
 __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
 {
   __shared__ RedSmem sm;
   const int t = threadIdx.x;
   const uint32_t c = blockIdx.x;
   const BeamArrays& b = a.b;
-  Particle p[kPerLane];
-  bool alive_in[kPerLane];
-#pragma unroll
-  for (int j = 0; j < (int)kPerLane; ++j) {
-    const uint32_t i = c * kChunk + j * kBlock + t;
-    alive_in[j] = false;
-    if (i < b.n) {
-      p[j].loss = b.loss[i];
-      p[j].lloss = b.lloss[i];
-      alive_in[j] = p[j].loss == 0;
-      p[j].x = b.x[i]; p[j].xp = b.xp[i]; p[j].y = b.y[i]; p[j].yp = b.yp[i];
-      p[j].phi = b.phi[i]; p[j].w = b.w[i];
-    } else {
-      p[j].loss = 1; p[j].lloss = 0;
-      p[j].x = p[j].xp = p[j].y = p[j].yp = p[j].phi = p[j].w = 0.0;
-    }
-  }
-  if (a.kick) {
-    const KickConst k = kick_const(a.sc, a.st, a.kick_len, b.mass, b.n);
-#pragma unroll
-    for (int j = 0; j < (int)kPerLane; ++j) {
-      const uint32_t i = c * kChunk + j * kBlock + t;
-      if (alive_in[j] && p[j].lloss == 0 && (i & a.kick_mask) != 0) kick(p[j], k, a.fld2, b.mass);
-    }
-  }
+  KickConst k{};
+  if (a.kick) k = kick_const(a.sc, a.st, a.kick_len, b.mass, b.n);
   const double xa = a.st->x_avg, ya = a.st->y_avg;
-  for (int o = a.op_begin; o < a.op_end; ++o) {
-    const Op& op = a.ops[o];
-#pragma unroll
-    for (int j = 0; j < (int)kPerLane; ++j) {
-      const uint32_t i = c * kChunk + j * kBlock + t;
-      if (p[j].loss == 0) apply_op(op, p[j], i, b.mass, b.charge, xa, ya);
-    }
-  }
-#pragma unroll
+  double acc[kScQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  uint32_t cnt = 0;
+  double mx = -__builtin_huge_val();
+#pragma unroll 1
   for (int j = 0; j < (int)kPerLane; ++j) {
     const uint32_t i = c * kChunk + j * kBlock + t;
-    if (alive_in[j]) {
-      b.x[i] = p[j].x; b.xp[i] = p[j].xp; b.y[i] = p[j].y; b.yp[i] = p[j].yp;
-      b.phi[i] = p[j].phi; b.w[i] = p[j].w;
-      if (p[j].loss != 0) b.loss[i] = p[j].loss;
+    const bool in = i < b.n;
+    Particle p;
+    p.loss = 1; p.lloss = 0;
+    p.x = p.xp = p.y = p.yp = p.phi = p.w = 0.0;
+    if (in) {
+      p.loss = b.loss[i];
+      p.lloss = b.lloss[i];
+      p.x = b.x[i]; p.xp = b.xp[i]; p.y = b.y[i]; p.yp = b.yp[i];
+      p.phi = b.phi[i]; p.w = b.w[i];
+    }
+    const bool alive_in = in && p.loss == 0;
+    if (a.kick && alive_in && p.lloss == 0 && (i & a.kick_mask) != 0) kick(p, k, a.fld2, b.mass);
+    for (int o = a.op_begin; o < a.op_end; ++o)
+      if (p.loss == 0) apply_op<kClass>(a.ops[o], p, i, b.mass, b.charge, xa, ya);
+    if (alive_in) {
+      b.x[i] = p.x; b.xp[i] = p.xp; b.y[i] = p.y; b.yp[i] = p.yp;
+      b.phi[i] = p.phi; b.w[i] = p.w;
+      if (p.loss != 0) b.loss[i] = p.loss;
+    }
+    if (a.stats == STAT_SC) {
+      if (in && p.lloss == 0) mx = ::fmax(mx, p.phi);
+      const bool g = in && is_good(p.loss, p.lloss);
+      const double vphi = g ? p.phi : 0.0, vx = g ? p.x : 0.0, vy = g ? p.y : 0.0,
+                   vw = g ? p.w : 0.0;
+      acc[0] = acc[0] + vphi;
+      acc[1] = acc[1] + vx;
+      acc[2] = acc[2] + vx * vx;
+      acc[3] = acc[3] + vy;
+      acc[4] = acc[4] + vy * vy;
+      acc[5] = acc[5] + vw;
+      cnt += g ? 1u : 0u;
+    } else if (a.stats == STAT_APERTURE) {
+      const bool ok = in && p.loss == 0;
+      acc[0] = acc[0] + (ok ? p.x : 0.0);
+      acc[1] = acc[1] + (ok ? p.y : 0.0);
+      cnt += (in && p.loss != 0) ? 1u : 0u;
     }
   }
-  if (a.stats == STAT_NONE) return;
-  if (a.stats == STAT_APERTURE) aper_chunk_stats(b, c, t, sm, a.pr, p);
-  else sc_chunk_stats(b, c, t, sm, a.pr, p, false);
+  const Partials& pr = a.pr;
+  if (a.stats == STAT_SC) {
+    block_reduce<kScQ>(acc, cnt, mx, sm.red, sm.ured, sm.mred, t);
+    if (t == 0) {
+#pragma unroll
+      for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = acc[q];
+      pr.cnt[c] = cnt;
+      pr.pmax[c] = mx;
+    }
+  } else if (a.stats == STAT_APERTURE) {
+    double ap[2] = {acc[0], acc[1]};
+    double m0 = 0.0;
+    block_reduce<2>(ap, cnt, m0, sm.red, sm.ured, sm.mred, t);
+    if (t == 0) { pr.sums[c] = ap[0]; pr.sums[pr.mpad + c] = ap[1]; pr.cnt[c] = cnt; }
+  }
 }
 
 // Cross-chunk step of the k_segment statistics in one workgroup, launched
@@ -377,16 +386,32 @@ __global__ __launch_bounds__(kBlock) void k_seg_final(SegArgs a)
   // STAT_SC: the longitudinal-loss update of Beam::UpdateLongitudinalLoss
   // (beam.cpp:776-782), then the good moments of beam.cpp:1008-1014.
   hps_beam_stats* st = a.st;
+  const uint32_t num_loss = st->num_loss;
+  // the chunk maxima are loaded together with the partials (one round trip)
+  const uint32_t K = pr.mpad > (uint32_t)kBlock ? pr.mpad / kBlock : 1;
+  double pm[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t cc = t + kBlock * k;
+    pm[k] = ((uint32_t)k < K && cc < pr.nch) ? pr.pmax[cc] : -__builtin_huge_val();
+  }
   double s[kScQ];
   uint32_t g1;
-  final_reduce<1>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t,
-                  *reinterpret_cast<double(*)[1]>(s), g1);
+  final_reduce<kScQ>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, g1);
   const double avg1 = g1 > 0 ? s[0] / (double)g1 : -99999;
   const double thr = avg1 + TWOPI + PI;
   // Chunks where a not-yet-flagged particle is beyond the threshold get their
   // lloss flags set and their partials recomputed (rare: once per particle).
+  if (t == 0) rescan_n = 0;
+  __syncthreads();
+  bool hit = false;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) hit = hit || pm[k] > thr;
+  if (hit) atomicAdd(&rescan_n, 1u);
+  __syncthreads();
   bool any = false;
-  for (uint32_t it = 0; it < pr.nch / 64 + 2; ++it) {
+  for (uint32_t it = 0; rescan_n != 0 && it < pr.nch / 64 + 2; ++it) {
+    __syncthreads();
     if (t == 0) rescan_n = 0;
     __syncthreads();
     for (uint32_t cc = t; cc < pr.nch; cc += kBlock) {
@@ -408,25 +433,17 @@ __global__ __launch_bounds__(kBlock) void k_seg_final(SegArgs a)
         if (i < b.n && b.phi[i] > thr) b.lloss[i] = 1;
       }
       __syncthreads();
-      sc_chunk_stats(b, cc, t, sm, pr, nullptr, true);
+      sc_chunk_stats(b, cc, t, sm, pr);
       __syncthreads();
     }
   }
   uint32_t g2 = g1;
-  if (any) {
+  if (any)
     final_reduce<kScQ>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, g2);
-  } else {
-    double r5[kScQ - 1];
-    uint32_t dummy;
-    final_reduce<kScQ - 1>(pr.sums + pr.mpad, nullptr, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred,
-                           t, r5, dummy);
-#pragma unroll
-    for (int q = 1; q < kScQ; ++q) s[q] = r5[q - 1];
-  }
   if (t == 0) {
     st->num_good = g2;
     st->phi_avg_good = g2 > 0 ? s[0] / (double)g2 : -99999;
-    const double surv = (double)(uint32_t)(b.n - st->num_loss);
+    const double surv = (double)(uint32_t)(b.n - num_loss);
     const double xavg = s[1] / surv, x2avg = s[2] / surv, yavg = s[3] / surv, y2avg = s[4] / surv;
     st->x_avg_good = xavg;
     st->x_sig_good = ::sqrt(x2avg - xavg * xavg);
@@ -483,10 +500,25 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
 {
   __shared__ RedSmem sm;
   __shared__ int remesh;
+  __shared__ uint32_t nbins;
   const int t = threadIdx.x;
   const BeamArrays& b = a.b;
   hps_beam_stats* st = a.st;
   ScDev* sc = a.sc;
+  // thread 0's inputs, loaded before the reduction so that both round trips
+  // overlap
+  uint32_t good_cnt = 0, num_loss = 0;
+  double x_sig_good = 0.0, y_sig_good = 0.0, ke = 0.0, phi_sig_good = 0.0;
+  hps_spch_state ss{};
+  if (t == 0) {
+    good_cnt = st->num_good;
+    num_loss = st->num_loss;
+    x_sig_good = st->x_sig_good;
+    y_sig_good = st->y_sig_good;
+    ke = st->w_avg_good;
+    phi_sig_good = st->phi_sig_good;
+    ss = sc->st;
+  }
   double s[3] = {0.0, 0.0, 0.0};
   if (a.do_sums) {
     uint32_t dummy;
@@ -494,12 +526,6 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
                     dummy);
   }
   if (t == 0) {
-    const uint32_t good_cnt = st->num_good;
-    const uint32_t num_loss = st->num_loss;
-    const double x_sig_good = st->x_sig_good, y_sig_good = st->y_sig_good;
-    const double ke = st->w_avg_good;
-    double phi_sig_good = st->phi_sig_good;
-    hps_spch_state ss = sc->st;
     if (a.do_sums) {
       const double gd = (double)good_cnt;
       const double avg = s[1] / gd, avg2 = s[2] / gd;
@@ -557,12 +583,11 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
     sc->remesh = rm;
     if (rm) sc->valid = 1;
     remesh = rm;
+    nbins = ss.nx * ss.nz;
   }
   __syncthreads();
-  if (remesh) {
-    const uint32_t nb = sc->st.nx * sc->st.nz;
-    for (uint32_t i = t; i < nb; i += kBlock) a.bins[i] = 0;
-  }
+  if (remesh)
+    for (uint32_t i = t; i < nbins; i += kBlock) a.bins[i] = 0;
 }
 
 // --- SCHEFF rebuild ----------------------------------------------------------------
@@ -610,16 +635,19 @@ __global__ __launch_bounds__(kBlock) void k_greens(const ScDev* sc, double* fld1
 }
 
 // Charge deposit, DistributeParticleKernel (space_charge_kernel.cu:122-224),
-// into an LDS-private int64 fixed-point histogram merged with integer atomics.
-__global__ __launch_bounds__(kBlock) void k_deposit(BeamArrays b, const hps_beam_stats* st,
-                                                    ScDev* sc, int64_t* bins)
+// into an LDS-private int64 fixed-point histogram per workgroup, stored as
+// that workgroup's row of part[gridDim.x][nb]; k_bins_reduce adds the rows.
+// Integer sums make the result independent of the order.
+__global__ __launch_bounds__(kDepositBlock) void k_deposit(BeamArrays b,
+                                                           const hps_beam_stats* st, ScDev* sc,
+                                                           int64_t* part)
 {
   extern __shared__ unsigned long long hist[];
-  __shared__ uint32_t ured[kBlock];
+  __shared__ uint32_t ured[kDepositBlock];
   if (!sc->remesh) return;
   const int t = threadIdx.x;
   const uint32_t nx = sc->st.nx, nz = sc->st.nz, nb = nx * nz;
-  for (uint32_t i = t; i < nb; i += kBlock) hist[i] = 0ull;
+  for (uint32_t i = t; i < nb; i += kDepositBlock) hist[i] = 0ull;
   __syncthreads();
   const double r_beta = sc->beta, r_freq = sc->freq;
   const double phimc = st->phi_avg_r;
@@ -634,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit(BeamArrays b, const hps_beam
   const double c2 = r_beta * gm * wave_len / TWOPI;
   const double drdr = dr * dr;
   uint32_t count = 0;
-  for (uint32_t np = blockIdx.x * kBlock + t; np < b.n; np += gridDim.x * kBlock) {
+  for (uint32_t np = blockIdx.x * kDepositBlock + t; np < b.n; np += gridDim.x * kDepositBlock) {
     if (!(b.loss[np] == 0 && b.lloss[np] == 0)) continue;
     const double x = b.x[np], y = b.y[np];
     double z = relative_phi(b.phi[np], ref);
@@ -684,13 +712,11 @@ __global__ __launch_bounds__(kBlock) void k_deposit(BeamArrays b, const hps_beam
     ++count;
   }
   __syncthreads();
-  for (uint32_t i = t; i < nb; i += kBlock) {
-    const unsigned long long v = hist[i];
-    if (v != 0ull) atomicAdd((unsigned long long*)&bins[i], v);
-  }
+  int64_t* row = part + (size_t)blockIdx.x * nb;
+  for (uint32_t i = t; i < nb; i += kDepositBlock) row[i] = (int64_t)hist[i];
   ured[t] = count;
   __syncthreads();
-  for (int o = kBlock / 2; o >= 1; o >>= 1) {
+  for (int o = kDepositBlock / 2; o >= 1; o >>= 1) {
     if (t < o) ured[t] += ured[t + o];
     __syncthreads();
   }
@@ -737,6 +763,108 @@ __global__ __launch_bounds__(kBlock) void k_field(const ScDev* sc, const int64_t
   if (lane == 0) {
     fld2[2 * (size_t)wave] = tx;
     fld2[2 * (size_t)wave + 1] = ty;
+  }
+}
+
+// bins[i] = sum over the k_deposit workgroups' rows of part[.][i]: a
+// workgroup takes 16 bins, its 16 row slices each add every 16th row, and
+// the slices are combined in LDS (integer adds, any order).
+__global__ __launch_bounds__(kBlock) void k_bins_reduce(const ScDev* sc, const int64_t* part,
+                                                        int64_t* bins, uint32_t nrows,
+                                                        uint32_t nb)
+{
+  __shared__ int64_t acc[kBlock];
+  if (!sc->remesh) return;
+  const int t = threadIdx.x;
+  const uint32_t bin = blockIdx.x * 16 + (t & 15);
+  const uint32_t slice = t >> 4;
+  int64_t v = 0;
+  if (bin < nb) {
+#pragma unroll 4
+    for (uint32_t r = slice; r < nrows; r += 16) v += part[(size_t)r * nb + bin];
+  }
+  acc[t] = v;
+  __syncthreads();
+  if (t < 16 && bin < nb) {
+    int64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) sum += acc[t + 16 * k];
+    bins[bin] = sum;
+  }
+}
+
+// k_field with the charge bins (as FP64) and the Green's slice of one source
+// radius rp staged in LDS.  Workgroup = (rp, zpb consecutive field rows zp);
+// each wavefront computes two field points zp, zp + 1 with the same lane
+// chains and halving tree as k_field (identical tables), the two points
+// giving four independent FMA chains per lane.  Needs 24 * nr * nz bytes
+// of LDS.
+constexpr int kFieldOpw = 2;
+__global__ __launch_bounds__(1024) void k_field_lds(const ScDev* sc, const int64_t* bins,
+                                                    const double* fld1, double* fld2,
+                                                    uint32_t nr, uint32_t nz, uint32_t zpb)
+{
+  extern __shared__ double lds[];
+  if (!sc->remesh) return;
+  const uint32_t fld_sz = nr * nz;
+  double2* lg = reinterpret_cast<double2*>(lds);   // [nz][nr] (Er, Ez), 16-byte aligned
+  double* lb = lds + 2 * (size_t)fld_sz;             // [fld_sz] charge
+  const uint32_t rp = blockIdx.x % (nr + 1);
+  const uint32_t zgrp = blockIdx.x / (nr + 1);
+  const double2* g = reinterpret_cast<const double2*>(fld1) + (size_t)rp * nz * nr;
+  for (uint32_t i = threadIdx.x; i < fld_sz; i += blockDim.x) {
+    lb[i] = (double)bins[i] * 0x1p-36;
+    lg[i] = g[i];
+  }
+  __syncthreads();
+  const uint32_t zbeg = zgrp * zpb;
+  const uint32_t zend = zbeg + zpb < nz + 1 ? zbeg + zpb : nz + 1;
+  const uint32_t zp0 = zbeg + (threadIdx.x >> 6) * kFieldOpw;
+  if (zp0 >= zend) return;
+  const bool two = zp0 + 1 < zend;
+  const uint32_t zp1 = two ? zp0 + 1 : zp0;
+  const int lane = threadIdx.x & 63;
+  const uint32_t dq = 64 / nr, dm = 64 % nr;
+  uint32_t zs = (uint32_t)lane / nr, rs = (uint32_t)lane % nr;
+  double tx0 = 0.0, ty0 = 0.0, tx1 = 0.0, ty1 = 0.0;
+#pragma unroll 2
+  for (uint32_t bi = lane; bi < fld_sz; bi += 64) {
+    const double bin = lb[bi];
+    double sign0 = 1.0, sign1 = 1.0;
+    uint32_t dz0, dz1;
+    if (zs >= zp0) { dz0 = zs - zp0; sign0 = -1.0; }
+    else dz0 = zp0 - zs - 1;
+    if (zs >= zp1) { dz1 = zs - zp1; sign1 = -1.0; }
+    else dz1 = zp1 - zs - 1;
+    const double2 g0 = lg[dz0 * nr + rs];
+    const double2 g1 = lg[dz1 * nr + rs];
+    tx0 = ::fma(bin, g0.x, tx0);
+    ty0 = ::fma(sign0 * bin, g0.y, ty0);
+    tx1 = ::fma(bin, g1.x, tx1);
+    ty1 = ::fma(sign1 * bin, g1.y, ty1);
+    rs += dm; zs += dq;
+    if (rs >= nr) { rs -= nr; zs += 1; }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double a0 = __shfl_down(tx0, o, 64);
+    const double b0 = __shfl_down(ty0, o, 64);
+    const double a1 = __shfl_down(tx1, o, 64);
+    const double b1 = __shfl_down(ty1, o, 64);
+    tx0 = tx0 + a0;
+    ty0 = ty0 + b0;
+    tx1 = tx1 + a1;
+    ty1 = ty1 + b1;
+  }
+  if (lane == 0) {
+    const size_t o0 = (size_t)zp0 * (nr + 1) + rp;
+    fld2[2 * o0] = tx0;
+    fld2[2 * o0 + 1] = ty0;
+    if (two) {
+      const size_t o1 = (size_t)zp1 * (nr + 1) + rp;
+      fld2[2 * o1] = tx1;
+      fld2[2 * o1 + 1] = ty1;
+    }
   }
 }
 
@@ -868,7 +996,12 @@ inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 hipError_t launch_segment(const SegArgs& a, hipStream_t s)
 {
   const uint32_t nch = (a.b.n + kChunk - 1) / kChunk;
-  hipLaunchKernelGGL(k_segment, dim3(nch), dim3(kBlock), 0, s, a);
+  if (a.op_class >= OPC_GAP)
+    hipLaunchKernelGGL(k_segment<OPC_GAP>, dim3(nch), dim3(kBlock), 0, s, a);
+  else if (a.op_class == OPC_QUAD)
+    hipLaunchKernelGGL(k_segment<OPC_QUAD>, dim3(nch), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_segment<OPC_LIGHT>, dim3(nch), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -900,20 +1033,43 @@ hipError_t launch_greens(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, uint
   return hipGetLastError();
 }
 
-hipError_t launch_deposit(BeamArrays b, const hps_beam_stats* st, ScDev* sc, int64_t* bins,
-                          uint32_t nr, uint32_t nz, hipStream_t s)
+uint32_t deposit_rows(uint32_t n)
 {
-  const size_t lds = (size_t)nr * nz * sizeof(unsigned long long);
-  uint32_t grid = (b.n + 4095) / 4096;
+  uint32_t grid = (n + 4 * kDepositBlock - 1) / (4 * kDepositBlock);
   if (grid < 1) grid = 1;
-  if (grid > 256) grid = 256;
-  hipLaunchKernelGGL(k_deposit, dim3(grid), dim3(kBlock), lds, s, b, st, sc, bins);
+  if (grid > kDepositMaxRows) grid = kDepositMaxRows;
+  return grid;
+}
+
+hipError_t launch_deposit(BeamArrays b, const hps_beam_stats* st, ScDev* sc, int64_t* part,
+                          int64_t* bins, uint32_t nr, uint32_t nz, hipStream_t s)
+{
+  const uint32_t nb = nr * nz;
+  const size_t lds = (size_t)nb * sizeof(unsigned long long);
+  const uint32_t rows = deposit_rows(b.n);
+  hipLaunchKernelGGL(k_deposit, dim3(rows), dim3(kDepositBlock), lds, s, b, st, sc, part);
+  hipLaunchKernelGGL(k_bins_reduce, dim3((nb + 15) / 16), dim3(kBlock), 0, s, sc, part, bins,
+                     rows, nb);
   return hipGetLastError();
 }
 
 hipError_t launch_field(const ScDev* sc, const int64_t* bins, const double* fld1, double* fld2,
-                        uint32_t nr, uint32_t nz, hipStream_t s)
+                        uint32_t nr, uint32_t nz, size_t max_lds, hipStream_t s)
 {
+  const size_t lds = (size_t)24 * nr * nz;
+  if (lds <= max_lds) {
+    // about one workgroup per CU: split the nz + 1 field rows of each rp
+    // into groups of at most 32 rows (16 wavefronts)
+    uint32_t groups = 256 / (nr + 1);
+    if (groups < 1) groups = 1;
+    uint32_t zpb = (nz + 1 + groups - 1) / groups;
+    if (zpb > 16 * kFieldOpw) zpb = 16 * kFieldOpw;
+    groups = (nz + 1 + zpb - 1) / zpb;
+    const uint32_t waves = (zpb + kFieldOpw - 1) / kFieldOpw;
+    hipLaunchKernelGGL(k_field_lds, dim3((nr + 1) * groups), dim3(waves * 64), lds, s, sc, bins,
+                       fld1, fld2, nr, nz, zpb);
+    return hipGetLastError();
+  }
   const uint32_t nout = (nz + 1) * (nr + 1);
   const uint32_t grid = (nout + 3) / 4;
   hipLaunchKernelGGL(k_field, dim3(grid), dim3(kBlock), 0, s, sc, bins, fld1, fld2, nr, nz);

@@ -6,6 +6,11 @@
  * in the product links or calls this file.  Every function cites the
  * reference file:line it restates (paths relative to the reference root).
  *
+ * The particle, chunk and table loops carry OpenMP pragmas: every iteration
+ * writes only its own index (the deposit adds integers atomically), so the
+ * OpenMP build liboracle_omp.so gives bit-identical results to the serial
+ * liboracle.so, which ignores the pragmas.
+ *
  * Deliberate, documented differences from the reference (applied identically
  * in the GPU product so that parity is bit-exact):
  *  - Reductions use one canonical order (ORC_CANON_*: each of 256 lanes sums
@@ -32,6 +37,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define CLIGHT 299.792458                       /* constant.h:7 */
 #define PI 3.14159265358979323846              /* constant.h:8 */
@@ -59,8 +67,9 @@ static double canon_sum(const double* v, uint32_t n)
   if (nch == 0) nch = 1;
   uint32_t m = orc_t_nextpow2(nch);
   double* part = (double*)calloc(m, sizeof(double));
-  double lane[ORC_CANON_LANES];
+  #pragma omp parallel for schedule(static)
   for (uint32_t c = 0; c < nch; ++c) {
+    double lane[ORC_CANON_LANES];
     for (uint32_t t = 0; t < ORC_CANON_LANES; ++t) {
       double acc = 0.0;
       for (uint32_t j = 0; j < ORC_CANON_PER_LANE; ++j) {
@@ -80,6 +89,16 @@ static double canon_sum(const double* v, uint32_t n)
   return r;
 }
 double orc_t_canon_sum(const double* v, uint32_t n) { return canon_sum(v, n); }
+
+/* Threads the loops of this build use (1 for the serial build). */
+int orc_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 /* ------------------------------------------------------------------ */
 /* Beam                                                                */
@@ -162,6 +181,7 @@ static inline int is_good(const orc_beam* b, uint32_t i) { return (uint32_t)(b->
 
 static double sum_filtered(orc_beam* b, const double* a, int good, int square)
 {
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) {
     int inc = good ? is_good(b, i) : is_tgood(b, i);
     double v = inc ? a[i] : 0.0;
@@ -172,6 +192,7 @@ static double sum_filtered(orc_beam* b, const double* a, int good, int square)
 
 static double sum_product(orc_beam* b, const double* a1, const double* a2)
 {
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i)
     b->tmp[i] = is_tgood(b, i) ? a1[i] * a2[i] : 0.0;
   return canon_sum(b->tmp, b->n);
@@ -181,6 +202,7 @@ static double sum_product(orc_beam* b, const double* a1, const double* a2)
 void orc_beam_update_loss(orc_beam* b)
 {
   uint32_t c = 0;
+  #pragma omp parallel for reduction(+:c) schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) c += (b->loss[i] != 0);
   b->st.num_loss = c;
 }
@@ -188,6 +210,7 @@ void orc_beam_update_loss(orc_beam* b)
 static void update_lloss_count(orc_beam* b)
 {
   uint32_t c = 0;
+  #pragma omp parallel for reduction(+:c) schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) c += (b->lloss[i] != 0);
   b->st.num_lloss = c;
 }
@@ -196,6 +219,7 @@ static void update_lloss_count(orc_beam* b)
 static void update_good_count(orc_beam* b)
 {
   uint32_t c = 0;
+  #pragma omp parallel for reduction(+:c) schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) c += is_good(b, i);
   b->st.num_good = c;
 }
@@ -229,6 +253,7 @@ static double sig_of(orc_beam* b, const double* a, int good)
 static void update_relative_phi(orc_beam* b, int use_good)
 {
   double ref = use_good ? b->st.phi_avg_good : b->phi[0];
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) {
     double tmp = b->phi[i] - ref + PI;
     double sign = 1.0;
@@ -244,6 +269,7 @@ static void update_longitudinal_loss(orc_beam* b)
   update_good_count(b);
   b->st.phi_avg_good = avg_of(b, b->phi, 1);
   double thr = b->st.phi_avg_good + TWOPI + PI;
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i)
     if (b->phi[i] > thr) b->lloss[i] = 1;
   update_lloss_count(b);
@@ -414,6 +440,7 @@ static void change_frequency(orc_beam* b, double f)
 {
   double ratio = f / b->freq;
   b->current *= ratio;
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) b->phi[i] *= ratio;
   b->freq = f;
   b->st.phi_avg = avg_of(b, b->phi, 0);
@@ -702,6 +729,7 @@ void orc_t_ttf(double beta, const orc_gap_param* g, double* t, double* tp, doubl
 /* SimulatePartialDriftKernel, simulation_kernel.cu:86-118 */
 static void map_partial_drift(orc_beam* b, double wavelen, double h, double aper, uint32_t idx)
 {
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) {
     if (b->loss[i] != 0) continue;
     double aper2 = aper * aper;
@@ -728,6 +756,7 @@ static void map_half_quad(orc_beam* b, double wavelen, double length, double ape
   double aper2 = aper;
   aper2 *= aper2;
   double qg = gradient * b->charge;
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) {
     if (b->loss[i] != 0) continue;
     double x = b->x[i], y = b->y[i], xp = b->xp[i], yp = b->yp[i];
@@ -772,6 +801,7 @@ static void map_gap_first_half(orc_beam* b, double design_w_in, double phi_in,
 {
   double wave_len = CLIGHT / gap->frequency;
   *wavelen = wave_len;
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) {
     if (b->loss[i] != 0) continue;
     double x = b->x[i], y = b->y[i], xp = b->xp[i], yp = b->yp[i];
@@ -813,6 +843,7 @@ static void map_gap_second_half(orc_beam* b, double wavelen, double design_w_in,
     double phi_in, const orc_gap_param* gap, double length, uint32_t ccl_cell_num,
     double q1, double q2, int ccl)
 {
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) {
     if (b->loss[i] != 0) continue;
     double cell_len = length + q1 * 0.5 + q2 * 0.5;
@@ -934,6 +965,7 @@ static void map_rotation(orc_beam* b, double angle)
 {
   double sina, cosa;
   orc_sincos(angle, &sina, &cosa);
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) {
     if (b->loss[i] != 0) continue;
     double x = b->x[i], y = b->y[i], xp = b->xp[i], yp = b->yp[i];
@@ -947,6 +979,7 @@ static void map_rotation(orc_beam* b, double angle)
 /* SimulateSteererKernel, simulation_kernel.cu:120-140 */
 static void map_steerer(orc_beam* b, double blh, double blv)
 {
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 0; i < b->n; ++i) {
     if (b->loss[i] != 0) continue;
     double gm1 = b->w[i] / b->mass;
@@ -960,6 +993,7 @@ static void map_steerer(orc_beam* b, double blh, double blv)
 /* SimulateCircularApertureKernel, simulation_kernel.cu:543-556 */
 static void map_circular_aperture(orc_beam* b, double aper, uint32_t idx)
 {
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 1; i < b->n; ++i) {
     if (b->loss[i] != 0) continue;
     double aper2 = aper * aper;
@@ -974,6 +1008,7 @@ static void map_rect_aperture(orc_beam* b, double xl, double xr, double yt, doub
 {
   double xmin = b->st.x_avg - xl, xmax = b->st.x_avg + xr;
   double ymin = b->st.y_avg - yb, ymax = b->st.y_avg + yt;
+  #pragma omp parallel for schedule(static)
   for (uint32_t i = 1; i < b->n; ++i) {
     if (b->loss[i] != 0) continue;
     double x = b->x[i], y = b->y[i];
@@ -1050,6 +1085,7 @@ void orc_t_green_table(uint32_t nr, uint32_t nz, double dr_, double dz_, double 
   double dr = dr_;
   double dzg = dz_ * gm;
   double c1 = 5721.67 * r_current_ov_mc2 / (r_freq * 1e6 * (double)r_num_part);
+  #pragma omp parallel for collapse(3) schedule(static)
   for (uint32_t rpi = 0; rpi <= nr; ++rpi)
     for (uint32_t k = 0; k < nr; ++k)
       for (uint32_t t = 0; t < nz; ++t) {
@@ -1084,6 +1120,7 @@ void orc_t_green_table(uint32_t nr, uint32_t nz, double dr_, double dz_, double 
 void orc_t_field(uint32_t nr, uint32_t nz, const int64_t* bins, const double* fld1, double* fld2)
 {
   uint32_t fld_sz = nr * nz;
+  #pragma omp parallel for collapse(2) schedule(static)
   for (uint32_t zp = 0; zp <= nz; ++zp)
     for (uint32_t rp = 0; rp <= nr; ++rp) {
       double ax[64], ay[64];
@@ -1118,6 +1155,8 @@ static void deposit(orc_spch* s, orc_beam* b)
   s->s.num_on_mesh = 0;
   if (b->blck_size / 4 == 0) return;   /* launch <<<grid, blck/4>>> is empty */
   double r_beta = s->beta, r_freq = s->freq;
+  uint32_t on_mesh = 0;
+  #pragma omp parallel for reduction(+:on_mesh) schedule(static)
   for (uint32_t np = 0; np < b->n; ++np) {
     if (!(b->loss[np] == 0 && b->lloss[np] == 0)) continue;
     double x = b->x[np], y = b->y[np], z = b->phi_r[np];
@@ -1166,16 +1205,23 @@ static void deposit(orc_spch* s, orc_beam* b)
     bb = 1.0 - a;
     d = 1.0 - cc;
     if (jm1 < 1 || jm1 > (int)nz) continue;
+    /* integer adds: the same totals in any order */
+    int64_t* bins = s->bins;
     k = (jm1 - 1) * (int)nx + i;
-    s->bins[k - 1] += (int64_t)(a * cc * 0x1p36);
+    #pragma omp atomic
+    bins[k - 1] += (int64_t)(a * cc * 0x1p36);
     k += i1 - i;
-    s->bins[k - 1] += (int64_t)(bb * cc * 0x1p36);
+    #pragma omp atomic
+    bins[k - 1] += (int64_t)(bb * cc * 0x1p36);
     k = (j1 - 1) * (int)nx + i;
-    s->bins[k - 1] += (int64_t)(a * d * 0x1p36);
+    #pragma omp atomic
+    bins[k - 1] += (int64_t)(a * d * 0x1p36);
     k += i1 - i;
-    s->bins[k - 1] += (int64_t)(bb * d * 0x1p36);
-    s->s.num_on_mesh += 1;
+    #pragma omp atomic
+    bins[k - 1] += (int64_t)(bb * d * 0x1p36);
+    on_mesh += 1;
   }
+  s->s.num_on_mesh = on_mesh;
 }
 
 /* ApplySpchKickKernel, space_charge_kernel.cu:572-706; launch geometry of
@@ -1189,6 +1235,7 @@ static void kick(orc_spch* s, orc_beam* b, double r_length, double r_ratio_r,
   uint32_t nx = s->s.nx, nz = s->s.nz;
   double r_mass = b->mass, r_current = b->current * s->frac, r_freq = s->freq, r_beta = s->beta;
   uint32_t r_num_part = b->n;
+  #pragma omp parallel for schedule(static)
   for (uint32_t indx = 0; indx < b->n; ++indx) {
     if (indx % stride == 0) continue;
     double x = b->x[indx], y = b->y[indx], xp = b->xp[indx], yp = b->yp[indx];

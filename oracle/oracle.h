@@ -83,6 +83,8 @@ typedef struct orc_sim orc_sim;
 #define ORC_DEPOSIT_SHIFT 36
 
 const char* orc_last_error(void);
+/* Threads the loops of this build use (1 for the serial liboracle.so). */
+int orc_threads(void);
 
 /* --- beam --- */
 orc_beam* orc_beam_create(uint32_t num, double mass, double charge, double current);

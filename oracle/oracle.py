@@ -14,6 +14,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
+_LIB_OMP = os.path.join(_HERE, "liboracle_omp.so")
+_use_omp = False
 
 EL_TYPES = {"Drift": 1, "Quad": 2, "RFGap-DTL": 3, "RFGap-CCL": 4, "ApertureC": 5,
             "ApertureR": 6, "SpchComp": 7, "Diagnostics": 8, "Rotation": 9, "Steerer": 10}
@@ -65,16 +67,28 @@ def build():
 _lib = None
 
 
+def use_openmp():
+    """Route every later call to liboracle_omp.so (OpenMP loops on all
+    OMP_NUM_THREADS cores, bit-identical results).  Must come before the
+    first oracle object is created in this process."""
+    global _use_omp
+    if _lib is not None and not _use_omp:
+        raise RuntimeError("the serial oracle library is already in use in this process")
+    _use_omp = True
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB):
+        path = _LIB_OMP if _use_omp else _LIB
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB)
+        L = C.CDLL(path)
         P = C.c_void_p
         dp = C.POINTER(C.c_double)
         up = C.POINTER(C.c_uint32)
         L.orc_last_error.restype = C.c_char_p
+        L.orc_threads.restype = C.c_int
         L.orc_beam_create.restype = P
         L.orc_beam_create.argtypes = [C.c_uint32, C.c_double, C.c_double, C.c_double]
         L.orc_beam_destroy.argtypes = [P]

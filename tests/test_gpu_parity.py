@@ -234,3 +234,19 @@ def test_waterbag_and_dc_beams_match_oracle_then_track(oracle_lib):
                     HPSim.SpaceCharge(nr=wl.nr, nz=wl.nz, interval=wl.interval)).simulate()
     O.Simulator(o, elems, O.SpaceCharge(wl.nr, wl.nz, wl.interval)).simulate()
     assert H.compare_arrays(g.arrays(), o.arrays(), "dc beam tracked") is None
+
+
+@pytest.mark.parametrize("nr,nz", [(32, 128), (16, 64), (7, 33), (64, 256)])
+def test_space_charge_tables_bit_exact(nr, nz):
+    """Green's table, charge histogram and field table of the last remesh,
+    for several mesh shapes (the LDS-staged and the direct field kernel)."""
+    wl = H.workload(0, num=8192, periods=2, sc=True)
+    wl.nr, wl.nz = nr, nz
+    ob, osc, _ = H.run_oracle(wl)
+    gb, gsc, _ = H.run_gpu(wl)
+    assert gsc.state()["remesh_count"] == osc.state()["remesh_count"] > 0
+    for name, g, o in zip(("greens", "field", "bins"), gsc.tables(), osc.tables()):
+        assert g.shape == o.shape, name
+        assert np.array_equal(g.view(np.uint64), o.view(np.uint64)), name
+    err = H.compare_arrays(gb.arrays(), ob.arrays())
+    assert err is None, err

@@ -464,3 +464,28 @@ def test_dc_beam_ramp_and_4ball_moments(oracle_lib):
         # uniform 4-ball: <r_i^2> = 1/6
         assert abs(er * 6 / e - 1) < 0.01
         assert abs(alr - al) < 0.02 and abs(btr / bt - 1) < 0.01
+
+
+def test_openmp_oracle_is_bit_identical_to_serial(oracle_lib, tmp_path):
+    """liboracle_omp.so (OpenMP loops, used for the all-core CPU baseline)
+    reproduces the serial checker bit for bit on a space-charge run."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = tmp_path / "omp.npz"
+    code = (
+        "import sys, numpy as np\n"
+        f"sys.path.insert(0, {root!r})\n"
+        "from oracle import oracle as O\n"
+        "O.use_openmp()\n"
+        "from tests.helpers import workload, run_oracle\n"
+        "b, _, _ = run_oracle(workload(0, num=6000, periods=3))\n"
+        f"np.savez({str(out)!r}, **b.arrays())\n")
+    env = dict(os.environ, OMP_NUM_THREADS="4")
+    subprocess.run([sys.executable, "-c", code], check=True, env=env, cwd=root)
+    from tests.helpers import run_oracle, workload
+    ser, _, _ = run_oracle(workload(0, num=6000, periods=3))
+    a, o = ser.arrays(), np.load(out)
+    for k in a:
+        assert np.array_equal(a[k].view(np.uint8), o[k].view(np.uint8)), k
