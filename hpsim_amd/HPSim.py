@@ -19,6 +19,7 @@ import numpy as np
 
 from . import _native as N
 from ._native import check, dptr, uptr
+from .db import DBConnection, beamline_elements  # noqa: F401  (the reference module type)
 
 _COORD = ("x", "xp", "y", "yp", "phi", "w")
 
@@ -291,21 +292,25 @@ def _to_element(e) -> N.Element:
 
 
 class BeamLine:
-    """BeamLine(DBConnection) in the reference (wrap_beamline.cpp:24-42).
-
-    Here a BeamLine is filled from element descriptions
-    (hpsim_amd.configs.Element or any object with the same attributes), from
-    a database connection object that yields them (``elements()``), or with
-    add_element()."""
+    """BeamLine(DBConnection), wrap_beamline.cpp:24-42: the lattice of the
+    connected databases (hpsim_amd.db, init.cpp:20-429).  A BeamLine can
+    also be filled from element descriptions (hpsim_amd.configs.Element or
+    any object with the same attributes) or with add_element()."""
 
     def __init__(self, dbconnection=None, elements=None):
+        if dbconnection is not None:
+            if elements is not None:
+                raise TypeError("BeamLine takes a DBConnection or elements, not both")
+            elements = beamline_elements(dbconnection)
+        elements = list(elements or ())
+        bad = sorted({e.kind for e in elements if e.kind not in N.KINDS})
+        if bad:
+            raise NotImplementedError(f"element kinds without an engine map: {', '.join(bad)}")
         h = C.c_void_p()
         check(_lib().hps_beamline_create(C.byref(h)))
         self._h = h
         self._elements = []
-        if dbconnection is not None:
-            elements = dbconnection.elements()
-        for e in elements or ():
+        for e in elements:
             self.add_element(e)
 
     def __del__(self):

@@ -138,7 +138,8 @@ GAP_FIELDS = ("length", "frequency", "cell_length_over_beta_lambda", "amplitude"
 class Element:
     """One beamline element (reference beamline_element.h).  ``kind`` is the
     reference type string: Drift, Quad, RFGap-DTL, RFGap-CCL, ApertureC,
-    ApertureR, SpchComp, Diagnostics, Rotation, Steerer."""
+    ApertureR, SpchComp, Diagnostics, Rotation, Steerer (the engine's maps),
+    or Buncher / Dipole (read from lattice databases, not tracked yet)."""
     name: str
     kind: str
     length: float = 0.0
@@ -154,6 +155,7 @@ class Element:
     aperture_in: bool = False
     fraction: float = 1.0
     gap: dict = field(default_factory=dict)
+    params: dict = field(default_factory=dict)   # Buncher / Dipole model fields
 
 
 def _beta(w, mass):

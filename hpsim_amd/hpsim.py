@@ -7,8 +7,9 @@ numpy analysis helpers (masks, Twiss parameters, rms emittances).  Every
 call lands in hpsim_amd.HPSim and from there in the gfx950 engine.
 
 Left out: the matplotlib plotting classes (BeamPlot, DistPlot) and the
-database / EPICS helpers (DBConnection, DBState, set_db_epics, ...): the
-lattice is given as element records (BeamLine(elements=[...])).
+EPICS helpers (DBState, set_db_epics, ...).  The lattice comes from the
+reference's SQLite databases (DBConnection + BeamLine()) or from element
+records (BeamLine(elements=[...])).
 """
 from __future__ import annotations
 
@@ -323,17 +324,47 @@ def _results_text(src, mask):
     return "\n".join(lines) + "\n"
 
 
+class DBConnection:
+    """Lattice databases, kept class-level for BeamLine() (hpsim.py:875-932):
+    the first database opens as ``main``, the others attach as db1, db2, ...
+    in beamline order."""
+
+    dbconnection = None
+
+    def __init__(self, db_dir, databases, libsql_dir=None, libsql_file=None):
+        import os
+        self.dbconnection = None
+        for k, name in enumerate(databases):
+            path = os.path.join(os.path.abspath(db_dir), name)
+            if k == 0:
+                self.dbconnection = HPSim.DBConnection(path)
+            else:
+                self.dbconnection.attach_db(path, f"db{k}")
+        DBConnection.dbconnection = self.dbconnection
+        if libsql_dir is not None and libsql_file is not None:
+            self.dbconnection.load_lib(os.path.join(os.path.abspath(libsql_dir), libsql_file))
+        self.clear_model_index()
+
+    def print_dbs(self): self.dbconnection.print_dbs()
+    def print_libs(self): self.dbconnection.print_libs()
+    def clear_model_index(self): self.dbconnection.clear_model_index()
+    def get_epics_channels(self): return self.dbconnection.get_epics_channels()
+
+
 class BeamLine:
-    """The beamline, kept class-level for Simulator(beam) (hpsim.py:935-980).
-    Built from element records (hpsim_amd.configs.Element or dicts)."""
+    """The beamline, kept class-level for Simulator(beam) (hpsim.py:935-980):
+    from the class-level DBConnection, or from element records
+    (hpsim_amd.configs.Element)."""
 
     beamline = None
 
     def __init__(self, elements=None):
         if elements is None:
-            raise ValueError("BeamLine needs element records: the lattice database "
-                             "(DBConnection) is not read by this engine")
-        self.beamline = HPSim.BeamLine(elements=elements)
+            if DBConnection.dbconnection is None:
+                raise ValueError("create a DBConnection (or pass element records) first")
+            self.beamline = HPSim.BeamLine(DBConnection.dbconnection)
+        else:
+            self.beamline = HPSim.BeamLine(elements=elements)
         BeamLine.beamline = self.beamline
 
     def print_out(self): self.beamline.print_out()
