@@ -288,6 +288,10 @@ def _to_element(e) -> N.Element:
     gap = getattr(e, "gap", None) or {}
     for f in N.GAP_FIELDS:
         setattr(o.gap, f, float(gap.get(f, 0.0)))
+    params = getattr(e, "params", None) or {}
+    for f in N.ELEMENT_EXTRA:
+        setattr(o, f, float(params.get(f, 0.0)))
+    o.on = 1 if params.get("on", False) else 0
     return o
 
 

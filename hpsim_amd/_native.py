@@ -17,7 +17,11 @@ GAP_FIELDS = ("length", "frequency", "cell_length_over_beta_lambda", "amplitude"
               "fit_t3", "fit_t4", "fit_t5", "fit_s1", "fit_s2", "fit_s3", "fit_s4", "fit_s5")
 
 KINDS = {"Drift": 1, "Quad": 2, "RFGap-DTL": 3, "RFGap-CCL": 4, "ApertureC": 5,
-         "ApertureR": 6, "SpchComp": 7, "Diagnostics": 8, "Rotation": 9, "Steerer": 10}
+         "ApertureR": 6, "SpchComp": 7, "Diagnostics": 8, "Rotation": 9, "Steerer": 10,
+         "Buncher": 11, "Dipole": 12}
+# Buncher and Dipole model fields of hps_element (after the gap block)
+ELEMENT_EXTRA = ("voltage", "frequency", "phase", "rho", "edge_angle_in", "edge_angle_out",
+                 "half_gap", "k1", "k2", "field_index", "kinetic_energy")
 KIND_NAMES = {v: k for k, v in KINDS.items()}
 
 
@@ -30,7 +34,9 @@ class Element(C.Structure):
                 ("aperture", C.c_double), ("gradient", C.c_double), ("angle", C.c_double),
                 ("bl_h", C.c_double), ("bl_v", C.c_double), ("aper_xl", C.c_double),
                 ("aper_xr", C.c_double), ("aper_yt", C.c_double), ("aper_yb", C.c_double),
-                ("fraction", C.c_double), ("gap", GapParam)]
+                ("fraction", C.c_double), ("gap", GapParam)] + [
+                    (f, C.c_double) for f in ELEMENT_EXTRA] + [
+                    ("on", C.c_int32), ("pad_", C.c_int32)]
 
 
 STAT_FIELDS = ("x_avg", "x_avg_good", "xp_avg", "y_avg", "y_avg_good", "yp_avg", "phi_avg",

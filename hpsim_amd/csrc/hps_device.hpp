@@ -37,7 +37,10 @@ enum OpKind : int32_t {
   OP_ROT = 5,       // SimulateRotationKernel
   OP_STEER = 6,     // SimulateSteererKernel
   OP_APER_C = 7,    // SimulateCircularApertureKernel
-  OP_APER_R = 8     // SimulateRectangularApertureKernel
+  OP_APER_R = 8,    // SimulateRectangularApertureKernel
+  OP_BUNCH = 9,     // SimulateBuncherKernel
+  OP_DIP1 = 10,     // SimulateFirstHalfDipoleKernel
+  OP_DIP2 = 11      // SimulateSecondHalfDipoleKernel
 };
 
 // Map classes by register cost; a k_segment instantiation compiles only the
@@ -46,12 +49,14 @@ enum OpKind : int32_t {
 enum OpClass : int32_t {
   OPC_LIGHT = 0,    // drift, rotation, steerer, apertures
   OPC_QUAD = 1,     // + half quad
-  OPC_GAP = 2       // + RF gap halves
+  OPC_GAP = 2       // + RF gap halves, buncher, dipole halves
 };
 
 __host__ __device__ inline int32_t op_class_of(int32_t kind)
 {
-  if (kind == OP_GAP1 || kind == OP_GAP2) return OPC_GAP;
+  if (kind == OP_GAP1 || kind == OP_GAP2 || kind == OP_BUNCH || kind == OP_DIP1 ||
+      kind == OP_DIP2)
+    return OPC_GAP;
   if (kind == OP_QUAD) return OPC_QUAD;
   return OPC_LIGHT;
 }
@@ -64,12 +69,15 @@ __host__ __device__ inline int32_t op_class_of(int32_t kind)
 //  ROT    angle
 //  STEER  bl_h, bl_v
 //  APER_C aperture        APER_R  xl, xr, yt, yb
+//  BUNCH  w_ref, frequency, voltage, phase, aperture, frequency ratio, wavelength
+//  DIP1/2 rho, angle, edge angle (in / out), half gap, k1, k2, field index,
+//         kinetic energy, wavelength
 struct Op {
   int32_t kind;
   uint32_t elem;          // loss code written on loss (element model index)
   int32_t ccl;            // gap form: 0 DTL, 1 CCL
   uint32_t ccl_cell_num;
-  double a[6];
+  double a[10];
   hps_gap_param g;
 };
 
@@ -382,6 +390,141 @@ __device__ __forceinline__ void steerer(Particle& p, double blh, double blv, dou
 }
 
 // Apply one map step to an alive particle.
+// SimulateBuncherKernel body, simulation_kernel.cu:807-874 (rsqrt as 1/sqrt).
+__device__ __forceinline__ void buncher(Particle& p, const double* a, uint32_t elem, double mass,
+                                        double charge)
+{
+  const double w_ref = a[0], freq = a[1], voltage = a[2], phase = a[3], aper = a[4];
+  const double freq_ratio = a[5], wavelen = a[6];
+  const double x = p.x, y = p.y, xp = p.xp, yp = p.yp;
+  const double phi = p.phi * freq_ratio;
+  const double w = p.w;
+  const double r2 = x * x + y * y;
+  const double aper2 = aper * aper;
+  if (aper2 != 0.0 && r2 > aper2) { p.loss = elem; return; }
+  const double h = freq * wavelen / CLIGHT;
+  const double gm1 = w_ref / mass;
+  const double gm = gm1 + 1.0;
+  const double btgm = ::sqrt(gm1 * (gm1 + 2.0));
+  const double beta = btgm / gm;
+  const double k = h * TWOPI / (btgm * wavelen);
+  const double k2 = k * k;
+  const double q = charge > 0.0 ? charge : -charge;
+  const double con1 = TWOPI * h * voltage * q / (beta * wavelen * mass);
+  const double arg = k2 * r2 * 0.25;
+  const double ph = h * phi + phase;
+  double sph, cph;
+  hps::sincos(ph, &sph, &cph);
+  const double dw = q * voltage * cph * (1.0 + arg * (1.0 + arg * (0.25 + arg / 36.0)));
+  const double wb_par = w + 0.5 * dw;
+  const double wf = w + dw;
+  if (wb_par <= 0.0 || wf <= 0.0) { p.loss = elem; return; }
+  double gmtmp = wb_par / mass;
+  const double bav = ::sqrt(gmtmp * (gmtmp + 2.0)) / (gmtmp + 1.0);
+  gmtmp = w / mass;
+  const double bg = ::sqrt(gmtmp * (gmtmp + 2.0));
+  gmtmp = wf / mass;
+  const double rbgf = 1.0 / ::sqrt(gmtmp * (gmtmp + 2.0));
+  const double del = -con1 * sph * (1.0 - bav * beta) * (0.5 + arg * (0.25 + arg / 24.0)) / bav;
+  p.xp = (xp * bg + del * x) * rbgf;
+  p.yp = (yp * bg + del * y) * rbgf;
+  p.w = wf;
+}
+
+// Half a sector dipole (arc rho * angle / 2, field index n) and the pole-face
+// edge kick, SimulateFirst/SecondHalfDipoleKernel, simulation_kernel.cu:603-805.
+struct DipoleHalf {
+  double r11, r12, r16, r21, r22, r26, r33, r34, r43, r44, r51, r52, r56, beta, dl;
+};
+
+__device__ __forceinline__ DipoleHalf dipole_half(const double* a, double mass)
+{
+  DipoleHalf o;
+  const double rho = a[0], angle = a[1], findx = a[7], design_w = a[8];
+  const double h = 1.0 / rho;
+  const double h2 = h * h;
+  const double dl = 0.5 * rho * angle;
+  const double yk2 = findx * h2;
+  const double yk = ::sqrt(__builtin_fabs(yk2));
+  const double inv_yk = 1.0 / yk;
+  const double ykl = yk * dl;
+  const double xk2 = h2 - yk2;
+  const double inv_xk2 = 1.0 / xk2;
+  const double xk = ::sqrt(__builtin_fabs(xk2));
+  const double inv_xk = 1.0 / xk;
+  const double xkl = xk * dl;
+  double cxkl = 1.0, sxokx = dl, s, c;
+  if (xk2 > 0.0) { hps::sincos(xkl, &s, &c); cxkl = c; sxokx = s * inv_xk; }
+  else if (xk2 < 0.0) { cxkl = hps::cosh(xkl); sxokx = hps::sinh(xkl) * inv_xk; }
+  double cykl = 1.0, syoky = dl;
+  if (yk2 > 0.0) { hps::sincos(ykl, &s, &c); cykl = c; syoky = s * inv_yk; }
+  else if (yk2 < 0.0) { cykl = hps::cosh(ykl); syoky = hps::sinh(ykl) * inv_yk; }
+  o.r11 = cxkl; o.r12 = sxokx; o.r21 = -xk2 * sxokx; o.r22 = cxkl;
+  o.r33 = cykl; o.r34 = syoky; o.r43 = -yk2 * syoky; o.r44 = cykl;
+  o.r26 = h * sxokx;
+  o.r16 = h * (1.0 - cxkl) * inv_xk2;
+  o.r51 = -o.r26;
+  o.r52 = -o.r16;
+  const double gamma = design_w / mass + 1.0;
+  const double beta2 = 1.0 - 1.0 / (gamma * gamma);
+  o.beta = ::sqrt(beta2);
+  o.r56 = -h2 * (dl * beta2 - sxokx) * inv_xk2 + dl * (1.0 - h2 * inv_xk2) / (gamma * gamma);
+  o.dl = dl;
+  return o;
+}
+
+__device__ __forceinline__ void dipole_edge(const double* a, double b, double& r21, double& r43)
+{
+  const double h = 1.0 / a[0];
+  const double c1 = 2.0 * a[4] * a[5] * h;
+  double sinb, cosb;
+  hps::sincos(b, &sinb, &cosb);
+  const double inv_cosb = 1.0 / cosb;
+  const double psi = c1 * (1.0 + sinb * sinb) * inv_cosb * (1.0 - c1 * a[6] * sinb * inv_cosb);
+  r21 = hps::tan(b) * h;
+  r43 = hps::tan(b - psi) * h;
+}
+
+__device__ __forceinline__ void dipole_first_half(Particle& p, const double* a, double mass)
+{
+  double er21, er43;
+  dipole_edge(a, a[2], er21, er43);
+  const DipoleHalf d = dipole_half(a, mass);
+  const double design_w = a[8], wavelen = a[9];
+  const double gamma = design_w / mass + 1.0;
+  const double x = p.x, y = p.y;
+  double xp = p.xp, yp = p.yp;
+  xp += er21 * x;
+  yp -= er43 * y;
+  const double dpop = gamma / (design_w * (gamma + 1.0)) * (p.w - design_w);
+  p.x = d.r11 * x + d.r12 * xp + d.r16 * dpop;
+  p.xp = d.r21 * x + d.r22 * xp + d.r26 * dpop;
+  p.y = d.r33 * y + d.r34 * yp;
+  p.yp = d.r43 * y + d.r44 * yp;
+  p.phi += TWOPI * (-(d.r51 * x + d.r52 * xp + d.r56 * dpop) + d.dl) / (d.beta * wavelen);
+}
+
+__device__ __forceinline__ void dipole_second_half(Particle& p, const double* a, double mass)
+{
+  double er21, er43;
+  dipole_edge(a, a[3], er21, er43);
+  const DipoleHalf d = dipole_half(a, mass);
+  const double design_w = a[8], wavelen = a[9];
+  const double gamma = design_w / mass + 1.0;
+  const double x = p.x, y = p.y;
+  double xp = p.xp, yp = p.yp;
+  const double dpop = gamma / (design_w * (gamma + 1.0)) * (p.w - design_w);
+  p.phi += TWOPI * (-(d.r51 * x + d.r52 * xp + d.r56 * dpop) + d.dl) / (d.beta * wavelen);
+  const double x_new = d.r11 * x + d.r12 * xp + d.r16 * dpop;
+  xp = d.r21 * x + d.r22 * xp + d.r26 * dpop;
+  const double y_new = d.r33 * y + d.r34 * yp;
+  yp = d.r43 * y + d.r44 * yp;
+  p.x = x_new;
+  p.y = y_new;
+  p.xp = xp + er21 * x_new;
+  p.yp = yp - er43 * y_new;
+}
+
 template <int kClass>
 __device__ __forceinline__ void apply_op(const Op& op, Particle& p, uint32_t index, double mass,
                                          double charge, double x_avg, double y_avg)
@@ -400,6 +543,15 @@ __device__ __forceinline__ void apply_op(const Op& op, Particle& p, uint32_t ind
       if constexpr (kClass >= OPC_GAP)
         gap_second_half(p, op.g, op.a[0], op.a[1], op.a[2], op.a[3], op.a[4], op.a[5],
                         op.ccl_cell_num, op.ccl != 0, mass, charge);
+      break;
+    case OP_BUNCH:
+      if constexpr (kClass >= OPC_GAP) buncher(p, op.a, op.elem, mass, charge);
+      break;
+    case OP_DIP1:
+      if constexpr (kClass >= OPC_GAP) dipole_first_half(p, op.a, mass);
+      break;
+    case OP_DIP2:
+      if constexpr (kClass >= OPC_GAP) dipole_second_half(p, op.a, mass);
       break;
     case OP_ROT: rotation(p, op.a[0]); break;
     case OP_STEER: steerer(p, op.a[0], op.a[1], mass, charge); break;

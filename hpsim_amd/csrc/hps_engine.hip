@@ -743,7 +743,7 @@ int hps_beamline_destroy(hps_beamline* bl)
 int hps_beamline_add(hps_beamline* bl, const char* name, const hps_element* e)
 {
   if (!bl || !e) return fail("null argument");
-  if (e->kind < HPS_DRIFT || e->kind > HPS_STEERER) return fail("unknown element kind");
+  if (e->kind < HPS_DRIFT || e->kind > HPS_DIPOLE) return fail("unknown element kind");
   bl->names.push_back(name ? name : "");
   bl->el.push_back(*e);
   return 0;
@@ -1005,6 +1005,7 @@ struct Compiler {
   double frac = 1.0;
   double freq = 0;           // beam frequency as the program runs
   bool quad_gap_run = true;  // launches with blck/2 threads are empty for N = 1
+  bool dipole_run = true;    // launches with blck/4 threads are empty for N <= 2
   bool sc_stats = true;      // UpdateStatForSpaceCharge runs only for N > 2
 
   void close(int32_t stats, bool kick_point, double kick_len_next)
@@ -1173,6 +1174,30 @@ int compile_element(Compiler& c, uint32_t id)
       return 0;
     }
     case HPS_DIAGNOSTICS: return 0;
+    case HPS_BUNCHER: {   // SimulateBuncher, simulation_engine.cu:346-358
+      if (!e.on) return 0;
+      m->wavelen = CLIGHT / e.frequency;   // UpdateWaveLengthKernel(bfreq)
+      Op o = make_op(OP_BUNCH, id);
+      o.a[0] = m->design_w; o.a[1] = e.frequency; o.a[2] = e.voltage; o.a[3] = e.phase;
+      o.a[4] = e.aperture; o.a[5] = e.frequency / c.freq; o.a[6] = m->wavelen;
+      c.ops.push_back(o);
+      return 0;
+    }
+    case HPS_DIPOLE: {    // SimulateDipole, simulation_engine.cu:360-374
+      const bool run = c.dipole_run;   // launches <<<grid, blck/4>>>
+      Op o = make_op(OP_DIP1, id);
+      o.a[0] = e.rho; o.a[1] = e.angle; o.a[2] = e.edge_angle_in; o.a[3] = e.edge_angle_out;
+      o.a[4] = e.half_gap; o.a[5] = e.k1; o.a[6] = e.k2; o.a[7] = e.field_index;
+      o.a[8] = e.kinetic_energy; o.a[9] = m->wavelen;
+      if (run) c.ops.push_back(o);
+      if (sc) {
+        if (!m->sc) return fail("space charge is on but no SpaceCharge object was given");
+        c.kick_point(e.rho * e.angle);
+      }
+      o.kind = OP_DIP2;
+      if (run) c.ops.push_back(o);
+      return 0;
+    }
     default: return fail("unknown element kind");
   }
 }
@@ -1203,6 +1228,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   c.frac = m->sc ? m->sc->cfg.fraction : 1.0;
   c.freq = b->freq;
   c.quad_gap_run = b->blck_size / 2 != 0;
+  c.dipole_run = b->blck_size / 4 != 0;
   c.sc_stats = b->n > 2;
   for (uint32_t i = 0; (int32_t)i <= end_index; ++i)
     if ((int32_t)i >= start_index || m->bl->el[i].kind == HPS_SPCH_COMP)

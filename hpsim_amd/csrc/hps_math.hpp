@@ -181,6 +181,14 @@ HPS_HD double cosh(double x)
   return 0.5 * e + 0.5 / e;
 }
 
+// tan(x) = sin(x) / cos(x) from sincos.
+HPS_HD double tan(double x)
+{
+  double s, c;
+  sincos(x, &s, &c);
+  return s / c;
+}
+
 // Remainder of x >= 0 by y > 0 (finite), exact.
 HPS_HD double fmod_pos(double x, double y)
 {

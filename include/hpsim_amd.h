@@ -38,7 +38,9 @@ enum hps_element_kind {
   HPS_SPCH_COMP = 7,    /* "SpchComp"    */
   HPS_DIAGNOSTICS = 8,  /* "Diagnostics" */
   HPS_ROTATION = 9,     /* "Rotation"    */
-  HPS_STEERER = 10      /* "Steerer"     */
+  HPS_STEERER = 10,     /* "Steerer"     */
+  HPS_BUNCHER = 11,     /* "Buncher"     */
+  HPS_DIPOLE = 12       /* "Dipole"      */
 };
 
 /* RF gap parameters, field for field RFGapParameter
@@ -64,6 +66,13 @@ typedef struct {
   double aper_xl, aper_xr, aper_yt, aper_yb;  /* ApertureR, m */
   double fraction;         /* SpchComp */
   hps_gap_param gap;       /* RFGap-DTL / RFGap-CCL */
+  /* Buncher (beamline_element.h:83-107): voltage MV, frequency MHz, phase rad */
+  double voltage, frequency, phase;
+  /* Dipole (dipole_parameter in beamline_element.h:115-150); the bend angle
+   * is `angle` above */
+  double rho, edge_angle_in, edge_angle_out, half_gap, k1, k2, field_index,
+         kinetic_energy;
+  int32_t on, pad_;        /* Buncher::IsOn() */
 } hps_element;
 
 /* Scalar beam statistics: the device scalars of the reference Beam

@@ -721,6 +721,7 @@ void orc_t_sincos(double x, double* s, double* c) { orc_sincos(x, s, c); }
 double orc_t_sinh(double x) { return orc_sinh(x); }
 double orc_t_cosh(double x) { return orc_cosh(x); }
 double orc_t_fmod_pos(double x, double y) { return orc_fmod_pos(x, y); }
+double orc_t_tan(double x) { return orc_tan(x); }
 double orc_t_bessi0(double x) { return bessi0c(x); }
 double orc_t_bessi1(double x) { return bessi1c(x); }
 double orc_t_bessi1p(double x) { return bessi1p(x); }
@@ -1013,6 +1014,158 @@ static void map_rect_aperture(orc_beam* b, double xl, double xr, double yt, doub
     if (b->loss[i] != 0) continue;
     double x = b->x[i], y = b->y[i];
     if (x < xmin || x > xmax || y < ymin || y > ymax) b->loss[i] = idx;
+  }
+}
+
+/* SimulateBuncherKernel, simulation_kernel.cu:807-874 (launch
+ * <<<grid, blck>>>, every particle).  wavelen = CLIGHT / buncher frequency
+ * (UpdateWaveLengthKernel, simulation_engine.cu:352); the phase argument
+ * uses phi scaled by freq_ratio = buncher / beam frequency. */
+static void map_buncher(orc_beam* b, double w_ref, double freq, double voltage, double phase,
+    double aper, double freq_ratio, double wavelen, uint32_t idx)
+{
+  #pragma omp parallel for schedule(static)
+  for (uint32_t i = 0; i < b->n; ++i) {
+    if (b->loss[i] != 0) continue;
+    double x = b->x[i], y = b->y[i], xp = b->xp[i], yp = b->yp[i];
+    double phi = b->phi[i] * freq_ratio;
+    double w = b->w[i];
+    double r2 = x * x + y * y;
+    double aper2 = aper * aper;
+    if (aper2 != 0.0 && r2 > aper2) { b->loss[i] = idx; continue; }
+    double h = freq * wavelen / CLIGHT;
+    double gm1 = w_ref / b->mass;
+    double gm = gm1 + 1.0;
+    double btgm = sqrt(gm1 * (gm1 + 2.0));
+    double beta = btgm / gm;
+    double k = h * TWOPI / (btgm * wavelen);
+    double k2 = k * k;
+    double q = b->charge;
+    q = q > 0.0 ? q : -q;
+    double con1 = TWOPI * h * voltage * q / (beta * wavelen * b->mass);
+    double arg = k2 * r2 * 0.25;
+    double ph = h * phi + phase;
+    double sph, cph;
+    orc_sincos(ph, &sph, &cph);
+    double dw = q * voltage * cph * (1.0 + arg * (1.0 + arg * (0.25 + arg / 36.0)));
+    double wb_par = w + 0.5 * dw;
+    double wf = w + dw;
+    if (wb_par <= 0.0 || wf <= 0.0) { b->loss[i] = idx; continue; }
+    double gmtmp = wb_par / b->mass;
+    double bav = sqrt(gmtmp * (gmtmp + 2.0)) / (gmtmp + 1.0);
+    gmtmp = w / b->mass;
+    double bg = sqrt(gmtmp * (gmtmp + 2.0));
+    gmtmp = wf / b->mass;
+    double rbgf = 1.0 / sqrt(gmtmp * (gmtmp + 2.0));
+    double del = -con1 * sph * (1.0 - bav * beta) * (0.5 + arg * (0.25 + arg / 24.0)) / bav;
+    b->xp[i] = (xp * bg + del * x) * rbgf;
+    b->yp[i] = (yp * bg + del * y) * rbgf;
+    b->w[i] = wf;
+  }
+}
+
+/* Transfer coefficients of half a sector dipole (arc dl = rho * angle / 2)
+ * with field index n, simulation_kernel.cu:625-683 / 725-780. */
+typedef struct {
+  double r11, r12, r16, r21, r22, r26, r33, r34, r43, r44, r51, r52, r56, beta, dl;
+} orc_dipole_half;
+
+static orc_dipole_half dipole_half(const orc_element* e, double mass)
+{
+  orc_dipole_half o;
+  double h = 1.0 / e->rho;
+  double h2 = h * h;
+  double dl = 0.5 * e->rho * e->angle;
+  double yk2 = e->field_index * h2;
+  double yk = sqrt(fabs(yk2));
+  double inv_yk = 1.0 / yk;
+  double ykl = yk * dl;
+  double xk2 = h2 - yk2;
+  double inv_xk2 = 1.0 / xk2;
+  double xk = sqrt(fabs(xk2));
+  double inv_xk = 1.0 / xk;
+  double xkl = xk * dl;
+  double cxkl = 1.0, sxokx = dl, s, c;
+  if (xk2 > 0.0) { orc_sincos(xkl, &s, &c); cxkl = c; sxokx = s * inv_xk; }
+  else if (xk2 < 0.0) { cxkl = orc_cosh(xkl); sxokx = orc_sinh(xkl) * inv_xk; }
+  double cykl = 1.0, syoky = dl;
+  if (yk2 > 0.0) { orc_sincos(ykl, &s, &c); cykl = c; syoky = s * inv_yk; }
+  else if (yk2 < 0.0) { cykl = orc_cosh(ykl); syoky = orc_sinh(ykl) * inv_yk; }
+  o.r11 = cxkl; o.r12 = sxokx; o.r21 = -xk2 * sxokx; o.r22 = cxkl;
+  o.r33 = cykl; o.r34 = syoky; o.r43 = -yk2 * syoky; o.r44 = cykl;
+  o.r26 = h * sxokx;
+  o.r16 = h * (1.0 - cxkl) * inv_xk2;
+  o.r51 = -o.r26;
+  o.r52 = -o.r16;
+  double gamma = e->kinetic_energy / mass + 1.0;
+  double beta2 = 1.0 - 1.0 / (gamma * gamma);
+  o.beta = sqrt(beta2);
+  o.r56 = -h2 * (dl * beta2 - sxokx) * inv_xk2 + dl * (1.0 - h2 * inv_xk2) / (gamma * gamma);
+  o.dl = dl;
+  return o;
+}
+
+/* Edge kick of a dipole face at angle b with fringe-field correction psi,
+ * simulation_kernel.cu:628-641 / 788-796: returns r21 and r43. */
+static void dipole_edge(const orc_element* e, double b, double* r21, double* r43)
+{
+  double h = 1.0 / e->rho;
+  double c1 = 2.0 * e->half_gap * e->k1 * h;
+  double sinb, cosb;
+  orc_sincos(b, &sinb, &cosb);
+  double inv_cosb = 1.0 / cosb;
+  double psi = c1 * (1.0 + sinb * sinb) * inv_cosb * (1.0 - c1 * e->k2 * sinb * inv_cosb);
+  *r21 = orc_tan(b) * h;
+  *r43 = orc_tan(b - psi) * h;
+}
+
+/* SimulateFirstHalfDipoleKernel, simulation_kernel.cu:603-705: entrance
+ * edge, then the first half sector. */
+static void map_dipole_first_half(orc_beam* b, const orc_element* e, double wavelen)
+{
+  double er21, er43;
+  dipole_edge(e, e->edge_angle_in, &er21, &er43);
+  orc_dipole_half d = dipole_half(e, b->mass);
+  double design_w = e->kinetic_energy;
+  double gamma = design_w / b->mass + 1.0;
+  #pragma omp parallel for schedule(static)
+  for (uint32_t i = 0; i < b->n; ++i) {
+    if (b->loss[i] != 0) continue;
+    double x = b->x[i], xp = b->xp[i], y = b->y[i], yp = b->yp[i];
+    xp += er21 * x;
+    yp -= er43 * y;
+    double dpop = gamma / (design_w * (gamma + 1.0)) * (b->w[i] - design_w);
+    b->x[i] = d.r11 * x + d.r12 * xp + d.r16 * dpop;
+    b->xp[i] = d.r21 * x + d.r22 * xp + d.r26 * dpop;
+    b->y[i] = d.r33 * y + d.r34 * yp;
+    b->yp[i] = d.r43 * y + d.r44 * yp;
+    b->phi[i] += TWOPI * (-(d.r51 * x + d.r52 * xp + d.r56 * dpop) + d.dl) / (d.beta * wavelen);
+  }
+}
+
+/* SimulateSecondHalfDipoleKernel, simulation_kernel.cu:706-805: second half
+ * sector, then the exit edge. */
+static void map_dipole_second_half(orc_beam* b, const orc_element* e, double wavelen)
+{
+  double er21, er43;
+  dipole_edge(e, e->edge_angle_out, &er21, &er43);
+  orc_dipole_half d = dipole_half(e, b->mass);
+  double design_w = e->kinetic_energy;
+  double gamma = design_w / b->mass + 1.0;
+  #pragma omp parallel for schedule(static)
+  for (uint32_t i = 0; i < b->n; ++i) {
+    if (b->loss[i] != 0) continue;
+    double x = b->x[i], xp = b->xp[i], y = b->y[i], yp = b->yp[i];
+    double dpop = gamma / (design_w * (gamma + 1.0)) * (b->w[i] - design_w);
+    b->phi[i] += TWOPI * (-(d.r51 * x + d.r52 * xp + d.r56 * dpop) + d.dl) / (d.beta * wavelen);
+    double x_new = d.r11 * x + d.r12 * xp + d.r16 * dpop;
+    xp = d.r21 * x + d.r22 * xp + d.r26 * dpop;
+    double y_new = d.r33 * y + d.r34 * yp;
+    yp = d.r43 * y + d.r44 * yp;
+    b->x[i] = x_new;
+    b->y[i] = y_new;
+    b->xp[i] = xp + er21 * x_new;
+    b->yp[i] = yp - er43 * y_new;
   }
 }
 
@@ -1550,6 +1703,19 @@ static int visit(orc_sim* m, const orc_element* e)
       if (e->bl_h > 1e-10 || e->bl_v > 1e-10) map_steerer(m->b, e->bl_h, e->bl_v);
       return 0;
     case ORC_EL_DIAGNOSTICS: return 0;
+    case ORC_EL_BUNCHER:      /* SimulateBuncher, simulation_engine.cu:346-358 */
+      if (!e->on) return 0;
+      m->wavelen = CLIGHT / e->frequency;
+      map_buncher(m->b, m->design_w, e->frequency, e->voltage, e->phase, e->aperture,
+                  e->frequency / m->b->freq, m->wavelen, m->cur_id);
+      return 0;
+    case ORC_EL_DIPOLE: {     /* SimulateDipole, simulation_engine.cu:360-374 */
+      int run = m->b->blck_size / 4 != 0;   /* launches <<<grid, blck/4>>> */
+      if (run) map_dipole_first_half(m->b, e, m->wavelen);
+      if (m->sc_on && spch_start(m, e->rho * e->angle)) return -1;
+      if (run) map_dipole_second_half(m->b, e, m->wavelen);
+      return 0;
+    }
     default: return set_err("unknown element type");
   }
 }

@@ -27,7 +27,9 @@ enum {
   ORC_EL_SPCH_COMP = 7,    /* "SpchComp"    */
   ORC_EL_DIAGNOSTICS = 8,  /* "Diagnostics" */
   ORC_EL_ROTATION = 9,     /* "Rotation"    */
-  ORC_EL_STEERER = 10      /* "Steerer"     */
+  ORC_EL_STEERER = 10,     /* "Steerer"     */
+  ORC_EL_BUNCHER = 11,     /* "Buncher"     */
+  ORC_EL_DIPOLE = 12       /* "Dipole"      */
 };
 
 /* RFGapParameter, reference rfgap_parameter.h:4-32 (same field order). */
@@ -50,6 +52,10 @@ typedef struct {
   double aper_xl, aper_xr, aper_yt, aper_yb;  /* ApertureR */
   double fraction;         /* SpchComp */
   orc_gap_param gap;
+  double voltage, frequency, phase;   /* Buncher: MV, MHz, rad */
+  double rho, edge_angle_in, edge_angle_out, half_gap, k1, k2, field_index,
+         kinetic_energy;              /* Dipole (bend angle in `angle`) */
+  int32_t on, pad_;                   /* Buncher IsOn() */
 } orc_element;
 
 /* Scalar beam statistics (device scalars of reference beam.h:175-263). */
@@ -152,6 +158,7 @@ void orc_t_sincos(double x, double* s, double* c);
 double orc_t_sinh(double x);
 double orc_t_cosh(double x);
 double orc_t_fmod_pos(double x, double y);
+double orc_t_tan(double x);
 double orc_t_bessi0(double x);
 double orc_t_bessi1(double x);
 double orc_t_bessi1p(double x);

@@ -18,7 +18,8 @@ _LIB_OMP = os.path.join(_HERE, "liboracle_omp.so")
 _use_omp = False
 
 EL_TYPES = {"Drift": 1, "Quad": 2, "RFGap-DTL": 3, "RFGap-CCL": 4, "ApertureC": 5,
-            "ApertureR": 6, "SpchComp": 7, "Diagnostics": 8, "Rotation": 9, "Steerer": 10}
+            "ApertureR": 6, "SpchComp": 7, "Diagnostics": 8, "Rotation": 9, "Steerer": 10,
+            "Buncher": 11, "Dipole": 12}
 
 GAP_FIELDS = ("length", "frequency", "cell_length_over_beta_lambda", "amplitude",
               "phase_ref", "phase_shift", "energy_out", "beta_center", "dg", "t", "tp",
@@ -30,12 +31,20 @@ class GapParam(C.Structure):
     _fields_ = [(f, C.c_double) for f in GAP_FIELDS]
 
 
+# Buncher (MV, MHz, rad) and Dipole model fields after the gap block; the
+# Dipole bend angle is ``angle``
+ELEMENT_EXTRA = ("voltage", "frequency", "phase", "rho", "edge_angle_in", "edge_angle_out",
+                 "half_gap", "k1", "k2", "field_index", "kinetic_energy")
+
+
 class OrcElement(C.Structure):
     _fields_ = [("type", C.c_int32), ("aperture_in", C.c_int32), ("length", C.c_double),
                 ("aperture", C.c_double), ("gradient", C.c_double), ("angle", C.c_double),
                 ("bl_h", C.c_double), ("bl_v", C.c_double), ("aper_xl", C.c_double),
                 ("aper_xr", C.c_double), ("aper_yt", C.c_double), ("aper_yb", C.c_double),
-                ("fraction", C.c_double), ("gap", GapParam)]
+                ("fraction", C.c_double), ("gap", GapParam)] + [
+                    (f, C.c_double) for f in ELEMENT_EXTRA] + [
+                    ("on", C.c_int32), ("pad_", C.c_int32)]
 
 
 STAT_FIELDS = ("x_avg", "x_avg_good", "xp_avg", "y_avg", "y_avg_good", "yp_avg", "phi_avg",
@@ -142,6 +151,8 @@ def lib():
             getattr(L, f).restype = C.c_double
             getattr(L, f).argtypes = [C.c_double]
         L.orc_t_sincos.argtypes = [C.c_double, dp, dp]
+        L.orc_t_tan.restype = C.c_double
+        L.orc_t_tan.argtypes = [C.c_double]
         L.orc_t_fmod_pos.restype = C.c_double
         L.orc_t_fmod_pos.argtypes = [C.c_double, C.c_double]
         L.orc_t_ttf.argtypes = [C.c_double, C.POINTER(GapParam), dp, dp, dp]
@@ -180,6 +191,10 @@ def to_orc_elements(elements):
         if e.gap:
             for f in GAP_FIELDS:
                 setattr(o.gap, f, float(e.gap.get(f, 0.0)))
+        params = getattr(e, "params", None) or {}
+        for f in ELEMENT_EXTRA:
+            setattr(o, f, float(params.get(f, 0.0)))
+        o.on = 1 if params.get("on", False) else 0
     return arr
 
 

@@ -196,6 +196,15 @@ static inline double orc_cosh(double x)
   return 0.5 * e + 0.5 / e;
 }
 
+/* tan(x) = sin(x) / cos(x) from orc_sincos (one rounding on top of the
+ * two ~1-ulp results, so within a few ulp away from the poles). */
+static inline double orc_tan(double x)
+{
+  double s, c;
+  orc_sincos(x, &s, &c);
+  return s / c;
+}
+
 /* fmod for x >= 0 (or -0.0), y > 0 finite: exact remainder. */
 static inline double orc_fmod_pos(double x, double y)
 {

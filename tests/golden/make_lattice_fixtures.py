@@ -1,8 +1,9 @@
 """Copies the lattice databases that the reference's own simulation scripts
 load (pytest/sim-lbeg.py: dbs = ['tbtd.db', 'dtl.db', 'trst.db', 'ccl.db'])
-into this folder as fixtures: the DTL (dtl.db) and CCL (ccl.db) sections,
-whose element kinds the engine tracks.  Data only (SQLite tables, views and
-triggers); run once where the reference tree is present:
+into this folder as fixtures: the low-energy transport (tbtd.db, with
+bunchers and dipoles), the DTL (dtl.db), the transition region (trst.db)
+and the CCL (ccl.db).  Data only (SQLite tables, views and triggers); run
+once where the reference tree is present:
 
     python tests/golden/make_lattice_fixtures.py /root/reference/db
 """
@@ -15,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def main(src):
-    for name in ("dtl.db", "ccl.db"):
+    for name in ("tbtd.db", "dtl.db", "trst.db", "ccl.db"):
         dst = os.path.join(HERE, name)
         shutil.copyfile(os.path.join(src, name), dst)
         os.chmod(dst, 0o644)

@@ -147,9 +147,27 @@ def test_facade_database_beamline(tmp_path):
 
 def test_unsupported_element_kinds_are_refused():
     from hpsim_amd import HPSim
-    els = [cfg.Element("B1", "Buncher"), cfg.Element("D1", "Drift", length=0.1)]
-    with pytest.raises(NotImplementedError, match="Buncher"):
+    els = [cfg.Element("S1", "Solenoid"), cfg.Element("D1", "Drift", length=0.1)]
+    with pytest.raises(NotImplementedError, match="Solenoid"):
         HPSim.BeamLine(elements=els)
+
+
+def test_full_lansce_lattice_loads(tmp_path, oracle_lib):
+    """The four sections of the reference's LBEG script (sim-lbeg.py:40-44)
+    as one beamline: transport with bunchers and dipoles, DTL, transition
+    region, CCL; the engine and the oracle accept every element."""
+    from hpsim_amd import HPSim
+    c = db.DBConnection(_copy(tmp_path, "tbtd.db"))
+    for k, name in enumerate(("dtl.db", "trst.db", "ccl.db")):
+        c.attach_db(_copy(tmp_path, name), f"db{k + 1}")
+    c.clear_model_index()
+    bl = HPSim.BeamLine(c)
+    els = list(bl)
+    kinds = collections.Counter(e.kind for e in els)
+    assert len(els) == 6540 and kinds["RFGap-DTL"] == 165 and kinds["RFGap-CCL"] == 4959
+    assert kinds["Dipole"] == 8 and kinds["Buncher"] == 3
+    oracle_lib.Simulator(oracle_lib.Beam(4, MASS, -1.0, 0.0), els, None)
+    c.close()
 
 
 def _beam(n, w0, phi0, seed):
@@ -189,6 +207,31 @@ def test_gpu_dtl_lattice_with_space_charge_bit_exact(oracle_lib, dtl):
     _, els = dtl
     gb = _gpu_vs_oracle(oracle_lib, els, 2048, 0.75, -2.758, 16, 64)
     assert gb.get_loss_num("t") < 2048 // 2
+
+
+def _section(tmp_path, name):
+    c = db.DBConnection(_copy(tmp_path, name))
+    c.clear_model_index()
+    els = db.beamline_elements(c)
+    c.close()
+    return els
+
+
+@pytest.mark.gpu
+def test_gpu_tbtd_transport_with_bunchers_and_dipoles_bit_exact(oracle_lib, tmp_path):
+    """Low-energy transport: bunchers (one off), dipoles with fringe fields
+    and pole-face rotations, steerers, apertures, space-charge
+    compensation."""
+    els = _section(tmp_path, "tbtd.db")
+    _gpu_vs_oracle(oracle_lib, els, 2048, 0.7518, 0.0, 16, 64)
+
+
+@pytest.mark.gpu
+def test_gpu_trst_transition_dipoles_bit_exact(oracle_lib, tmp_path):
+    """Transition region at 100 MeV: defocusing field index (hyperbolic
+    branch of the dipole map), rotations, rectangular apertures."""
+    els = _section(tmp_path, "trst.db")
+    _gpu_vs_oracle(oracle_lib, els, 2048, 99.9636448918256, 0.0, 16, 64)
 
 
 @pytest.mark.gpu

@@ -489,3 +489,85 @@ def test_openmp_oracle_is_bit_identical_to_serial(oracle_lib, tmp_path):
     a, o = ser.arrays(), np.load(out)
     for k in a:
         assert np.array_equal(a[k].view(np.uint8), o[k].view(np.uint8)), k
+
+
+# --------------------------------------------------------------------------
+# dipole and buncher maps (simulation_kernel.cu:603-874)
+# --------------------------------------------------------------------------
+
+def test_tan_within_a_few_ulp(L):
+    xs = np.linspace(-1.4, 1.4, 20001)
+    got = vec(L.orc_t_tan, xs)
+    assert np.max(np.abs(got - np.tan(xs)) / np.abs(np.spacing(np.tan(xs)))) <= 4
+
+
+def _track(O, els, x, xp, y, yp, phi, w, freq=201.25):
+    b = O.Beam(len(x), 939.294, -1.0, 0.0)
+    b.set_distribution(x, xp, y, yp, phi, w)
+    b.set_frequency(freq)
+    b.set_ref_w(w[0])
+    O.Simulator(b, els, None).simulate()
+    return b.arrays()
+
+
+def _dipole_matrices(O, dip, w):
+    """Linear transfer blocks from four small unit offsets (the maps are
+    linear in x, x', y, y' at the design energy)."""
+    eps = 1e-7
+    z = np.zeros(5)
+    x, xp, y, yp = z.copy(), z.copy(), z.copy(), z.copy()
+    x[1], xp[2], y[3], yp[4] = eps, eps, eps, eps
+    a = _track(O, [dip], x, xp, y, yp, z, np.full(5, w))
+    mx = np.array([[a["x"][1], a["x"][2]], [a["xp"][1], a["xp"][2]]]) / eps
+    my = np.array([[a["y"][3], a["y"][4]], [a["yp"][3], a["yp"][4]]]) / eps
+    return mx, my, a
+
+
+def test_sector_dipole_matrices(oracle_lib):
+    """Field index n: x focuses with k^2 = h^2 (1 - n), y with h^2 n; n = 0 is
+    the textbook sector bend; the two halves compose to the full arc."""
+    rho, th, w = 1.5, math.radians(30), 0.75
+    for n in (0.0, 0.5):
+        dip = cfg.Element("B", "Dipole", angle=th,
+                          params=dict(rho=rho, kinetic_energy=w, field_index=n))
+        mx, my, a = _dipole_matrices(oracle_lib, dip, w)
+        L_ = rho * th
+        kx, ky = math.sqrt(1 - n) / rho, math.sqrt(n) / rho
+        ex = [[math.cos(kx * L_), math.sin(kx * L_) / kx],
+              [-kx * math.sin(kx * L_), math.cos(kx * L_)]]
+        ey = ([[1, L_], [0, 1]] if n == 0 else
+              [[math.cos(ky * L_), math.sin(ky * L_) / ky], [-ky * math.sin(ky * L_), math.cos(ky * L_)]])
+        assert np.allclose(mx, ex, atol=1e-8) and np.allclose(my, ey, atol=1e-8)
+        assert abs(np.linalg.det(mx) - 1) < 1e-8 and abs(np.linalg.det(my) - 1) < 1e-8
+        g = w / 939.294 + 1
+        beta = math.sqrt(1 - 1 / g ** 2)
+        assert a["phi"][0] == pytest.approx(2 * math.pi * L_ / (beta * 299.792458 / 201.25),
+                                            rel=1e-14)
+
+
+def test_dipole_edge_focusing_thin_lens(oracle_lib):
+    """A pole-face rotation b with no fringe field (K1 = 0) is a thin lens:
+    x' += tan(b)/rho x, y' -= tan(b)/rho y, at entrance and exit."""
+    rho, th, w, b = 2.0, 1e-9, 0.75, math.radians(12)
+    dip = cfg.Element("B", "Dipole", angle=th,
+                      params=dict(rho=rho, kinetic_energy=w, edge_angle_in=b))
+    mx, my, _ = _dipole_matrices(oracle_lib, dip, w)
+    assert mx[1, 0] == pytest.approx(math.tan(b) / rho, rel=1e-6)
+    assert my[1, 0] == pytest.approx(-math.tan(b) / rho, rel=1e-6)
+
+
+def test_buncher_energy_gain_and_adiabatic_damping(oracle_lib):
+    """On axis the gap gives dW = |q| V cos(h phi + phase); a particle with
+    x = 0 keeps beta*gamma*x' (x' scales by bg_in/bg_out)."""
+    w, v, ph0 = 0.75, 0.1, 0.3
+    bun = cfg.Element("U", "Buncher", params=dict(voltage=v, frequency=201.25, phase=ph0, on=True))
+    phi = np.array([0.0, 0.2, -0.5, 1.1])
+    xp = np.array([0.0, 1e-3, -2e-3, 5e-4])
+    a = _track(oracle_lib, [bun], np.zeros(4), xp, np.zeros(4), np.zeros(4), phi, np.full(4, w))
+    dw = v * np.cos(phi + ph0)
+    assert np.allclose(a["w"] - w, dw, rtol=1e-12, atol=1e-15)
+    bg = lambda e: np.sqrt((e / 939.294) * (e / 939.294 + 2))
+    assert np.allclose(a["xp"][1:], xp[1:] * bg(w) / bg(w + dw[1:]), rtol=1e-12)
+    off = cfg.Element("U", "Buncher", params=dict(voltage=v, frequency=201.25, phase=ph0, on=False))
+    b = _track(oracle_lib, [off], np.zeros(4), xp, np.zeros(4), np.zeros(4), phi, np.full(4, w))
+    assert np.array_equal(b["w"], np.full(4, w))
