@@ -32,6 +32,20 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
 SEG_BYTES_PER_PARTICLE = 104   # k_segment: reads 6 FP64 + 2 u32, writes 6 FP64
+# HBM bytes per k_segment launch at 1,048,576 particles, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE counter runs (calibrated on a plain device copy)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_segment_traffic_plain-fp64-1048576x200.md")
+
+
+def measured_traffic(num):
+    """Traffic per k_segment launch from the committed counter summary, for
+    the particle count it was measured at; None otherwise."""
+    if num != 1048576 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
 
 
 def parse():
@@ -166,7 +180,7 @@ def main():
                    "mesh": f"{wl.nr}x{wl.nz}", "sc_kicks_per_step": prof["phase_final"][1],
                    "transmitted": b.num - lost},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(b.num),
                      "kernel": "k_segment", "avg_launch_us": avg_s * 1e6,
                      "launches": seg_n, "bytes_per_launch": SEG_BYTES_PER_PARTICLE * b.num,
                      "timing": "HIP events around every launch of one eager step after "

@@ -61,6 +61,106 @@ __host__ __device__ inline int32_t op_class_of(int32_t kind)
   return OPC_LIGHT;
 }
 
+// Element constants of the two RF-gap half maps: every subexpression of
+// SimulateRFGapFirst/SecondHalfKernel that does not depend on the particle,
+// evaluated once per map on the host by gap_uniform() with exactly the
+// expressions the kernels use (IEEE double, no contraction), so the per
+// particle arithmetic below is unchanged bit for bit.
+struct GapUniform {
+  // first half (simulation_kernel.cu:239-285)
+  double cell_len, dd1, wave_len, twopi_dd1, beta_in1, pclbl_dps1, twopi_half_q1, pi_dcl1,
+         betag1;
+  int32_t dcl1_nonzero, pad0_;
+  // second half (simulation_kernel.cu:326-493)
+  double dd2, inv_mass, beta_in2, twopi_dd2, betag_ccl, gmg, inv_gmg2, q, beta_out,
+         concomm, concomm1, neg_con1, neg_con2nex, con2nex, inv_gmgcl, inv_fkis, fkr, sinps,
+         cosps, bgw, twopi_dg, cell_pi, q_efl, two_con1, efl_inv_mass, pclbl_dps2,
+         twopi_half_q2, pi_dcl2;
+  int32_t dcl2_nonzero, pad1_;
+};
+
+// design_w_in, phi_in (first half), phi (second half: phi_out for CCL),
+// length, q1, q2: the arguments the reference passes to the two kernels.
+HPS_HD GapUniform gap_uniform(const hps_gap_param& gap, double design_w_in, double phi_in1,
+                              double phi_in2, double length, double q1, double q2,
+                              uint32_t ccl_cell_num, bool ccl, double mass, double charge)
+{
+  GapUniform u{};
+  // first half
+  u.cell_len = length + q1 * 0.5 + q2 * 0.5;
+  u.dd1 = 0.5 * u.cell_len - gap.dg - 0.5 * q1;
+  u.wave_len = CLIGHT / gap.frequency;
+  u.twopi_dd1 = TWOPI * u.dd1;
+  {
+    const double gam = design_w_in / mass;
+    u.beta_in1 = ::sqrt((gam + 2.0) * gam) / (gam + 1.0);
+  }
+  if (!ccl) {
+    const double dps = gap.phase_ref - phi_in1;
+    const double d_cell_len = u.cell_len - gap.beta_center * u.wave_len;
+    u.twopi_half_q1 = TWOPI * 0.5 * q1;
+    u.dcl1_nonzero = (d_cell_len > 1e-15 || d_cell_len < -1e-15) ? 1 : 0;
+    u.pi_dcl1 = PI * d_cell_len;
+    u.pclbl_dps1 = PI * gap.cell_length_over_beta_lambda + dps;
+  } else {
+    const double dps = gap.phase_ref - phi_in1;
+    u.betag1 = u.cell_len / (gap.cell_length_over_beta_lambda * u.wave_len);
+    u.pclbl_dps1 = PI * gap.cell_length_over_beta_lambda + dps;
+  }
+  // second half
+  u.dd2 = 0.5 * u.cell_len + gap.dg - q2 * 0.5;
+  u.inv_mass = 1.0 / mass;
+  const double gm_in = design_w_in * u.inv_mass + 1.0;
+  u.beta_in2 = ::sqrt(1.0 - 1.0 / (gm_in * gm_in));
+  u.twopi_dd2 = TWOPI * u.dd2;
+  const double cl = u.cell_len / gap.cell_length_over_beta_lambda;
+  u.betag_ccl = cl / u.wave_len;
+  const double betag = ccl ? gap.beta_center : u.betag_ccl;
+  const double betag2 = betag * betag;
+  u.gmg = 1.0 / ::sqrt(1.0 - betag2);
+  u.inv_gmg2 = 1.0 / (u.gmg * u.gmg);
+  double q = charge;
+  q = q > 0.0 ? q : -q;
+  u.q = q;
+  const double gm_out = gap.energy_out * u.inv_mass + 1.0;
+  u.beta_out = ::sqrt(1.0 - 1.0 / (gm_out * gm_out));
+  const double efl = gap.amplitude * u.cell_len;
+  u.concomm = q * efl / (mass * betag2);
+  u.concomm1 = u.concomm * u.inv_gmg2 / u.gmg;
+  const double con1 = PI * u.concomm1;
+  u.neg_con1 = -con1;
+  u.con2nex = u.concomm * u.inv_gmg2;
+  u.neg_con2nex = -u.con2nex;
+  if (ccl) {
+    u.inv_gmgcl = 1.0 / (u.gmg * betag * u.wave_len);
+    u.inv_fkis = betag * u.wave_len / TWOPI;
+    u.bgw = u.betag_ccl * u.wave_len;
+  } else {
+    u.inv_gmgcl = 1.0 / (u.gmg * cl);
+    u.inv_fkis = cl / TWOPI;
+    u.bgw = u.beta_in2 * u.wave_len;
+  }
+  u.fkr = TWOPI * u.inv_gmgcl;
+  hps::sincos(gap.phase_ref, &u.sinps, &u.cosps);
+  u.twopi_dg = TWOPI * gap.dg;
+  u.cell_pi = ccl_cell_num * PI;
+  u.q_efl = q * efl;
+  u.two_con1 = 2.0 * con1;
+  u.efl_inv_mass = efl * u.inv_mass;
+  if (ccl) {
+    const double dps = phi_in2 - gap.phase_ref;
+    u.pclbl_dps2 = PI * gap.cell_length_over_beta_lambda + dps;
+  } else {
+    const double d_cell_len = u.cell_len - gap.beta_center * u.wave_len;
+    u.twopi_half_q2 = TWOPI * 0.5 * q2;
+    u.dcl2_nonzero = (d_cell_len > 1e-15 || d_cell_len < -1e-15) ? 1 : 0;
+    u.pi_dcl2 = PI * d_cell_len;
+    const double dps = gap.phase_ref - phi_in2;
+    u.pclbl_dps2 = PI * gap.cell_length_over_beta_lambda + dps;
+  }
+  return u;
+}
+
 // One map step.  a[] meaning per kind:
 //  DRIFT  h, aperture, wavelength
 //  QUAD   length, aperture, gradient, wavelength
@@ -79,6 +179,7 @@ struct Op {
   uint32_t ccl_cell_num;
   double a[10];
   hps_gap_param g;
+  GapUniform gu;          // GAP1 / GAP2 element constants (gap_uniform)
 };
 
 // Bessi0c, simulation_kernel.cu:15-36
@@ -125,6 +226,33 @@ __device__ __forceinline__ double bessi1p(double r_x)
 {
   if (r_x != 0.0) return bessi0c(r_x) - bessi1c(r_x) / r_x;
   return 0.5;
+}
+
+// bessi0c(x), bessi1c(x) and bessi1p(x) of one argument, sharing the
+// common factors (identical results to the three separate calls).
+__device__ __forceinline__ void bessel_all(double r_x, double& b0, double& b1, double& b1p)
+{
+  const double ax = __builtin_fabs(r_x);
+  if (ax < 3.75) {
+    double y = r_x / 3.75;
+    y = y * y;
+    b0 = 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 +
+         y * (0.360768e-1 + y * 0.45813e-2)))));
+    b1 = ax * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 + y * (0.2658733e-1 +
+         y * (0.301532e-2 + y * 0.32411e-3))))));
+  } else {
+    const double y = 3.75 / ax;
+    const double f = hps::exp(ax) * (1.0 / ::sqrt(ax));
+    b0 = f * (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 +
+         y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1 +
+         y * 0.392377e-2))))))));
+    double a1 = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
+    a1 = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 + y * (0.163801e-2 + y *
+         (-0.1031555e-1 + y * a1))));
+    b1 = a1 * f;
+  }
+  if (r_x < 0.0) b1 = -b1;
+  b1p = r_x != 0.0 ? b0 - b1 / r_x : 0.5;
 }
 
 // GetTransitTimeFactors, simulation_kernel.cu:290-308
@@ -185,7 +313,8 @@ __device__ __forceinline__ void half_quad(Particle& p, double length, double ape
     const double inv_k = 1.0 / k;
     const double kds = length * 0.25 * k;
     double s, c;
-    const double sh = hps::sinh(kds), ch = hps::cosh(kds);
+    double sh, ch;
+    hps::sinhcosh(kds, &sh, &ch);
     hps::sincos(kds, &s, &c);
     const double sovk = s * inv_k, shovk = sh * inv_k, sk = s * k, shk = sh * k;
     double xx, yy;
@@ -212,122 +341,85 @@ __device__ __forceinline__ void half_quad(Particle& p, double length, double ape
 
 // SimulateRFGapFirstHalfKernel body, simulation_kernel.cu:239-285
 __device__ __forceinline__ void gap_first_half(Particle& p, const hps_gap_param& gap,
-                                               double design_w_in, double phi_in, double length,
-                                               double q1, double q2, bool ccl, double mass)
+                                               const GapUniform& u, bool ccl, double mass)
 {
   const double x = p.x, y = p.y, xp = p.xp, yp = p.yp, phi = p.phi, w = p.w;
-  const double cell_len = length + q1 * 0.5 + q2 * 0.5;
-  const double dd1 = 0.5 * cell_len - gap.dg - 0.5 * q1;
-  p.x = x + dd1 * xp;
-  p.y = y + dd1 * yp;
-  const double wave_len = CLIGHT / gap.frequency;
+  p.x = x + u.dd1 * xp;
+  p.y = y + u.dd1 * yp;
   if (gap.amplitude == 0.0) {
     const double gm1 = w / mass;
     const double bt = ::sqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
-    p.phi += TWOPI * dd1 / (wave_len * bt);
+    p.phi += u.twopi_dd1 / (u.wave_len * bt);
   } else {
-    double gam = design_w_in / mass;
-    const double beta_in = ::sqrt((gam + 2.0) * gam) / (gam + 1.0);
-    gam = w / mass;
+    const double beta_in = u.beta_in1;
+    const double gam = w / mass;
     const double beta = ::sqrt((gam + 2.0) * gam) / (gam + 1.0);
     if (!ccl) {
-      const double dps = gap.phase_ref - phi_in;
-      const double d_cell_len = cell_len - gap.beta_center * wave_len;
-      const double dphi_qlen1 = TWOPI * 0.5 * q1 / (beta * wave_len);
+      const double dphi_qlen1 = u.twopi_half_q1 / (beta * u.wave_len);
       double dphi_clen = 0.0;
-      if (d_cell_len > 1e-15 || d_cell_len < -1e-15)
-        dphi_clen = PI * d_cell_len / (beta * wave_len);
-      p.phi = phi + (1.0 - (beta - beta_in) / beta) *
-              (PI * gap.cell_length_over_beta_lambda + dps) + dphi_clen - dphi_qlen1;
+      if (u.dcl1_nonzero) dphi_clen = u.pi_dcl1 / (beta * u.wave_len);
+      p.phi = phi + (1.0 - (beta - beta_in) / beta) * u.pclbl_dps1 + dphi_clen - dphi_qlen1;
     } else {
-      const double dps = gap.phase_ref - phi_in;
-      const double betag = cell_len / (gap.cell_length_over_beta_lambda * wave_len);
-      p.phi = phi + (1.0 - (beta - beta_in) * betag / (beta * beta_in)) *
-              (PI * gap.cell_length_over_beta_lambda + dps);
+      p.phi = phi + (1.0 - (beta - beta_in) * u.betag1 / (beta * beta_in)) * u.pclbl_dps1;
     }
   }
 }
 
 // SimulateRFGapSecondHalfKernel body, simulation_kernel.cu:326-493
 __device__ __forceinline__ void gap_second_half(Particle& p, const hps_gap_param& gap,
-                                                double design_w_in, double phi_in,
-                                                double length, double q1, double q2,
-                                                double wavelen, uint32_t ccl_cell_num, bool ccl,
-                                                double mass, double charge)
+                                                const GapUniform& u, double wavelen, bool ccl,
+                                                double mass)
 {
-  const double cell_len = length + q1 * 0.5 + q2 * 0.5;
-  const double dd2 = 0.5 * cell_len + gap.dg - q2 * 0.5;
   double x = p.x, y = p.y, xp = p.xp, yp = p.yp;
-  const double inv_mass = 1.0 / mass;
+  const double inv_mass = u.inv_mass;
   const double phi = p.phi;
   double w = p.w;
-  const double gm_in = design_w_in * inv_mass + 1.0;
-  const double beta_in = ::sqrt(1.0 - 1.0 / (gm_in * gm_in));
+  const double beta_in = u.beta_in2;
   double gam = w * inv_mass;
   const double bg1 = ::sqrt(gam * (gam + 2.0));
   double beta = bg1 / (gam + 1.0);
   double t_fac, tp_fac, sp_fac;
   ttf(beta, gap, t_fac, tp_fac, sp_fac);
   if (gap.amplitude == 0.0 || (ccl && beta < gap.fit_beta_min)) {
-    p.x = x + dd2 * xp;
-    p.y = y + dd2 * yp;
+    p.x = x + u.dd2 * xp;
+    p.y = y + u.dd2 * yp;
     const double gm1 = p.w / mass;
     const double bt = ::sqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
-    p.phi += TWOPI * dd2 / (wavelen * bt);
+    p.phi += u.twopi_dd2 / (wavelen * bt);
     return;
   }
-  const double wave_len = CLIGHT / gap.frequency;
-  const double cl = cell_len / gap.cell_length_over_beta_lambda;
-  const double betag_ccl = cl / wave_len;
-  const double betag = ccl ? gap.beta_center : betag_ccl;
-  const double betag2 = betag * betag;
-  const double gmg = 1.0 / ::sqrt(1.0 - betag2);
-  const double inv_gmg2 = 1.0 / (gmg * gmg);
-  double q = charge;
-  q = q > 0.0 ? q : -q;
-  const double gm_out = gap.energy_out * inv_mass + 1.0;
-  const double beta_out = ::sqrt(1.0 - 1.0 / (gm_out * gm_out));
-  const double efl = gap.amplitude * cell_len;
-  const double concomm = q * efl / (mass * betag2);
-  const double concomm1 = concomm * inv_gmg2 / gmg;
-  const double con1 = PI * concomm1;
-  const double con2nex = concomm * inv_gmg2;
-  double con5nex, tw, inv_fkis, inv_gmgcl;
+  const double wave_len = u.wave_len;
+  const double gmg = u.gmg, inv_gmg2 = u.inv_gmg2;
+  double con5nex, tw;
   if (ccl) {
-    inv_gmgcl = 1.0 / (gmg * betag * wave_len);
-    inv_fkis = betag * wave_len / TWOPI;
-    con5nex = concomm * (t_fac * inv_fkis - betag_ccl * wave_len * tp_fac * inv_gmg2);
-    tw = t_fac - TWOPI * tp_fac * (beta_in / beta - 1.0) * betag_ccl / beta_in;
+    con5nex = u.concomm * (t_fac * u.inv_fkis - u.bgw * tp_fac * inv_gmg2);
+    tw = t_fac - TWOPI * tp_fac * (beta_in / beta - 1.0) * u.betag_ccl / beta_in;
   } else {
-    inv_gmgcl = 1.0 / (gmg * cl);
-    inv_fkis = cl / TWOPI;
-    con5nex = concomm * (t_fac * inv_fkis - beta_in * wave_len * tp_fac * inv_gmg2);
+    con5nex = u.concomm * (t_fac * u.inv_fkis - u.bgw * tp_fac * inv_gmg2);
     tw = t_fac;
   }
-  const double con5nex1 = concomm1 * t_fac;
-  const double con5nex2 = -con2nex * t_fac * inv_fkis;
-  const double fkr = TWOPI * inv_gmgcl;
-  double sinps, cosps;
-  hps::sincos(gap.phase_ref, &sinps, &cosps);
+  const double con5nex1 = u.concomm1 * t_fac;
+  const double con5nex2 = u.neg_con2nex * t_fac * u.inv_fkis;
+  const double fkr = u.fkr;
+  const double sinps = u.sinps, cosps = u.cosps;
   double phi1, sinp1, cosp1;
   if (ccl)
-    phi1 = phi + gap.phase_shift - TWOPI * gap.dg / (beta * wave_len) + ccl_cell_num * PI;
+    phi1 = phi + gap.phase_shift - u.twopi_dg / (beta * wave_len) + u.cell_pi;
   else
     phi1 = phi + gap.phase_shift;
   hps::sincos(phi1, &sinp1, &cosp1);
-  const double prome1 = -con1 * (tp_fac * (sinp1 - sinps) + sp_fac * (cosp1 - cosps));
+  const double prome1 = u.neg_con1 * (tp_fac * (sinp1 - sinps) + sp_fac * (cosp1 - cosps));
   hps::sincos(phi1 + prome1, &sinp1, &cosp1);
   const double rs_p = x * x + y * y;
   const double rrp = x * xp + y * yp;
   double rpris = 0.0, bes1covr;
   const double ris = ::sqrt(rs_p);
   const double fkrris = fkr * ris;
-  const double b0c = bessi0c(fkrris);
-  const double b1c = bessi1c(fkrris);
-  const double b1p = bessi1p(fkrris);
+  double b0c, b1c, b1p;
+  bessel_all(fkrris, b0c, b1c, b1p);
   if (ris != 0.0) bes1covr = b1c / ris;
   else bes1covr = fkr * 0.5;
-  const double delte = q * efl * (tw * b0c * cosp1 + ((tw - TWOPI * tp_fac) / gmg * bes1covr +
+  const double delte = u.q_efl * (tw * b0c * cosp1 + ((tw - TWOPI * tp_fac) / gmg * bes1covr +
                        tw / gmg * fkr * b1p - tw * gmg * bes1covr) * rrp * sinp1);
   w += delte;
   gam = w * inv_mass;
@@ -335,29 +427,25 @@ __device__ __forceinline__ void gap_second_half(Particle& p, const hps_gap_param
   beta = bg / (gam + 1.0);
   const double bgr = bg1 / bg;
   const double onema = 1.0 - (con5nex * bes1covr + con5nex1 * b1p + con5nex2 * bes1covr) * cosp1;
-  const double dovbl = con2nex * bes1covr * tw * sinp1;
+  const double dovbl = u.con2nex * bes1covr * tw * sinp1;
   const double bgrovonema = bgr / onema;
   xp = -dovbl * x + bgrovonema * xp;
-  x = x * onema + dd2 * xp;
+  x = x * onema + u.dd2 * xp;
   yp = -dovbl * y + bgrovonema * yp;
-  y = y * onema + dd2 * yp;
+  y = y * onema + u.dd2 * yp;
   if (rs_p > 0.0) rpris = rrp / ris;
-  const double prome = 2.0 * con1 * (-tp_fac * b0c + ris * tw * b1c * inv_gmgcl) * (sinp1 - sinps) -
-                       efl * inv_mass * tw * b1c * rpris * (cosp1 - cosps);
+  const double prome = u.two_con1 * (-tp_fac * b0c + ris * tw * b1c * u.inv_gmgcl) *
+                       (sinp1 - sinps) - u.efl_inv_mass * tw * b1c * rpris * (cosp1 - cosps);
   double phi_new;
   if (ccl) {
-    const double dps = phi_in - gap.phase_ref;
-    phi_new = phi + (1.0 - (beta - beta_out) * betag_ccl / (beta * beta_out)) *
-              (PI * gap.cell_length_over_beta_lambda + dps) + prome;
+    phi_new = phi + (1.0 - (beta - u.beta_out) * u.betag_ccl / (beta * u.beta_out)) *
+              u.pclbl_dps2 + prome;
   } else {
-    const double d_cell_len = cell_len - gap.beta_center * wave_len;
-    const double dphi_qlen2 = TWOPI * 0.5 * q2 / (beta * wave_len);
+    const double dphi_qlen2 = u.twopi_half_q2 / (beta * wave_len);
     double dphi_clen = 0.0;
-    if (d_cell_len > 1e-15 || d_cell_len < -1e-15)
-      dphi_clen = PI * d_cell_len / (beta * wave_len);
-    const double dps = gap.phase_ref - phi_in;
-    phi_new = phi + (1.0 - (beta - beta_out) / beta) * (PI * gap.cell_length_over_beta_lambda +
-              dps) + prome + dphi_clen - dphi_qlen2;
+    if (u.dcl2_nonzero) dphi_clen = u.pi_dcl2 / (beta * wave_len);
+    phi_new = phi + (1.0 - (beta - u.beta_out) / beta) * u.pclbl_dps2 + prome + dphi_clen -
+              dphi_qlen2;
   }
   if (w <= 0.0 || x != x || y != y || xp != xp || yp != yp || w != w || phi_new != phi_new) {
     p.loss = 33333333;
@@ -455,10 +543,10 @@ __device__ __forceinline__ DipoleHalf dipole_half(const double* a, double mass)
   const double xkl = xk * dl;
   double cxkl = 1.0, sxokx = dl, s, c;
   if (xk2 > 0.0) { hps::sincos(xkl, &s, &c); cxkl = c; sxokx = s * inv_xk; }
-  else if (xk2 < 0.0) { cxkl = hps::cosh(xkl); sxokx = hps::sinh(xkl) * inv_xk; }
+  else if (xk2 < 0.0) { double sh; hps::sinhcosh(xkl, &sh, &cxkl); sxokx = sh * inv_xk; }
   double cykl = 1.0, syoky = dl;
   if (yk2 > 0.0) { hps::sincos(ykl, &s, &c); cykl = c; syoky = s * inv_yk; }
-  else if (yk2 < 0.0) { cykl = hps::cosh(ykl); syoky = hps::sinh(ykl) * inv_yk; }
+  else if (yk2 < 0.0) { double sh; hps::sinhcosh(ykl, &sh, &cykl); syoky = sh * inv_yk; }
   o.r11 = cxkl; o.r12 = sxokx; o.r21 = -xk2 * sxokx; o.r22 = cxkl;
   o.r33 = cykl; o.r34 = syoky; o.r43 = -yk2 * syoky; o.r44 = cykl;
   o.r26 = h * sxokx;
@@ -536,13 +624,11 @@ __device__ __forceinline__ void apply_op(const Op& op, Particle& p, uint32_t ind
         half_quad(p, op.a[0], op.a[1], op.a[2], op.a[3], op.elem, mass, charge);
       break;
     case OP_GAP1:
-      if constexpr (kClass >= OPC_GAP)
-        gap_first_half(p, op.g, op.a[0], op.a[1], op.a[2], op.a[3], op.a[4], op.ccl != 0, mass);
+      if constexpr (kClass >= OPC_GAP) gap_first_half(p, op.g, op.gu, op.ccl != 0, mass);
       break;
     case OP_GAP2:
       if constexpr (kClass >= OPC_GAP)
-        gap_second_half(p, op.g, op.a[0], op.a[1], op.a[2], op.a[3], op.a[4], op.a[5],
-                        op.ccl_cell_num, op.ccl != 0, mass, charge);
+        gap_second_half(p, op.g, op.gu, op.a[5], op.ccl != 0, mass);
       break;
     case OP_BUNCH:
       if constexpr (kClass >= OPC_GAP) buncher(p, op.a, op.elem, mass, charge);

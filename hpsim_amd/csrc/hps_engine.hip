@@ -1126,8 +1126,11 @@ int compile_element(Compiler& c, uint32_t id)
         cell = m->ccl_cell_num++;
       }
       const bool run = ccl || c.quad_gap_run;
+      const GapUniform gu = gap_uniform(e.gap, m->design_w, phi_in, ccl ? phi_out : phi_in,
+                                        e.length, q1, q2, cell, ccl, m->b->mass,
+                                        m->b->charge);
       Op o1 = make_op(OP_GAP1, id);
-      o1.ccl = ccl; o1.g = e.gap;
+      o1.ccl = ccl; o1.g = e.gap; o1.gu = gu;
       o1.a[0] = m->design_w; o1.a[1] = phi_in; o1.a[2] = e.length; o1.a[3] = q1; o1.a[4] = q2;
       if (run) {
         c.ops.push_back(o1);
@@ -1138,7 +1141,7 @@ int compile_element(Compiler& c, uint32_t id)
         c.kick_point(e.length);
       }
       Op o2 = make_op(OP_GAP2, id);
-      o2.ccl = ccl; o2.ccl_cell_num = cell; o2.g = e.gap;
+      o2.ccl = ccl; o2.ccl_cell_num = cell; o2.g = e.gap; o2.gu = gu;
       o2.a[0] = m->design_w; o2.a[1] = ccl ? phi_out : phi_in; o2.a[2] = e.length;
       o2.a[3] = q1; o2.a[4] = q2; o2.a[5] = m->wavelen;
       if (run) c.ops.push_back(o2);

@@ -27,6 +27,10 @@ struct ScDev {
   int32_t valid;        // tables hold a field
   double beta, freq, frac, current;
   double ratio_r, ratio_z, ratio_q, ratio_g;
+  // kick constants of ApplySpchKickKernel (space_charge_kernel.cu:578-640)
+  // that the control step fixes, computed once per kick point
+  double k_xbar, k_ybar, k_epsq, k_gmg, k_dr, k_dzg, k_hl, k_c1, k_c2, k_bgmg;
+  double k_xfac, k_yfac, k_er_ratio, k_ez_ratio;
 };
 
 // Scratch for the canonical reductions: per-chunk partial sums, padded to a

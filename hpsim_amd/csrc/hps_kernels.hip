@@ -132,34 +132,62 @@ struct KickConst {
   uint32_t nx, nz;
 };
 
+// The kick constants of ApplySpchKickKernel (space_charge_kernel.cu:578-640):
+// the parts fixed by the control step come from ScDev (kick_prepare in
+// k_phase_final); the element length, the on-mesh count (k_deposit) and the
+// phase references (which a host frequency step may change) are read here.
 __device__ __forceinline__ KickConst kick_const(const ScDev* sc, const hps_beam_stats* st,
-                                                double r_length, double r_mass, uint32_t r_num_part)
+                                                double r_length)
 {
   KickConst k;
-  const double r_beta = sc->beta, r_freq = sc->freq;
-  const double r_current = sc->current * sc->frac;
   k.phimc = st->phi_avg_r;
-  k.xbar = st->x_avg_good;
-  k.ybar = st->y_avg_good;
-  k.epsq = st->x_sig_good / st->y_sig_good;
-  k.gmg = 1.0 / ::sqrt(1.0 - r_beta * r_beta);
-  k.dr = sc->st.dr * sc->ratio_r;
-  k.dzg = sc->st.dz * k.gmg * sc->ratio_z;
-  const double wave_len = CLIGHT / r_freq;
+  k.xbar = sc->k_xbar;
+  k.ybar = sc->k_ybar;
+  k.epsq = sc->k_epsq;
+  k.gmg = sc->k_gmg;
+  k.dr = sc->k_dr;
+  k.dzg = sc->k_dzg;
   k.nx = sc->st.nx;
   k.nz = sc->st.nz;
-  k.hl = k.nz * 0.5 * k.dzg;
-  const double c1 = 5721.67 * r_current / (r_freq * 1e6 * (double)r_num_part * r_mass);
-  k.c2 = r_beta * k.gmg * wave_len / TWOPI;
-  k.c3 = r_length / (r_beta * k.gmg);
-  k.xfac = 2.0 / (k.epsq + 1.0);
-  k.yfac = k.epsq * k.xfac;
-  k.er_ratio = sc->ratio_q / (sc->ratio_r * sc->ratio_z * sc->ratio_g * sc->ratio_g);
-  k.ez_ratio = sc->ratio_q / (sc->ratio_z * sc->ratio_z);
-  k.cbgcoef = c1 * k.c3 * PI * 0.5 * sc->st.num_on_mesh;
-  k.r_beta = r_beta;
+  k.hl = sc->k_hl;
+  k.c2 = sc->k_c2;
+  k.c3 = r_length / sc->k_bgmg;
+  k.xfac = sc->k_xfac;
+  k.yfac = sc->k_yfac;
+  k.er_ratio = sc->k_er_ratio;
+  k.ez_ratio = sc->k_ez_ratio;
+  k.cbgcoef = sc->k_c1 * k.c3 * PI * 0.5 * sc->st.num_on_mesh;
+  k.r_beta = sc->beta;
   k.phi_ref = st->phi_avg_good;
   return k;
+}
+
+// The control-step part of the kick constants, in the expressions of
+// ApplySpchKickKernel; r_num_part, r_mass: the beam's size and mass.
+__device__ __forceinline__ void kick_prepare(ScDev* sc, double r_beta, double r_freq,
+    double r_current, double x_avg_good, double y_avg_good, double x_sig_good,
+    double y_sig_good, double dr, double dz, uint32_t nz, double ratio_r, double ratio_z,
+    double ratio_q, double ratio_g, uint32_t r_num_part, double r_mass)
+{
+  const double epsq = x_sig_good / y_sig_good;
+  const double gmg = 1.0 / ::sqrt(1.0 - r_beta * r_beta);
+  const double dzg = dz * gmg * ratio_z;
+  const double wave_len = CLIGHT / r_freq;
+  const double xfac = 2.0 / (epsq + 1.0);
+  sc->k_xbar = x_avg_good;
+  sc->k_ybar = y_avg_good;
+  sc->k_epsq = epsq;
+  sc->k_gmg = gmg;
+  sc->k_dr = dr * ratio_r;
+  sc->k_dzg = dzg;
+  sc->k_hl = nz * 0.5 * dzg;
+  sc->k_c1 = 5721.67 * r_current / (r_freq * 1e6 * (double)r_num_part * r_mass);
+  sc->k_c2 = r_beta * gmg * wave_len / TWOPI;
+  sc->k_bgmg = r_beta * gmg;
+  sc->k_xfac = xfac;
+  sc->k_yfac = epsq * xfac;
+  sc->k_er_ratio = ratio_q / (ratio_r * ratio_z * ratio_g * ratio_g);
+  sc->k_ez_ratio = ratio_q / (ratio_z * ratio_z);
 }
 
 __device__ __forceinline__ void kick(Particle& p, const KickConst& k, const double* tbl,
@@ -274,6 +302,21 @@ __device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem& 
   }
 }
 
+// Particle i of the beam, or an excluded placeholder (loss = 1) past the end.
+__device__ __forceinline__ Particle load_particle(const BeamArrays& b, uint32_t i)
+{
+  Particle p;
+  p.loss = 1; p.lloss = 0;
+  p.x = p.xp = p.y = p.yp = p.phi = p.w = 0.0;
+  if (i < b.n) {
+    p.loss = b.loss[i];
+    p.lloss = b.lloss[i];
+    p.x = b.x[i]; p.xp = b.xp[i]; p.y = b.y[i]; p.yp = b.yp[i];
+    p.phi = b.phi[i]; p.w = b.w[i];
+  }
+  return p;
+}
+
 // One workgroup per 1024-particle chunk; each lane carries its four
 // particles (i = chunk*1024 + j*256 + lane) one after the other through the
 // kick, the maps and the store, and accumulates the chunk statistics in the
@@ -283,8 +326,8 @@ __device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem& 
 //                  flagged as longitudinally lost;
 //   STAT_APERTURE  x and y over loss == 0 and the count of loss != 0
 //                  (UpdateLoss + UpdateAvgXY, simulation_engine.cu:311-312).
-// One particle in flight per lane keeps the register count low enough for
-// four wavefronts per SIMD.
+// One particle pushed at a time per lane (the next one already loading)
+// keeps the register count down.
 template <int kClass>
 __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
 {
@@ -293,24 +336,20 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
   const uint32_t c = blockIdx.x;
   const BeamArrays& b = a.b;
   KickConst k{};
-  if (a.kick) k = kick_const(a.sc, a.st, a.kick_len, b.mass, b.n);
+  if (a.kick) k = kick_const(a.sc, a.st, a.kick_len);
   const double xa = a.st->x_avg, ya = a.st->y_avg;
   double acc[kScQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   uint32_t cnt = 0;
   double mx = -__builtin_huge_val();
+  // particle j + 1 is loaded before particle j is pushed, so its memory
+  // latency overlaps the maps
+  Particle nxt = load_particle(b, c * kChunk + t);
 #pragma unroll 1
   for (int j = 0; j < (int)kPerLane; ++j) {
     const uint32_t i = c * kChunk + j * kBlock + t;
     const bool in = i < b.n;
-    Particle p;
-    p.loss = 1; p.lloss = 0;
-    p.x = p.xp = p.y = p.yp = p.phi = p.w = 0.0;
-    if (in) {
-      p.loss = b.loss[i];
-      p.lloss = b.lloss[i];
-      p.x = b.x[i]; p.xp = b.xp[i]; p.y = b.y[i]; p.yp = b.yp[i];
-      p.phi = b.phi[i]; p.w = b.w[i];
-    }
+    Particle p = nxt;
+    if (j + 1 < (int)kPerLane) nxt = load_particle(b, i + kBlock);
     const bool alive_in = in && p.loss == 0;
     if (a.kick && alive_in && p.lloss == 0 && (i & a.kick_mask) != 0) kick(p, k, a.fld2, b.mass);
     for (int o = a.op_begin; o < a.op_end; ++o)
@@ -509,12 +548,15 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
   // overlap
   uint32_t good_cnt = 0, num_loss = 0;
   double x_sig_good = 0.0, y_sig_good = 0.0, ke = 0.0, phi_sig_good = 0.0;
+  double x_avg_good = 0.0, y_avg_good = 0.0;
   hps_spch_state ss{};
   if (t == 0) {
     good_cnt = st->num_good;
     num_loss = st->num_loss;
     x_sig_good = st->x_sig_good;
     y_sig_good = st->y_sig_good;
+    x_avg_good = st->x_avg_good;
+    y_avg_good = st->y_avg_good;
     ke = st->w_avg_good;
     phi_sig_good = st->phi_sig_good;
     ss = sc->st;
@@ -582,6 +624,9 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
     sc->ratio_g = ratio_g;
     sc->remesh = rm;
     if (rm) sc->valid = 1;
+    kick_prepare(sc, beta, a.freq, a.current * a.frac, x_avg_good, y_avg_good, x_sig_good,
+                 y_sig_good, ss.dr, ss.dz, ss.nz, ratio_r, ratio_z, ratio_q, ratio_g, b.n,
+                 b.mass);
     remesh = rm;
     nbins = ss.nx * ss.nz;
   }

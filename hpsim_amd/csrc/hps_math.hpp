@@ -181,6 +181,22 @@ HPS_HD double cosh(double x)
   return 0.5 * e + 0.5 / e;
 }
 
+// sinh and cosh of one argument with a single exp: the same two results as
+// sinh(x) and cosh(x) above, bit for bit.
+HPS_HD void sinhcosh(double x, double* shp, double* chp)
+{
+  const double ax = __builtin_fabs(x);
+  if (ax < 1.0) {
+    *shp = hps::sinh(x);
+    *chp = hps::cosh(x);
+    return;
+  }
+  const double e = hps::exp(ax);
+  const double h = 0.5 * e - 0.5 / e;
+  *shp = x < 0.0 ? -h : h;
+  *chp = 0.5 * e + 0.5 / e;
+}
+
 // tan(x) = sin(x) / cos(x) from sincos.
 HPS_HD double tan(double x)
 {
