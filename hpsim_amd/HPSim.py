@@ -464,8 +464,7 @@ class Simulator:
 
     def profile(self):
         """{family: (total ms, launches)} since profiling was enabled."""
-        names = ("segment", "phase", "greens", "deposit", "field", "segment_final",
-                 "phase_final")
+        names = ("segment", "phase", "sc_build", "field", "segment_final", "phase_final")
         ms = (C.c_double * len(names))()
         n = (C.c_uint32 * len(names))()
         check(_lib().hps_sim_get_profile(self._h, ms, n))

@@ -275,7 +275,6 @@ struct hps_spch {
   double* fld1 = nullptr;
   double* fld2 = nullptr;
   int64_t* bins = nullptr;
-  int64_t* part = nullptr;   // [kDepositMaxRows][nr*nz] per-workgroup deposit rows
   size_t max_lds = 0;        // LDS bytes per workgroup on this device
 };
 
@@ -774,10 +773,9 @@ int hps_beamline_index(const hps_beamline* bl, const char* name, uint32_t* out)
 // ---- SpaceCharge ----
 static int spch_alloc_tables(hps_spch* s)
 {
-  (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins); (void)hipFree(s->part);
-  s->fld1 = nullptr; s->fld2 = nullptr; s->bins = nullptr; s->part = nullptr;
+  (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins);
+  s->fld1 = nullptr; s->fld2 = nullptr; s->bins = nullptr;
   const uint32_t nr = s->cfg.nx, nz = s->cfg.nz;
-  HPS_CHECK(hipMalloc(&s->part, (size_t)kDepositMaxRows * nr * nz * 8));
   const size_t n1 = (size_t)(nr + 1) * nr * nz * 2, n2 = (size_t)(nz + 1) * (nr + 1) * 2;
   HPS_CHECK(hipMalloc(&s->fld1, n1 * 8));
   HPS_CHECK(hipMalloc(&s->fld2, n2 * 8));
@@ -835,7 +833,6 @@ int hps_spch_destroy(hps_spch* s)
   if (!s) return 0;
   (void)hipStreamSynchronize(s->stream);
   (void)hipFree(s->dev); (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins);
-  (void)hipFree(s->part);
   delete s;
   return 0;
 }
@@ -1253,7 +1250,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
 
   // The launch list of this call.
   // kinds 0..6 are the profiling families of the header; 7 is a host step
-  enum { L_SEG = 0, L_PHASE, L_GREENS, L_DEPOSIT, L_FIELD, L_SEG_FINAL, L_PHASE_FINAL, L_HOST };
+  enum { L_SEG = 0, L_PHASE, L_BUILD, L_FIELD, L_SEG_FINAL, L_PHASE_FINAL, L_HOST };
   struct Launch {
     int kind;
     SegArgs seg;
@@ -1308,9 +1305,8 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       Launch g = l;
       g.kind = L_PHASE_FINAL;
       launches.push_back(g);
-      g.kind = L_GREENS;
+      g.kind = L_BUILD;
       launches.push_back(g);
-      if (deposit_ok) { g.kind = L_DEPOSIT; launches.push_back(g); }
       g.kind = L_FIELD;
       launches.push_back(g);
     }
@@ -1322,9 +1318,9 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       case L_SEG_FINAL: return launch_seg_final(l.seg, st);
       case L_PHASE: return launch_phase(l.ph, st);
       case L_PHASE_FINAL: return launch_phase_final(l.ph, st);
-      case L_GREENS: return launch_greens(sc->dev, sc->fld1, sc->cfg.nx, sc->cfg.nz, b->n, b->mass, st);
-      case L_DEPOSIT:
-        return launch_deposit(b->d, b->st, sc->dev, sc->part, sc->bins, sc->cfg.nx, sc->cfg.nz, st);
+      case L_BUILD:
+        return launch_sc_build(sc->dev, sc->fld1, sc->cfg.nx, sc->cfg.nz, b->d, b->st, sc->bins,
+                               deposit_ok, st);
       default:
         return launch_field(sc->dev, sc->bins, sc->fld1, sc->fld2, sc->cfg.nx, sc->cfg.nz,
                             sc->max_lds, st);
@@ -1378,7 +1374,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
     put(c.ops.data(), c.ops.size() * sizeof(Op));
     if (m->sc) {
       const hps_spch* sc = m->sc;
-      const void* ptrs[5] = {sc->dev, sc->fld1, sc->fld2, sc->bins, sc->part};
+      const void* ptrs[4] = {sc->dev, sc->fld1, sc->fld2, sc->bins};
       put(ptrs, sizeof ptrs);
       put(&sc->cfg.nx, sizeof sc->cfg.nx);
       put(&sc->cfg.nz, sizeof sc->cfg.nz);

@@ -80,19 +80,18 @@ hipError_t launch_segment(const SegArgs& a, hipStream_t s);
 hipError_t launch_seg_final(const SegArgs& a, hipStream_t s);
 hipError_t launch_phase(const PhaseArgs& a, hipStream_t s);
 hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s);
-hipError_t launch_greens(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, uint32_t num,
-                         double mass, hipStream_t s);
-// k_deposit workgroup size (four particles per thread at full grid) and the
-// LDS it uses besides its histogram.
+// k_sc_build workgroup size (deposit: four particles per thread at full
+// grid), the most deposit workgroups, and the LDS a deposit workgroup uses
+// besides its histogram.
 constexpr uint32_t kDepositBlock = 1024;
+constexpr uint32_t kDepositMaxBlocks = 256;
 constexpr size_t kBlockLdsReserve = 4 * kDepositBlock;
-// k_deposit workgroups, each writing one row of partial histograms.
-constexpr uint32_t kDepositMaxRows = 256;
-uint32_t deposit_rows(uint32_t n);
-hipError_t launch_deposit(BeamArrays b, const hps_beam_stats* st, ScDev* sc, int64_t* part,
-                          int64_t* bins, uint32_t nr, uint32_t nz, hipStream_t s);
-// max_lds: LDS bytes a workgroup may use; the field kernel stages its
-// inputs there when they fit.
+uint32_t deposit_blocks(uint32_t n);
+// Green's table + charge deposit (deposit = false for beams whose reference
+// deposit launch is empty).
+hipError_t launch_sc_build(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, BeamArrays b,
+                           const hps_beam_stats* st, int64_t* bins, bool deposit,
+                           hipStream_t s);
 hipError_t launch_field(const ScDev* sc, const int64_t* bins, const double* fld1, double* fld2,
                         uint32_t nr, uint32_t nz, size_t max_lds, hipStream_t s);
 

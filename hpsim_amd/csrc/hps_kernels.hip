@@ -302,6 +302,20 @@ __device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem& 
   }
 }
 
+// Particles of one lane in flight at a time in k_segment, per map class.
+#ifndef HPS_SEG_GROUP_GAP
+#define HPS_SEG_GROUP_GAP 1
+#endif
+#ifndef HPS_SEG_GROUP_QUAD
+#define HPS_SEG_GROUP_QUAD 1
+#endif
+#ifndef HPS_SEG_GROUP_LIGHT
+#define HPS_SEG_GROUP_LIGHT 1
+#endif
+constexpr int kSegGroupGap = HPS_SEG_GROUP_GAP;
+constexpr int kSegGroupQuad = HPS_SEG_GROUP_QUAD;
+constexpr int kSegGroupLight = HPS_SEG_GROUP_LIGHT;
+
 // Particle i of the beam, or an excluded placeholder (loss = 1) past the end.
 __device__ __forceinline__ Particle load_particle(const BeamArrays& b, uint32_t i)
 {
@@ -318,19 +332,20 @@ __device__ __forceinline__ Particle load_particle(const BeamArrays& b, uint32_t 
 }
 
 // One workgroup per 1024-particle chunk; each lane carries its four
-// particles (i = chunk*1024 + j*256 + lane) one after the other through the
-// kick, the maps and the store, and accumulates the chunk statistics in the
+// particles (i = chunk*1024 + j*256 + lane) through the kick, the maps and
+// the store in groups of kGroup, and accumulates the chunk statistics in the
 // canonical lane order (j = 0..3, from +0.0):
 //   STAT_SC        0 phi, 1 x, 2 x^2, 3 y, 4 y^2, 5 w over (loss + lloss) == 0,
 //                  the good count and the largest phi among particles not yet
 //                  flagged as longitudinally lost;
 //   STAT_APERTURE  x and y over loss == 0 and the count of loss != 0
 //                  (UpdateLoss + UpdateAvgXY, simulation_engine.cu:311-312).
-// One particle pushed at a time per lane (the next one already loading)
-// keeps the register count down.
-template <int kClass>
+// The group size trades registers (waves per SIMD) against independent
+// chains per wave.
+template <int kClass, int kGroup>
 __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
 {
+  static_assert(kPerLane % kGroup == 0, "group must divide the particles per lane");
   __shared__ RedSmem sm;
   const int t = threadIdx.x;
   const uint32_t c = blockIdx.x;
@@ -341,41 +356,62 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
   double acc[kScQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   uint32_t cnt = 0;
   double mx = -__builtin_huge_val();
-  // particle j + 1 is loaded before particle j is pushed, so its memory
-  // latency overlaps the maps
-  Particle nxt = load_particle(b, c * kChunk + t);
 #pragma unroll 1
-  for (int j = 0; j < (int)kPerLane; ++j) {
-    const uint32_t i = c * kChunk + j * kBlock + t;
-    const bool in = i < b.n;
-    Particle p = nxt;
-    if (j + 1 < (int)kPerLane) nxt = load_particle(b, i + kBlock);
-    const bool alive_in = in && p.loss == 0;
-    if (a.kick && alive_in && p.lloss == 0 && (i & a.kick_mask) != 0) kick(p, k, a.fld2, b.mass);
-    for (int o = a.op_begin; o < a.op_end; ++o)
-      if (p.loss == 0) apply_op<kClass>(a.ops[o], p, i, b.mass, b.charge, xa, ya);
-    if (alive_in) {
-      b.x[i] = p.x; b.xp[i] = p.xp; b.y[i] = p.y; b.yp[i] = p.yp;
-      b.phi[i] = p.phi; b.w[i] = p.w;
-      if (p.loss != 0) b.loss[i] = p.loss;
+  for (int j0 = 0; j0 < (int)kPerLane; j0 += kGroup) {
+    // kGroup particles of this lane in flight: the maps of one element run
+    // over the whole group, giving the scheduler independent chains
+    Particle p[kGroup];
+    bool alive_in[kGroup];
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) {
+      const uint32_t i = c * kChunk + (j0 + g) * kBlock + t;
+      p[g] = load_particle(b, i);
+      alive_in[g] = i < b.n && p[g].loss == 0;
     }
-    if (a.stats == STAT_SC) {
-      if (in && p.lloss == 0) mx = ::fmax(mx, p.phi);
-      const bool g = in && is_good(p.loss, p.lloss);
-      const double vphi = g ? p.phi : 0.0, vx = g ? p.x : 0.0, vy = g ? p.y : 0.0,
-                   vw = g ? p.w : 0.0;
-      acc[0] = acc[0] + vphi;
-      acc[1] = acc[1] + vx;
-      acc[2] = acc[2] + vx * vx;
-      acc[3] = acc[3] + vy;
-      acc[4] = acc[4] + vy * vy;
-      acc[5] = acc[5] + vw;
-      cnt += g ? 1u : 0u;
-    } else if (a.stats == STAT_APERTURE) {
-      const bool ok = in && p.loss == 0;
-      acc[0] = acc[0] + (ok ? p.x : 0.0);
-      acc[1] = acc[1] + (ok ? p.y : 0.0);
-      cnt += (in && p.loss != 0) ? 1u : 0u;
+    if (a.kick) {
+#pragma unroll
+      for (int g = 0; g < kGroup; ++g) {
+        const uint32_t i = c * kChunk + (j0 + g) * kBlock + t;
+        if (alive_in[g] && p[g].lloss == 0 && (i & a.kick_mask) != 0)
+          kick(p[g], k, a.fld2, b.mass);
+      }
+    }
+    for (int o = a.op_begin; o < a.op_end; ++o) {
+      const Op& op = a.ops[o];
+#pragma unroll
+      for (int g = 0; g < kGroup; ++g) {
+        const uint32_t i = c * kChunk + (j0 + g) * kBlock + t;
+        if (p[g].loss == 0) apply_op<kClass>(op, p[g], i, b.mass, b.charge, xa, ya);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) {
+      const uint32_t i = c * kChunk + (j0 + g) * kBlock + t;
+      const bool in = i < b.n;
+      const Particle& q = p[g];
+      if (alive_in[g]) {
+        b.x[i] = q.x; b.xp[i] = q.xp; b.y[i] = q.y; b.yp[i] = q.yp;
+        b.phi[i] = q.phi; b.w[i] = q.w;
+        if (q.loss != 0) b.loss[i] = q.loss;
+      }
+      if (a.stats == STAT_SC) {
+        if (in && q.lloss == 0) mx = ::fmax(mx, q.phi);
+        const bool gd = in && is_good(q.loss, q.lloss);
+        const double vphi = gd ? q.phi : 0.0, vx = gd ? q.x : 0.0, vy = gd ? q.y : 0.0,
+                     vw = gd ? q.w : 0.0;
+        acc[0] = acc[0] + vphi;
+        acc[1] = acc[1] + vx;
+        acc[2] = acc[2] + vx * vx;
+        acc[3] = acc[3] + vy;
+        acc[4] = acc[4] + vy * vy;
+        acc[5] = acc[5] + vw;
+        cnt += gd ? 1u : 0u;
+      } else if (a.stats == STAT_APERTURE) {
+        const bool ok = in && q.loss == 0;
+        acc[0] = acc[0] + (ok ? q.x : 0.0);
+        acc[1] = acc[1] + (ok ? q.y : 0.0);
+        cnt += (in && q.loss != 0) ? 1u : 0u;
+      }
     }
   }
   const Partials& pr = a.pr;
@@ -637,14 +673,14 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
 
 // --- SCHEFF rebuild ----------------------------------------------------------------
 
-// Ring Green's table, UpdateFldTbl1Kernel (space_charge_kernel.cu:38-100),
-// stored [rp][dz][rs] so that the field sum reads consecutive rs.
-__global__ __launch_bounds__(kBlock) void k_greens(const ScDev* sc, double* fld1, uint32_t nr,
-                                                   uint32_t nz, uint32_t r_num_part, double mass)
+// One entry of the ring Green's table, UpdateFldTbl1Kernel
+// (space_charge_kernel.cu:38-100), stored [rp][dz][rs] so that the field sum
+// reads consecutive rs.
+__device__ __forceinline__ void greens_entry(const ScDev* sc, double* fld1, uint32_t nr,
+                                             uint32_t nz, uint32_t r_num_part, double mass,
+                                             uint32_t idx)
 {
-  if (!sc->remesh) return;
   const uint32_t total = (nr + 1) * nz * nr;
-  const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx >= total) return;
   const uint32_t k = idx % nr;            // source ring index rs
   const uint32_t t = (idx / nr) % nz;     // dz
@@ -680,16 +716,14 @@ __global__ __launch_bounds__(kBlock) void k_greens(const ScDev* sc, double* fld1
 }
 
 // Charge deposit, DistributeParticleKernel (space_charge_kernel.cu:122-224),
-// into an LDS-private int64 fixed-point histogram per workgroup, stored as
-// that workgroup's row of part[gridDim.x][nb]; k_bins_reduce adds the rows.
-// Integer sums make the result independent of the order.
-__global__ __launch_bounds__(kDepositBlock) void k_deposit(BeamArrays b,
-                                                           const hps_beam_stats* st, ScDev* sc,
-                                                           int64_t* part)
+// of workgroup `blk` of `nblk` into an LDS-private int64 fixed-point
+// histogram, merged into bins with 64-bit integer atomics: integer sums make
+// the result independent of the order.
+__device__ __forceinline__ void deposit_block(const BeamArrays& b, const hps_beam_stats* st,
+                                              ScDev* sc, int64_t* bins, uint32_t blk,
+                                              uint32_t nblk, unsigned long long* hist,
+                                              uint32_t* ured)
 {
-  extern __shared__ unsigned long long hist[];
-  __shared__ uint32_t ured[kDepositBlock];
-  if (!sc->remesh) return;
   const int t = threadIdx.x;
   const uint32_t nx = sc->st.nx, nz = sc->st.nz, nb = nx * nz;
   for (uint32_t i = t; i < nb; i += kDepositBlock) hist[i] = 0ull;
@@ -707,7 +741,7 @@ __global__ __launch_bounds__(kDepositBlock) void k_deposit(BeamArrays b,
   const double c2 = r_beta * gm * wave_len / TWOPI;
   const double drdr = dr * dr;
   uint32_t count = 0;
-  for (uint32_t np = blockIdx.x * kDepositBlock + t; np < b.n; np += gridDim.x * kDepositBlock) {
+  for (uint32_t np = blk * kDepositBlock + t; np < b.n; np += nblk * kDepositBlock) {
     if (!(b.loss[np] == 0 && b.lloss[np] == 0)) continue;
     const double x = b.x[np], y = b.y[np];
     double z = relative_phi(b.phi[np], ref);
@@ -757,8 +791,10 @@ __global__ __launch_bounds__(kDepositBlock) void k_deposit(BeamArrays b,
     ++count;
   }
   __syncthreads();
-  int64_t* row = part + (size_t)blockIdx.x * nb;
-  for (uint32_t i = t; i < nb; i += kDepositBlock) row[i] = (int64_t)hist[i];
+  for (uint32_t i = t; i < nb; i += kDepositBlock) {
+    const unsigned long long v = hist[i];
+    if (v != 0ull) atomicAdd((unsigned long long*)&bins[i], v);
+  }
   ured[t] = count;
   __syncthreads();
   for (int o = kDepositBlock / 2; o >= 1; o >>= 1) {
@@ -766,6 +802,28 @@ __global__ __launch_bounds__(kDepositBlock) void k_deposit(BeamArrays b,
     __syncthreads();
   }
   if (t == 0 && ured[0]) atomicAdd(&sc->st.num_on_mesh, ured[0]);
+}
+
+// The SCHEFF rebuild after a remesh decision, one launch: workgroups
+// [0, greens_blocks) fill the Green's table (one entry per thread), the
+// next dep_blocks workgroups deposit the charge.  Both read only the control
+// step's results, so they run side by side; all exit when there is no
+// remesh.
+__global__ __launch_bounds__(kDepositBlock) void k_sc_build(ScDev* sc, double* fld1, uint32_t nr,
+                                                            uint32_t nz, BeamArrays b,
+                                                            const hps_beam_stats* st,
+                                                            int64_t* bins,
+                                                            uint32_t greens_blocks,
+                                                            uint32_t dep_blocks)
+{
+  extern __shared__ unsigned long long hist[];
+  __shared__ uint32_t ured[kDepositBlock];
+  if (!sc->remesh) return;
+  if (blockIdx.x < greens_blocks) {
+    greens_entry(sc, fld1, nr, nz, b.n, b.mass, blockIdx.x * kDepositBlock + threadIdx.x);
+    return;
+  }
+  deposit_block(b, st, sc, bins, blockIdx.x - greens_blocks, dep_blocks, hist, ured);
 }
 
 // Field table E(zp, rp) = sum over bins of charge * Green's value,
@@ -808,33 +866,6 @@ __global__ __launch_bounds__(kBlock) void k_field(const ScDev* sc, const int64_t
   if (lane == 0) {
     fld2[2 * (size_t)wave] = tx;
     fld2[2 * (size_t)wave + 1] = ty;
-  }
-}
-
-// bins[i] = sum over the k_deposit workgroups' rows of part[.][i]: a
-// workgroup takes 16 bins, its 16 row slices each add every 16th row, and
-// the slices are combined in LDS (integer adds, any order).
-__global__ __launch_bounds__(kBlock) void k_bins_reduce(const ScDev* sc, const int64_t* part,
-                                                        int64_t* bins, uint32_t nrows,
-                                                        uint32_t nb)
-{
-  __shared__ int64_t acc[kBlock];
-  if (!sc->remesh) return;
-  const int t = threadIdx.x;
-  const uint32_t bin = blockIdx.x * 16 + (t & 15);
-  const uint32_t slice = t >> 4;
-  int64_t v = 0;
-  if (bin < nb) {
-#pragma unroll 4
-    for (uint32_t r = slice; r < nrows; r += 16) v += part[(size_t)r * nb + bin];
-  }
-  acc[t] = v;
-  __syncthreads();
-  if (t < 16 && bin < nb) {
-    int64_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) sum += acc[t + 16 * k];
-    bins[bin] = sum;
   }
 }
 
@@ -1042,11 +1073,11 @@ hipError_t launch_segment(const SegArgs& a, hipStream_t s)
 {
   const uint32_t nch = (a.b.n + kChunk - 1) / kChunk;
   if (a.op_class >= OPC_GAP)
-    hipLaunchKernelGGL(k_segment<OPC_GAP>, dim3(nch), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_segment<OPC_GAP, kSegGroupGap>), dim3(nch), dim3(kBlock), 0, s, a);
   else if (a.op_class == OPC_QUAD)
-    hipLaunchKernelGGL(k_segment<OPC_QUAD>, dim3(nch), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_segment<OPC_QUAD, kSegGroupQuad>), dim3(nch), dim3(kBlock), 0, s, a);
   else
-    hipLaunchKernelGGL(k_segment<OPC_LIGHT>, dim3(nch), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_segment<OPC_LIGHT, kSegGroupLight>), dim3(nch), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1069,32 +1100,23 @@ hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_greens(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, uint32_t num,
-                         double mass, hipStream_t s)
-{
-  const uint32_t total = (nr + 1) * nz * nr;
-  hipLaunchKernelGGL(k_greens, dim3(blocks_for(total)), dim3(kBlock), 0, s, sc, fld1, nr, nz,
-                     num, mass);
-  return hipGetLastError();
-}
-
-uint32_t deposit_rows(uint32_t n)
+uint32_t deposit_blocks(uint32_t n)
 {
   uint32_t grid = (n + 4 * kDepositBlock - 1) / (4 * kDepositBlock);
   if (grid < 1) grid = 1;
-  if (grid > kDepositMaxRows) grid = kDepositMaxRows;
+  if (grid > kDepositMaxBlocks) grid = kDepositMaxBlocks;
   return grid;
 }
 
-hipError_t launch_deposit(BeamArrays b, const hps_beam_stats* st, ScDev* sc, int64_t* part,
-                          int64_t* bins, uint32_t nr, uint32_t nz, hipStream_t s)
+hipError_t launch_sc_build(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, BeamArrays b,
+                           const hps_beam_stats* st, int64_t* bins, bool deposit,
+                           hipStream_t s)
 {
-  const uint32_t nb = nr * nz;
-  const size_t lds = (size_t)nb * sizeof(unsigned long long);
-  const uint32_t rows = deposit_rows(b.n);
-  hipLaunchKernelGGL(k_deposit, dim3(rows), dim3(kDepositBlock), lds, s, b, st, sc, part);
-  hipLaunchKernelGGL(k_bins_reduce, dim3((nb + 15) / 16), dim3(kBlock), 0, s, sc, part, bins,
-                     rows, nb);
+  const uint32_t greens_blocks = ((nr + 1) * nz * nr + kDepositBlock - 1) / kDepositBlock;
+  const uint32_t dep = deposit ? deposit_blocks(b.n) : 0;
+  const size_t lds = (size_t)nr * nz * sizeof(unsigned long long);
+  hipLaunchKernelGGL(k_sc_build, dim3(greens_blocks + dep), dim3(kDepositBlock), lds, s, sc,
+                     fld1, nr, nz, b, st, bins, greens_blocks, dep);
   return hipGetLastError();
 }
 

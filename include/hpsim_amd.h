@@ -199,13 +199,13 @@ int hps_sim_reset(hps_sim* m);
 int hps_sim_set_space_charge(hps_sim* m, int on);
 /* Per-kernel-family device timing with HIP events around every launch of
  * later simulate() calls (families: 0 fused element maps + kick + per-chunk
- * moments, 1 per-chunk relative-phase moments, 2 Green's table, 3 charge
- * deposit, 4 field sum, 5 cross-chunk moments and longitudinal loss, 6
+ * moments, 1 per-chunk relative-phase moments, 2 Green's table + charge
+ * deposit, 3 field sum, 4 cross-chunk moments and longitudinal loss, 5
  * relative-phase moments + SCHEFF control).  ms and launches hold
  * HPS_PROFILE_FAMILIES entries.  Enabling resets the counters.  No reference
  * counterpart: the reference only times whole simulate() calls
  * (timer.cpp:4-19). */
-#define HPS_PROFILE_FAMILIES 7
+#define HPS_PROFILE_FAMILIES 6
 int hps_sim_set_profiling(hps_sim* m, int on);
 int hps_sim_get_profile(hps_sim* m, double* ms, uint32_t* launches);
 /* Launch the kernel sequence of simulate() as a HIP graph (default on),
