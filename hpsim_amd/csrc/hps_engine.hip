@@ -248,6 +248,7 @@ void free_beam(hps_beam* b)
   (void)hipFree(b->pr.sums);
   (void)hipFree(b->pr.cnt);
   (void)hipFree(b->pr.pmax);
+  (void)hipFree(b->pr.rmax2);
   (void)hipFree(b->pr.out);
   (void)hipFree(b->pr.out_cnt);
   for (int k = 0; k < 2; ++k) {
@@ -346,6 +347,8 @@ int hps_beam_create(hps_beam** out, uint32_t num, double mass, double charge, do
        hipMemset(b->pr.cnt, 0, 3 * (size_t)b->pr.mpad * 4) == hipSuccess;
   ok = ok && hipMalloc(&b->pr.pmax, (size_t)b->pr.mpad * 8) == hipSuccess &&
        hipMemset(b->pr.pmax, 0, (size_t)b->pr.mpad * 8) == hipSuccess;
+  ok = ok && hipMalloc(&b->pr.rmax2, (size_t)b->pr.mpad * 8) == hipSuccess &&
+       hipMemset(b->pr.rmax2, 0, (size_t)b->pr.mpad * 8) == hipSuccess;
   ok = ok && hipMalloc(&b->pr.out, kMaxQ * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc(&b->pr.out_cnt, 4 * sizeof(uint32_t)) == hipSuccess;
   if (!ok) { free_beam(b); delete b; return fail("device allocation failed for the beam"); }
@@ -771,6 +774,10 @@ int hps_beamline_index(const hps_beamline* bl, const char* name, uint32_t* out)
 }
 
 // ---- SpaceCharge ----
+// Bound on adjacent bunches (each image is one more ring per Green's-table
+// entry and one more point charge per off-mesh kick).
+constexpr uint32_t kMaxAdjBunch = 1024;
+
 static int spch_alloc_tables(hps_spch* s)
 {
   (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins);
@@ -803,6 +810,7 @@ int hps_spch_create(hps_spch** out, uint32_t nr, uint32_t nz, double interval, u
 {
   if (!out) return fail("null output handle");
   if (nr == 0 || nz == 0) return fail("SpaceCharge needs nr > 0 and nz > 0");
+  if (adj_bunch > kMaxAdjBunch) return fail("adj_bunch is limited to 1024 bunches each side");
   hps_spch* s = new hps_spch();
   if (hipGetDevice(&s->device) != hipSuccess || stream_for(s->device, &s->stream)) {
     delete s;
@@ -843,7 +851,11 @@ int hps_spch_set(hps_spch* s, int what, double v)
   switch (what) {
     case 0: s->cfg.interval = v; s->cfg.interval_init = v; break;
     case 1: s->cfg.remesh_val = v; break;
-    case 2: s->cfg.adj_bunch = (uint32_t)v; break;
+    case 2:
+      if (!(v >= 0.0 && v <= (double)kMaxAdjBunch))
+        return fail("adj_bunch must lie in [0, 1024]");
+      s->cfg.adj_bunch = (uint32_t)v;
+      break;
     case 3: s->cfg.adj_bunch_w = v; break;
     case 4: s->cfg.mesh_w = v; break;
     default: return fail("unknown space-charge setting");
@@ -1214,7 +1226,6 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   const int32_t end_index = end < 0 ? (int32_t)nel - 1 : end;
   if (end_index >= (int32_t)nel || start_index >= (int32_t)nel) return fail("element index out of range");
   if (m->sc_on && m->sc) {
-    if (m->sc->cfg.adj_bunch != 0) return fail("adjacent-bunch space charge (adj_bunch > 0) is not implemented");
     if (m->sc->cfg.mesh_w != 0.0) return fail("mesh_size_cutoff_w is not implemented");
     // the deposit keeps one int64 histogram of nr * nz bins per workgroup in LDS
     if ((size_t)m->sc->cfg.nx * m->sc->cfg.nz * 8 + kBlockLdsReserve > m->sc->max_lds)
@@ -1284,6 +1295,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       a.sc = m->sc ? m->sc->dev : nullptr;
       a.fld2 = m->sc ? m->sc->fld2 : nullptr;
       a.stats = st.stats;
+      a.rmax = (st.stats == STAT_SC && m->sc && m->sc->cfg.adj_bunch != 0) ? 1 : 0;
       a.op_class = OPC_LIGHT;
       for (int32_t o = st.op_begin; o < st.op_end; ++o)
         a.op_class = std::max(a.op_class, op_class_of(c.ops[o].kind));
@@ -1300,6 +1312,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       PhaseArgs& p = l.ph;
       p.b = b->d; p.st = b->st; p.pr = b->pr; p.sc = sc->dev; p.bins = sc->bins;
       p.do_sums = b->n > 2 ? 1 : 0;
+      p.rmax = sc->cfg.adj_bunch == 0 ? 0 : (c.sc_stats ? 1 : 2);
       p.frac = st.frac; p.current = b->current; p.freq = b->freq;
       if (p.do_sums) launches.push_back(l);
       Launch g = l;

@@ -31,6 +31,9 @@ struct ScDev {
   // that the control step fixes, computed once per kick point
   double k_xbar, k_ybar, k_epsq, k_gmg, k_dr, k_dzg, k_hl, k_c1, k_c2, k_bgmg;
   double k_xfac, k_yfac, k_er_ratio, k_ez_ratio;
+  double k_pl;          // bunch spacing beta*gamma*lambda (adjacent-bunch images)
+  uint32_t adj_eff;     // adjacent bunches each side at this kick point
+  uint32_t pad_;
 };
 
 // Scratch for the canonical reductions: per-chunk partial sums, padded to a
@@ -39,6 +42,7 @@ struct Partials {
   double* sums;        // [kMaxQ][mpad]
   uint32_t* cnt;       // [3][mpad]
   double* pmax;        // [mpad]
+  double* rmax2;       // [mpad] largest r^2 about (x_avg, y_avg), loss == 0
   double* out;         // [kMaxQ] final sums for host-side getters
   uint32_t* out_cnt;   // [1]
   uint32_t mpad;       // NextPow2(nchunks)
@@ -61,6 +65,7 @@ struct SegArgs {
   const double* fld2;      // (nz+1)*(nr+1) double2, [zp][rp]
   int32_t stats;           // StatKind computed after the maps
   int32_t op_class;        // OpClass: heaviest map in [op_begin, op_end)
+  int32_t rmax;            // STAT_SC also keeps the per-chunk largest r^2
 };
 
 struct PhaseArgs {
@@ -70,6 +75,7 @@ struct PhaseArgs {
   ScDev* sc;
   int64_t* bins;
   int32_t do_sums;         // n > 2 (UpdateStatForSpaceCharge guard)
+  int32_t rmax;            // the segment kept per-chunk r^2 maxima (adj_bunch > 0)
   double frac;             // current fraction at this kick point
   double current;          // beam current (A)
   double freq;             // beam frequency (MHz)

@@ -10,7 +10,8 @@
 //   k_phase    per-chunk relative-phase moments; k_phase_final finishes them
 //              and runs the Scheff::Start control logic on the device (no
 //              host round trip per kick).
-//   k_greens / k_deposit / k_field   the SCHEFF rebuild, run only when the
+//   k_sc_build / k_field   the SCHEFF rebuild (Green's table and charge
+//              deposit in one launch, then the field sum), run only when the
 //              control step asks for a remesh (flag in device memory).
 // Reductions follow one canonical order (see kChunk): lane sums of 4
 // particles strided by 256, halving tree over 256 lanes, halving tree over
@@ -63,6 +64,28 @@ __device__ __forceinline__ void block_reduce(double (&v)[Q], uint32_t& cnt, doub
     }
   }
   __syncthreads();
+}
+
+// Largest of the 256 lane values (valid in thread 0).  The comparison
+// a < v keeps a when v is NaN; a maximum does not depend on the order.
+// buf holds kBlock / 64 doubles.
+__device__ __forceinline__ double block_max(double v, double* buf, int t)
+{
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double u = __shfl_down(v, o, 64);
+    v = v < u ? u : v;
+  }
+  if ((t & 63) == 0) buf[t >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (t == 0) {
+    r = buf[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) r = r < buf[w] ? buf[w] : r;
+  }
+  __syncthreads();
+  return r;
 }
 
 // Cross-chunk trees of Q quantities stored P[q * mpad + c] (zero padded to
@@ -128,8 +151,8 @@ __device__ __forceinline__ bool is_good(uint32_t loss, uint32_t lloss)
 
 struct KickConst {
   double phimc, xbar, ybar, epsq, gmg, dr, dzg, hl, c2, c3, xfac, yfac;
-  double er_ratio, ez_ratio, cbgcoef, r_beta, phi_ref;
-  uint32_t nx, nz;
+  double er_ratio, ez_ratio, cbgcoef, r_beta, phi_ref, pl;
+  uint32_t nx, nz, adj;
 };
 
 // The kick constants of ApplySpchKickKernel (space_charge_kernel.cu:578-640):
@@ -159,6 +182,8 @@ __device__ __forceinline__ KickConst kick_const(const ScDev* sc, const hps_beam_
   k.cbgcoef = sc->k_c1 * k.c3 * PI * 0.5 * sc->st.num_on_mesh;
   k.r_beta = sc->beta;
   k.phi_ref = st->phi_avg_good;
+  k.pl = sc->k_pl;
+  k.adj = sc->adj_eff;
   return k;
 }
 
@@ -183,6 +208,7 @@ __device__ __forceinline__ void kick_prepare(ScDev* sc, double r_beta, double r_
   sc->k_hl = nz * 0.5 * dzg;
   sc->k_c1 = 5721.67 * r_current / (r_freq * 1e6 * (double)r_num_part * r_mass);
   sc->k_c2 = r_beta * gmg * wave_len / TWOPI;
+  sc->k_pl = r_beta * gmg * wave_len;
   sc->k_bgmg = r_beta * gmg;
   sc->k_xfac = xfac;
   sc->k_yfac = epsq * xfac;
@@ -216,8 +242,20 @@ __device__ __forceinline__ void kick(Particle& p, const KickConst& k, const doub
     z = -k.c2 * (phi - k.phimc);
     const double d = ::sqrt(z * z + r * r);
     const double invd3 = 1.0 / (d * d * d);
-    const double rod3 = r * invd3;
-    const double zod3 = z * invd3;
+    double rod3 = r * invd3;
+    double zod3 = z * invd3;
+    // point charges of the neighbouring bunches (space_charge_kernel.cu:650-662)
+    for (uint32_t nb = 1; nb <= k.adj; ++nb) {
+      int xi = (int)nb;
+      for (int lr = 0; lr < 2; ++lr) {
+        const double sz = z + xi * k.pl;
+        const double dd = ::sqrt(sz * sz + r * r);
+        const double iv = 1.0 / (dd * dd * dd);
+        rod3 += r * iv;
+        zod3 += sz * iv;
+        xi = -xi;
+      }
+    }
     cbgr = k.cbgcoef * rod3;
     cbgzs = k.cbgcoef * zod3;
   } else {
@@ -356,6 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
   double acc[kScQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   uint32_t cnt = 0;
   double mx = -__builtin_huge_val();
+  double r2m = 0.0;
 #pragma unroll 1
   for (int j0 = 0; j0 < (int)kPerLane; j0 += kGroup) {
     // kGroup particles of this lane in flight: the maps of one element run
@@ -406,6 +445,12 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
         acc[4] = acc[4] + vy * vy;
         acc[5] = acc[5] + vw;
         cnt += gd ? 1u : 0u;
+        if (a.rmax && in && q.loss == 0) {
+          // MaxR2PhiReduceKernel's first pass (beam_kernel.cu:826-843)
+          const double tx = q.x - xa, ty = q.y - ya;
+          const double r2 = tx * tx + ty * ty;
+          r2m = r2m < r2 ? r2 : r2m;
+        }
       } else if (a.stats == STAT_APERTURE) {
         const bool ok = in && q.loss == 0;
         acc[0] = acc[0] + (ok ? q.x : 0.0);
@@ -422,6 +467,10 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
       for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = acc[q];
       pr.cnt[c] = cnt;
       pr.pmax[c] = mx;
+    }
+    if (a.rmax) {
+      const double m = block_max(r2m, sm.mred, t);
+      if (t == 0) pr.rmax2[c] = m;
     }
   } else if (a.stats == STAT_APERTURE) {
     double ap[2] = {acc[0], acc[1]};
@@ -567,8 +616,8 @@ __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
 }
 
 // One workgroup after k_phase: the phi_r moments, then the Scheff::Start
-// control logic of space_charge.cpp:84-138 (adj_bunch = 0,
-// mesh_size_cutoff_w = 0) on the device, and the cleared charge histogram
+// control logic of space_charge.cpp:84-138 (mesh_size_cutoff_w = 0) on the
+// device, and the cleared charge histogram
 // when it decides to remesh.  Thread 0 reads every input into registers
 // before it writes any output.
 __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
@@ -603,6 +652,27 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
     final_reduce<3>(a.pr.sums, nullptr, a.pr.mpad, a.pr.nch, sm.red, sm.ured, sm.mred, t, s,
                     dummy);
   }
+  // Beam::UpdateMaxRPhi (beam.cu:829-851): the largest r^2 about the last
+  // all-particle centroid over loss == 0, for the adjacent-bunch mesh; from
+  // the segment's chunk maxima, or directly for beams of at most two
+  // particles (no segment statistics).
+  double r2max = 0.0;
+  if (a.rmax == 1) {
+    double v = 0.0;
+    for (uint32_t c = t; c < a.pr.nch; c += kBlock) {
+      const double u = a.pr.rmax2[c];
+      v = v < u ? u : v;
+    }
+    r2max = block_max(v, sm.mred, t);
+  } else if (a.rmax == 2 && t == 0) {
+    const double xa = st->x_avg, ya = st->y_avg;
+    for (uint32_t i = 0; i < b.n; ++i) {
+      if (b.loss[i] != 0) continue;
+      const double tx = b.x[i] - xa, ty = b.y[i] - ya;
+      const double r2 = tx * tx + ty * ty;
+      r2max = r2max < r2 ? r2 : r2max;
+    }
+  }
   if (t == 0) {
     if (a.do_sums) {
       const double gd = (double)good_cnt;
@@ -623,6 +693,8 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
     const double cratioz = sigz * sigz * gamma * gamma;
     const double cratior_old = ss.sigr_old * ss.sigz_old;
     const double cratioz_old = ss.sigz_old * ss.sigz_old * ss.gm_old * ss.gm_old;
+    // adjacent bunches below the cutoff energy only (space_charge.cpp:102-105)
+    const uint32_t adj = (ss.adj_bunch_w == 0.0 || ke < ss.adj_bunch_w) ? ss.adj_bunch : 0u;
     double ratio_r = 1.0, ratio_z = 1.0, ratio_q = 1.0, ratio_g = 1.0;
     int rm = 0;
     ss.kick_count += 1;
@@ -638,8 +710,15 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
       // compares the device pointers x_sig_good > y_sig_good (space_charge.cu:131);
       // x_sig_good is allocated first (beam.cu:51,54), so sigma_y is used.
       const double bl = beta * CLIGHT / a.freq;
-      ss.dr = 3.0 * y_sig_good / ss.nx;
-      ss.dz = 3.0 * bl * phi_sig_good / (ss.nz * PI);
+      if (adj != 0) {
+        // with adjacent bunches the mesh spans the largest radius and one
+        // RF period (UpdateScheffMeshSizeKernel with r_coef = 1)
+        ss.dr = 1.0 * ::sqrt(r2max) / ss.nx;
+        ss.dz = bl / ss.nz;
+      } else {
+        ss.dr = 3.0 * y_sig_good / ss.nx;
+        ss.dz = 3.0 * bl * phi_sig_good / (ss.nz * PI);
+      }
       ss.num_on_mesh = 0;
       rm = 1;
     } else {
@@ -659,6 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
     sc->ratio_q = ratio_q;
     sc->ratio_g = ratio_g;
     sc->remesh = rm;
+    sc->adj_eff = adj;
     if (rm) sc->valid = 1;
     kick_prepare(sc, beta, a.freq, a.current * a.frac, x_avg_good, y_avg_good, x_sig_good,
                  y_sig_good, ss.dr, ss.dz, ss.nz, ratio_r, ratio_z, ratio_q, ratio_g, b.n,
@@ -695,21 +775,45 @@ __device__ __forceinline__ void greens_entry(const ScDev* sc, double* fld1, uint
   const double zp = (t + 1) * dzg;
   const double rp = rpi * dr;
   const double d = zp - zs;
-  const double d2 = d * d;
+  double d2 = d * d;
   const double c = (rp - rs) * (rp - rs);
   const double b = (rp + rs) * (rp + rs);
   const double rsrp = rs * rp;
   double a = 4.0 * rsrp / (b + d2);
-  const double g = 1.0 - a;
-  const double h = hps::log(g);
-  const double ee = 1.0 + g * (0.4630106 - 0.2452740 * h + g * (0.1077857 - 0.04125321 * h));
-  const double ek = 1.38629436 - 0.5 * h + g * (0.1119697 - 0.1213486 * h + g *
-                    (0.07253230 - 0.02887472 * h));
+  double g = 1.0 - a;
+  double h = hps::log(g);
+  double ee = 1.0 + g * (0.4630106 - 0.2452740 * h + g * (0.1077857 - 0.04125321 * h));
+  double ek = 1.38629436 - 0.5 * h + g * (0.1119697 - 0.1213486 * h + g *
+              (0.07253230 - 0.02887472 * h));
   double er1 = 0.0;
   a = ::sqrt(b + d2);
   const double rs2mrp2 = (rs * rs - rp * rp);
   if (rp != 0.0) er1 = (ek - (rs2mrp2 + d2) * ee / (c + d2)) / (2.0 * rp * a);
-  const double ez1 = d * ee / (a * (c + d2));
+  double ez1 = d * ee / (a * (c + d2));
+  // image rings of the neighbouring bunches, one RF period apart in the
+  // bunch frame (space_charge_kernel.cu:74-94)
+  const uint32_t adj = sc->adj_eff;
+  if (adj != 0) {
+    const double wave_len = CLIGHT / r_freq;
+    const double pl = r_beta * gm * wave_len;
+    for (uint32_t i = 1; i <= adj; ++i) {
+      double xi = (double)i;
+      for (int j = 0; j < 2; ++j) {
+        const double d1 = d - xi * pl;
+        d2 = d1 * d1;
+        a = 4.0 * rsrp / (b + d2);
+        g = 1.0 - a;
+        h = hps::log(g);
+        ee = 1.0 + g * (0.4630106 - 0.2452740 * h + g * (0.1077857 - 0.04125321 * h));
+        ek = 1.38629436 - 0.5 * h + g * (0.1119697 - 0.1213486 * h + g *
+             (0.07253230 - 0.02887472 * h));
+        a = ::sqrt(b + d2);
+        if (rp != 0.0) er1 += (ek - (rs2mrp2 + d2) * ee / (c + d2)) / (2.0 * rp * a);
+        ez1 += d1 * ee / (a * (c + d2));
+        xi = -xi;
+      }
+    }
+  }
   const double c1 = 5721.67 * r_current_ov_mc2 / (r_freq * 1e6 * (double)r_num_part);
   fld1[2 * (size_t)idx] = c1 * er1;
   fld1[2 * (size_t)idx + 1] = c1 * ez1;

@@ -1230,9 +1230,12 @@ void orc_spch_get_tables(const orc_spch* s, double* fld1, double* fld2, int64_t*
   if (bins) memcpy(bins, s->bins, (size_t)nr * nz * sizeof(int64_t));
 }
 
-/* UpdateFldTbl1Kernel, space_charge_kernel.cu:38-100 (adj_bunch = 0) */
+/* UpdateFldTbl1Kernel, space_charge_kernel.cu:38-100, including the image
+ * rings of r_adj_bunch neighbouring bunches on each side, one RF period
+ * (beta*gamma*lambda in the bunch frame) apart (:74-94). */
 void orc_t_green_table(uint32_t nr, uint32_t nz, double dr_, double dz_, double r_freq,
-    double r_beta, uint32_t r_num_part, double r_current_ov_mc2, double* fld1)
+    double r_beta, uint32_t r_num_part, double r_current_ov_mc2, uint32_t r_adj_bunch,
+    double* fld1)
 {
   double gm = 1.0 / sqrt(1.0 - r_beta * r_beta);
   double dr = dr_;
@@ -1263,6 +1266,25 @@ void orc_t_green_table(uint32_t nr, uint32_t nz, double dr_, double dz_, double 
         double rs2mrp2 = (rs * rs - rp * rp);
         if (rp != 0.0) er1 = (ek - (rs2mrp2 + d2) * ee / (c + d2)) / (2.0 * rp * a);
         double ez1 = d * ee / (a * (c + d2));
+        double wave_len = CLIGHT / r_freq;
+        double pl = r_beta * gm * wave_len;
+        for (int i = 1; i <= (int)r_adj_bunch; ++i) {
+          double xi = i;
+          for (int j = 0; j < 2; ++j) {
+            double d1 = d - xi * pl;
+            d2 = d1 * d1;
+            a = 4.0 * rsrp / (b + d2);
+            g = 1.0 - a;
+            h = orc_log(g);
+            ee = 1.0 + g * (0.4630106 - 0.2452740 * h + g * (0.1077857 - 0.04125321 * h));
+            ek = 1.38629436 - 0.5 * h + g * (0.1119697 - 0.1213486 * h + g *
+                 (0.07253230 - 0.02887472 * h));
+            a = sqrt(b + d2);
+            if (rp != 0.0) er1 += (ek - (rs2mrp2 + d2) * ee / (c + d2)) / (2.0 * rp * a);
+            ez1 += d1 * ee / (a * (c + d2));
+            xi = -xi;
+          }
+        }
         fld1[2 * cnt] = c1 * er1;
         fld1[2 * cnt + 1] = c1 * ez1;
       }
@@ -1377,6 +1399,30 @@ static void deposit(orc_spch* s, orc_beam* b)
   s->s.num_on_mesh = on_mesh;
 }
 
+/* Field factors r/d^3 and z/d^3 of the bunch as a point charge at
+ * distance (r, z), plus its r_adj_bunch neighbours on each side at z +- k*pl
+ * (the off-mesh branch of ApplySpchKickKernel, space_charge_kernel.cu:642-662). */
+void orc_t_offmesh_field(double z, double r, double pl, uint32_t r_adj_bunch, double* rod3,
+                         double* zod3)
+{
+  double d = sqrt(z * z + r * r);
+  double invd3 = 1.0 / (d * d * d);
+  double ro = r * invd3, zo = z * invd3;
+  for (int nb = 1; nb <= (int)r_adj_bunch; ++nb) {
+    int xi = nb;
+    for (int lr = 1; lr <= 2; ++lr) {
+      double sz = z + xi * pl;
+      d = sqrt(sz * sz + r * r);
+      invd3 = 1.0 / (d * d * d);
+      ro += r * invd3;
+      zo += sz * invd3;
+      xi = -xi;
+    }
+  }
+  *rod3 = ro;
+  *zod3 = zo;
+}
+
 /* ApplySpchKickKernel, space_charge_kernel.cu:572-706; launch geometry of
  * KickBeamKernelCall space_charge.cu:230-238 (Q3: indices that are
  * multiples of grid*(blck/4) are never kicked) */
@@ -1432,10 +1478,7 @@ static void kick(orc_spch* s, orc_beam* b, double r_length, double r_ratio_r,
     z = -c2 * ddz;
     if (r > nx * dr || fabs(z) > hl) {
       z = -c2 * (phi - phimc);
-      d = sqrt(z * z + r * r);
-      double invd3 = 1.0 / (d * d * d);
-      rod3 = r * invd3;
-      zod3 = z * invd3;
+      orc_t_offmesh_field(z, r, r_beta * gmg * wave_len, s->adj_eff, &rod3, &zod3);
       double cbgcoef = c1 * c3 * PI * 0.5 * s->s.num_on_mesh;
       cbgr = cbgcoef * rod3;
       cbgzs = cbgcoef * zod3;
@@ -1473,6 +1516,23 @@ static void kick(orc_spch* s, orc_beam* b, double r_length, double r_ratio_r,
   }
 }
 
+/* Largest (x - x_avg)^2 + (y - y_avg)^2 over particles with loss == 0
+ * (MaxR2PhiReduceKernel, beam_kernel.cu:793-930): the comparison a < v
+ * keeps a when v is NaN, as fmax does; a maximum is order-free. */
+static double max_r2(const orc_beam* b)
+{
+  double r2_max = 0.0;
+  double xa = b->st.x_avg, ya = b->st.y_avg;
+  #pragma omp parallel for reduction(max:r2_max) schedule(static)
+  for (uint32_t i = 0; i < b->n; ++i) {
+    if (b->loss[i] != 0) continue;
+    double tx = b->x[i] - xa, ty = b->y[i] - ya;
+    double r2 = tx * tx + ty * ty;
+    r2_max = r2_max < r2 ? r2 : r2_max;
+  }
+  return r2_max;
+}
+
 /* Scheff::Start, space_charge.cpp:82-139 */
 int orc_spch_start(orc_spch* s, orc_beam* b, double r_length)
 {
@@ -1494,7 +1554,6 @@ int orc_spch_start(orc_spch* s, orc_beam* b, double r_length)
   uint32_t adj_bunch;
   if (s->s.adj_bunch_w == 0.0 || ke < s->s.adj_bunch_w) adj_bunch = s->s.adj_bunch;
   else adj_bunch = 0;
-  if (adj_bunch != 0) return set_err("adjacent-bunch space charge (adj_bunch > 0) is not implemented");
   if (s->s.mesh_w != 0.0) return set_err("mesh_size_cutoff_w is not implemented");
   /* SetParameters, space_charge.cu:77-86 */
   s->freq = b->freq; s->beta = beta; s->frac = s->s.fraction; s->adj_eff = adj_bunch;
@@ -1513,10 +1572,19 @@ int orc_spch_start(orc_spch* s, orc_beam* b, double r_length)
      * pointers x_sig_good > y_sig_good; x_sig_good is allocated first
      * (beam.cu:51,54), so the y branch is the one taken. */
     double bl = beta * CLIGHT / s->freq;
-    s->s.dr = 3.0 * b->st.y_sig_good / s->s.nx;
-    s->s.dz = 3.0 * bl * b->st.phi_sig_good / (s->s.nz * PI);
+    if (adj_bunch != 0) {
+      /* UpdateMaxRPhi (beam.cu:829-851, beam_kernel.cu:793-930,1378-1383):
+       * largest r^2 about the beam's last all-particle centroid (x_avg,
+       * y_avg) over loss == 0; the mesh spans it radially and one RF
+       * period longitudinally (space_charge_kernel.cu:18-32 with r_coef 1) */
+      s->s.dr = 1.0 * sqrt(max_r2(b)) / s->s.nx;
+      s->s.dz = bl / s->s.nz;
+    } else {
+      s->s.dr = 3.0 * b->st.y_sig_good / s->s.nx;
+      s->s.dz = 3.0 * bl * b->st.phi_sig_good / (s->s.nz * PI);
+    }
     orc_t_green_table(s->s.nx, s->s.nz, s->s.dr, s->s.dz, s->freq, s->beta, b->n,
-                      b->current * s->frac / b->mass, s->fld1);
+                      b->current * s->frac / b->mass, adj_bunch, s->fld1);
     deposit(s, b);
     orc_t_field(s->s.nx, s->s.nz, s->bins, s->fld1, s->fld2);
     kick(s, b, r_length, 1.0, 1.0, 1.0, 1.0);

@@ -165,7 +165,9 @@ double orc_t_bessi1p(double x);
 void orc_t_ttf(double beta, const orc_gap_param* g, double* t, double* tp, double* sp);
 double orc_t_canon_sum(const double* v, uint32_t n);
 void orc_t_green_table(uint32_t nr, uint32_t nz, double dr, double dz, double freq,
-    double beta, uint32_t num, double current_ov_mc2, double* fld1);
+    double beta, uint32_t num, double current_ov_mc2, uint32_t adj_bunch, double* fld1);
+void orc_t_offmesh_field(double z, double r, double pl, uint32_t adj_bunch, double* rod3,
+                         double* zod3);
 void orc_t_field(uint32_t nr, uint32_t nz, const int64_t* bins, const double* fld1, double* fld2);
 uint32_t orc_t_nextpow2(uint32_t v);
 

@@ -135,6 +135,7 @@ def lib():
         for f in ("orc_spch_set_interval", "orc_spch_set_remesh_threshold",
                   "orc_spch_set_adj_bunch_cutoff_w", "orc_spch_set_mesh_size_cutoff_w"):
             getattr(L, f).argtypes = [P, C.c_double]
+        L.orc_spch_set_adj_bunch.argtypes = [P, C.c_uint32]
         L.orc_spch_get_state.argtypes = [P, C.POINTER(SpchState)]
         L.orc_spch_get_tables.argtypes = [P, dp, dp, C.POINTER(C.c_int64)]
         L.orc_spch_start.restype = C.c_int
@@ -159,7 +160,9 @@ def lib():
         L.orc_t_canon_sum.restype = C.c_double
         L.orc_t_canon_sum.argtypes = [dp, C.c_uint32]
         L.orc_t_green_table.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_double,
-                                        C.c_double, C.c_double, C.c_uint32, C.c_double, dp]
+                                        C.c_double, C.c_double, C.c_uint32, C.c_double,
+                                        C.c_uint32, dp]
+        L.orc_t_offmesh_field.argtypes = [C.c_double, C.c_double, C.c_double, C.c_uint32, dp, dp]
         L.orc_t_field.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_int64), dp, dp]
         L.orc_t_nextpow2.restype = C.c_uint32
         L.orc_t_nextpow2.argtypes = [C.c_uint32]
@@ -281,6 +284,9 @@ class SpaceCharge:
             self.h = None
 
     def set_remesh_threshold(self, v): lib().orc_spch_set_remesh_threshold(self.h, v)
+    def set_adj_bunch(self, v): lib().orc_spch_set_adj_bunch(self.h, int(v))
+    def set_adj_bunch_cutoff_w(self, v): lib().orc_spch_set_adj_bunch_cutoff_w(self.h, v)
+    def set_mesh_size_cutoff_w(self, v): lib().orc_spch_set_mesh_size_cutoff_w(self.h, v)
 
     def state(self):
         s = SpchState()

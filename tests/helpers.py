@@ -19,7 +19,7 @@ def workload(idx=0, num=4096, periods=4, current=None, sc=True, seed=None):
     return wl
 
 
-def run_oracle(wl, elems=None, dist=None):
+def run_oracle(wl, elems=None, dist=None, adj=0, adj_w=0.0):
     from oracle import oracle as O
     b = wl.beam
     elems = elems if elems is not None else cfg.build_lattice(wl.lattice)
@@ -27,13 +27,15 @@ def run_oracle(wl, elems=None, dist=None):
     beam.set_distribution(*(dist if dist is not None else cfg.make_distribution(b)))
     beam.set_frequency(b.freq)
     beam.set_ref_w(b.w0)
-    sc = O.SpaceCharge(wl.nr, wl.nz, wl.interval) if wl.space_charge else None
+    sc = O.SpaceCharge(wl.nr, wl.nz, wl.interval, adj) if wl.space_charge else None
+    if sc is not None and adj_w:
+        sc.set_adj_bunch_cutoff_w(adj_w)
     sim = O.Simulator(beam, elems, sc)
     sim.simulate()
     return beam, sc, sim
 
 
-def run_gpu(wl, elems=None, dist=None):
+def run_gpu(wl, elems=None, dist=None, adj=0, adj_w=0.0):
     from hpsim_amd import HPSim
     b = wl.beam
     elems = elems if elems is not None else cfg.build_lattice(wl.lattice)
@@ -42,7 +44,10 @@ def run_gpu(wl, elems=None, dist=None):
     beam.set_frequency(b.freq)
     beam.set_ref_w(b.w0)
     bl = HPSim.BeamLine(elements=elems)
-    sc = HPSim.SpaceCharge(nr=wl.nr, nz=wl.nz, interval=wl.interval) if wl.space_charge else None
+    sc = (HPSim.SpaceCharge(nr=wl.nr, nz=wl.nz, interval=wl.interval, adj_bunch=adj)
+          if wl.space_charge else None)
+    if sc is not None and adj_w:
+        sc.set_adj_bunch_cutoff_w(adj_w)
     sim = HPSim.Simulator(beam, bl, sc)
     sim.simulate()
     return beam, sc, sim

@@ -250,3 +250,22 @@ def test_space_charge_tables_bit_exact(nr, nz):
         assert np.array_equal(g.view(np.uint64), o.view(np.uint64)), name
     err = H.compare_arrays(gb.arrays(), ob.arrays())
     assert err is None, err
+
+
+@pytest.mark.parametrize("num,adj,adj_w", [(4096, 2, 0.0), (6000, 3, 1.6), (2, 2, 0.0), (5, 1, 0.0)])
+def test_adjacent_bunch_space_charge_bit_exact(num, adj, adj_w):
+    """adj_bunch > 0: image rings in the Green's table, image point charges
+    for off-mesh particles, the mesh sized by the largest radius and one RF
+    period; with a cutoff energy the beam crosses mid-lattice (the images
+    switch off there).  N = 2 takes the direct r-max scan."""
+    wl = H.workload(0, num=num, periods=6, sc=True)
+    ob, osc, _ = H.run_oracle(wl, adj=adj, adj_w=adj_w)
+    gb, gsc, _ = H.run_gpu(wl, adj=adj, adj_w=adj_w)
+    err = H.compare_arrays(gb.arrays(), ob.arrays())
+    assert err is None, err
+    gs, os_ = gsc.state(), osc.state()
+    for k in ("remesh_count", "kick_count", "num_on_mesh", "dr", "dz"):
+        assert gs[k] == os_[k], (k, gs[k], os_[k])
+    if num > 2:
+        for name, g, o in zip(("greens", "field", "bins"), gsc.tables(), osc.tables()):
+            assert np.array_equal(g.view(np.uint64), o.view(np.uint64)), name

@@ -179,7 +179,8 @@ def _beam(n, w0, phi0, seed):
     return x, xp, y, yp, phi, w
 
 
-def _gpu_vs_oracle(oracle_lib, els, n, w0, phi0, nr, nz):
+def _gpu_vs_oracle(oracle_lib, els, n, w0, phi0, nr, nz, interval=0.05, adj=0, adj_w=0.0,
+                   remesh=None):
     from hpsim_amd import HPSim
     from tests.helpers import compare_arrays
     O = oracle_lib
@@ -188,13 +189,21 @@ def _gpu_vs_oracle(oracle_lib, els, n, w0, phi0, nr, nz):
     ob.set_distribution(*dist)
     ob.set_frequency(FREQ)
     ob.set_ref_w(w0)
-    osc = O.SpaceCharge(nr, nz, 0.05)
+    osc = O.SpaceCharge(nr, nz, interval, adj)
+    if adj_w:
+        osc.set_adj_bunch_cutoff_w(adj_w)
+    if remesh is not None:
+        osc.set_remesh_threshold(remesh)
     O.Simulator(ob, els, osc).simulate()
     gb = HPSim.Beam(mass=MASS, charge=-1.0, current=0.015, num=n)
     gb.set_distribution(*dist)
     gb.set_frequency(FREQ)
     gb.set_ref_w(w0)
-    gsc = HPSim.SpaceCharge(nr=nr, nz=nz, interval=0.05)
+    gsc = HPSim.SpaceCharge(nr=nr, nz=nz, interval=interval, adj_bunch=adj)
+    if adj_w:
+        gsc.set_adj_bunch_cutoff_w(adj_w)
+    if remesh is not None:
+        gsc.set_remesh_threshold(remesh)
     HPSim.Simulator(gb, HPSim.BeamLine(elements=els), gsc).simulate()
     err = compare_arrays(gb.arrays(), ob.arrays())
     assert err is None, err
@@ -224,6 +233,16 @@ def test_gpu_tbtd_transport_with_bunchers_and_dipoles_bit_exact(oracle_lib, tmp_
     compensation."""
     els = _section(tmp_path, "tbtd.db")
     _gpu_vs_oracle(oracle_lib, els, 2048, 0.7518, 0.0, 16, 64)
+
+
+@pytest.mark.gpu
+def test_gpu_tbtd_with_adjacent_bunches_bit_exact(oracle_lib, tmp_path):
+    """The space-charge settings of the reference's LBEG script
+    (sim-lbeg.py:72-78): three adjacent bunches each side below 0.8 MeV,
+    interval 0.025 m, remesh threshold 0.02."""
+    els = _section(tmp_path, "tbtd.db")
+    _gpu_vs_oracle(oracle_lib, els, 2048, 0.7518, 0.0, 32, 128, interval=0.025, adj=3,
+                   adj_w=0.8, remesh=0.02)
 
 
 @pytest.mark.gpu

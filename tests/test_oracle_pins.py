@@ -11,6 +11,7 @@ import pytest
 import scipy.special as sp
 
 from hpsim_amd import configs as cfg
+from tests import helpers as H
 
 ULP = np.finfo(np.float64).eps
 
@@ -148,7 +149,7 @@ def test_green_table_matches_exact_ring_field(L):
     polynomials of the reference with scipy's complete elliptic integrals."""
     nr, nz, dr, dz, freq, beta, num, cur = 8, 16, 1e-4, 2e-4, 201.25, 0.05, 1000, 0.01 / 939.294
     f1 = np.empty((nr + 1) * nr * nz * 2)
-    L.orc_t_green_table(nr, nz, dr, dz, freq, beta, num, cur, f1.ctypes.data_as(C.POINTER(C.c_double)))
+    L.orc_t_green_table(nr, nz, dr, dz, freq, beta, num, cur, 0, f1.ctypes.data_as(C.POINTER(C.c_double)))
     f1 = f1.reshape(nr + 1, nr, nz, 2)
     gm = 1 / math.sqrt(1 - beta * beta)
     dzg = dz * gm
@@ -177,6 +178,143 @@ def test_green_table_matches_exact_ring_field(L):
                     er = (K - R * E) / (2 * rp * a)
                     bound = (eK + eE * abs(R)) / (2 * rp * a) * 1.01
                     assert abs(f1[rp_i, k, t, 0] / c1 - er) <= bound
+
+
+def _ring_field(rs, rp, d):
+    """E_r, E_z of a unit charged ring (radius rs) at (rp, axial offset d),
+    from scipy's complete elliptic integrals, and error bounds of the
+    Hastings polynomials (A&S 17.3.34 / 17.3.36) for the same terms."""
+    eK, eE = 3e-5, 4e-5
+    b = (rp + rs) ** 2
+    c = (rp - rs) ** 2
+    m = 4 * rs * rp / (b + d * d)
+    K, E = sp.ellipk(m), sp.ellipe(m)
+    a = math.sqrt(b + d * d)
+    ez = d * E / (a * (c + d * d))
+    bz = eE * abs(d) / (a * (c + d * d)) * 1.01
+    if rp == 0:
+        return 0.0, ez, 0.0, bz
+    R = (rs * rs - rp * rp + d * d) / (c + d * d)
+    er = (K - R * E) / (2 * rp * a)
+    return er, ez, (eK + eE * abs(R)) / (2 * rp * a) * 1.01, bz
+
+
+def test_green_table_adjacent_bunch_images(L):
+    """With adj_bunch = k the table adds the rings of k neighbouring bunches
+    on each side, one RF period beta*gamma*lambda apart in the bunch frame
+    (space_charge_kernel.cu:74-94): the sum of 2k + 1 exact ring fields."""
+    nr, nz, dr, dz, freq, beta, num, cur = 6, 12, 1.5e-3, 4e-3, 201.25, 0.04, 1000, 0.01 / 939.294
+    adj = 2
+    f1 = np.empty((nr + 1) * nr * nz * 2)
+    L.orc_t_green_table(nr, nz, dr, dz, freq, beta, num, cur, adj,
+                        f1.ctypes.data_as(C.POINTER(C.c_double)))
+    f1 = f1.reshape(nr + 1, nr, nz, 2)
+    gm = 1 / math.sqrt(1 - beta * beta)
+    dzg = dz * gm
+    pl = beta * gm * 299.792458 / freq
+    c1 = 5721.67 * cur / (freq * 1e6 * num)
+    for rp_i in range(nr + 1):
+        for k in range(nr):
+            for t in range(nz):
+                rs = math.sqrt(0.5 * (k * k + (k + 1) ** 2)) * dr
+                rp = rp_i * dr
+                d = (t + 0.5) * dzg
+                er = ez = ber = bez = 0.0
+                for img in range(-adj, adj + 1):
+                    e_r, e_z, b_r, b_z = _ring_field(rs, rp, d - img * pl)
+                    er, ez, ber, bez = er + e_r, ez + e_z, ber + b_r, bez + b_z
+                assert abs(f1[rp_i, k, t, 1] / c1 - ez) <= bez + 1e-13 * abs(ez)
+                if rp_i == 0:
+                    assert f1[rp_i, k, t, 0] == 0.0
+                else:
+                    assert abs(f1[rp_i, k, t, 0] / c1 - er) <= ber + 1e-13 * abs(er)
+    # the images matter at this spacing: they change E_z by more than the bound
+    f0 = np.empty_like(f1.ravel())
+    L.orc_t_green_table(nr, nz, dr, dz, freq, beta, num, cur, 0,
+                        f0.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.max(np.abs(f1.ravel() - f0)[1::2] / c1) > 100 * 4e-5
+
+
+def test_offmesh_field_is_gradient_of_bunch_chain_potential(L):
+    """Off-mesh particles see the bunch, and with adj_bunch = k also its k
+    neighbours on each side, as point charges one period pl apart
+    (space_charge_kernel.cu:642-662): (r/d^3, z/d^3) summed over the chain is
+    minus the gradient of the chain's potential sum 1/d, checked here by
+    central differences."""
+    rng = np.random.default_rng(12)
+    pl = 0.0596
+    ro, zo = C.c_double(), C.c_double()
+
+    def phi(r, z, k):
+        return sum(1.0 / math.hypot(r, z + j * pl) for j in range(-k, k + 1))
+
+    for k in (0, 1, 3):
+        for _ in range(40):
+            r = rng.uniform(1e-3, 8e-3)
+            z = rng.uniform(-0.025, 0.025)
+            L.orc_t_offmesh_field(z, r, pl, k, C.byref(ro), C.byref(zo))
+            h = 1e-6
+            gr = -(phi(r + h, z, k) - phi(r - h, z, k)) / (2 * h)
+            gz = -(phi(r, z + h, k) - phi(r, z - h, k)) / (2 * h)
+            assert ro.value == pytest.approx(gr, rel=1e-6)
+            assert zo.value == pytest.approx(gz, rel=1e-6, abs=1e-6 * abs(gr))
+    # a wrong pl or a one-sided chain breaks the symmetry z -> -z
+    L.orc_t_offmesh_field(0.01, 2e-3, pl, 3, C.byref(ro), C.byref(zo))
+    a = (ro.value, zo.value)
+    L.orc_t_offmesh_field(-0.01, 2e-3, pl, 3, C.byref(ro), C.byref(zo))
+    assert ro.value == pytest.approx(a[0], rel=1e-14) and zo.value == pytest.approx(-a[1], rel=1e-14)
+
+
+def test_adjacent_bunch_mesh_and_energy_cutoff(oracle_lib):
+    """With adjacent bunches the remesh spans the largest radius about the
+    beam's last all-particle centroid (x_avg, y_avg: stale until a getter or
+    an aperture updates them; particles with loss != 0 excluded, lloss
+    flags not) and one RF period beta*lambda (space_charge.cu:124-128,
+    beam.cu:829-851); above the cutoff energy the bunch is alone again
+    (space_charge.cpp:102-105) and the mesh goes back to 3 sigma."""
+    O = oracle_lib
+    n = 4096
+    wl = cfg.config(0, num=n, periods=1)
+    dist = [np.array(a) for a in cfg.make_distribution(wl.beam)]
+    loss = np.zeros(n, np.uint32)
+    lloss = np.zeros(n, np.uint32)
+    dist[0][100], dist[2][100] = 0.03, 0.0      # transversely lost, far out
+    loss[100] = 7
+    dist[0][200], dist[2][200] = 0.0, -0.012    # longitudinally flagged, still counted
+    lloss[200] = 1
+
+    def run(adj, cutoff):
+        b = O.Beam(n, 939.294, -1.0, 0.01)
+        b.set_distribution(*dist, loss=loss, lloss=lloss)
+        b.set_frequency(201.25)
+        b.set_ref_w(0.75)
+        b.get_avg("x")                    # x_avg of the untranslated beam
+        b.translate("x", 2e-3)            # leaves x_avg stale
+        st = b.stats()
+        sc = O.SpaceCharge(32, 128, 0.05, adj)
+        sc.set_adj_bunch_cutoff_w(cutoff)
+        x0 = b.arrays()
+        sc.start(b, 0.05)
+        return b, sc.state(), st, x0
+
+    b, s, st, a = run(2, 0.0)
+    assert st["x_avg"] != pytest.approx(np.mean(a["x"][loss == 0]), abs=1e-4)
+    ok = a["loss"] == 0
+    r2 = (a["x"][ok] - st["x_avg"]) ** 2 + (a["y"][ok] - st["y_avg"]) ** 2
+    assert s["dr"] * 32 == pytest.approx(math.sqrt(r2.max()), rel=1e-15)
+    assert np.argmax(r2) == np.flatnonzero(ok).tolist().index(200)
+    bst = b.stats()
+    gamma = bst["w_avg_good"] / 939.294 + 1
+    beta = math.sqrt(1 - 1 / gamma ** 2)
+    assert s["dz"] * 128 == pytest.approx(beta * 299.792458 / 201.25, rel=1e-15)
+    # cutoff above the beam energy: images on; below: off
+    _, s2, _, _ = run(2, 0.8)
+    assert s2["dr"] == s["dr"]
+    b3, s3, _, _ = run(2, 0.7)
+    assert s3["dr"] * 32 == pytest.approx(3 * b3.stats()["y_sig_good"], rel=1e-15)
+    b4, s4, _, _ = run(0, 0.0)
+    assert s4["dr"] == s3["dr"] and s4["dz"] == s3["dz"]
+    assert H.compare_arrays(b3.arrays(), b4.arrays()) is None
 
 
 def test_field_sum_brute_force(L):
