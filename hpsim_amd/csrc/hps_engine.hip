@@ -1216,6 +1216,37 @@ int compile_element(Compiler& c, uint32_t id)
 
 }  // namespace
 
+// Scheff::Start's mesh-size cutoff (space_charge.cpp:107-117), decided on
+// the host from the beam's good mean energy of this kick point: above
+// mesh_w half the mesh in each direction (tables reallocated zeroed, as
+// SetMeshSize does) and four times the kick interval.
+static int mesh_cutoff_step(hps_sim* m)
+{
+  hps_spch* sc = m->sc;
+  hps_beam* b = m->b;
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  double ke = 0.0;
+  HPS_CHECK(hipMemcpy(&ke, (const char*)b->st + offsetof(hps_beam_stats, w_avg_good), sizeof ke,
+                      hipMemcpyDeviceToHost));
+  hps_spch_state& c = sc->cfg;
+  const double mw = c.mesh_w;
+  if (ke > mw && (c.nx != c.nx_init / 2 || c.nz != c.nz_init / 2)) {
+    if (hps_spch_set_mesh_size(sc, c.nx_init / 2, c.nz_init / 2)) return -1;
+  } else if (ke <= mw && (c.nx != c.nx_init || c.nz != c.nz_init)) {
+    if (hps_spch_set_mesh_size(sc, c.nx_init, c.nz_init)) return -1;
+  }
+  bool push = false;
+  if (ke > mw && c.interval != c.interval_init * 4) {
+    c.interval = c.interval_init * 4;
+    push = true;
+  } else if (ke <= mw && c.interval != c.interval_init) {
+    c.interval = c.interval_init;
+    push = true;
+  }
+  if (push) return spch_push_cfg(sc);
+  return 0;
+}
+
 extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
 {
   if (!m) return fail("null simulator");
@@ -1225,10 +1256,17 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   const int32_t start_index = start < 0 ? 0 : start;
   const int32_t end_index = end < 0 ? (int32_t)nel - 1 : end;
   if (end_index >= (int32_t)nel || start_index >= (int32_t)nel) return fail("element index out of range");
+  // with a mesh-size cutoff energy the kick interval and the mesh follow the
+  // beam energy kick by kick: the program is compiled and issued kick point
+  // by kick point, with one host read of the mean energy at each
+  const bool stepped = m->sc_on && m->sc && m->sc->cfg.mesh_w != 0.0;
   if (m->sc_on && m->sc) {
-    if (m->sc->cfg.mesh_w != 0.0) return fail("mesh_size_cutoff_w is not implemented");
+    const hps_spch_state& cf = m->sc->cfg;
+    if (stepped && (cf.nx_init < 2 || cf.nz_init < 2))
+      return fail("mesh_size_cutoff_w needs a mesh of at least 2 x 2");
     // the deposit keeps one int64 histogram of nr * nz bins per workgroup in LDS
-    if ((size_t)m->sc->cfg.nx * m->sc->cfg.nz * 8 + kBlockLdsReserve > m->sc->max_lds)
+    const size_t bins = std::max((size_t)cf.nx * cf.nz, stepped ? (size_t)cf.nx_init * cf.nz_init : 0);
+    if (bins * 8 + kBlockLdsReserve > m->sc->max_lds)
       return fail("the SCHEFF mesh has more bins than one workgroup's LDS histogram holds");
   }
   // SimulationEngine::Simulate, simulation_engine.cpp:73-95
@@ -1241,26 +1279,14 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   c.quad_gap_run = b->blck_size / 2 != 0;
   c.dipole_run = b->blck_size / 4 != 0;
   c.sc_stats = b->n > 2;
-  for (uint32_t i = 0; (int32_t)i <= end_index; ++i)
-    if ((int32_t)i >= start_index || m->bl->el[i].kind == HPS_SPCH_COMP)
-      if (compile_element(c, i)) return -1;
-  c.close(STAT_NONE, false, 0);
 
-  if (c.ops.size() > m->ops_cap) {
-    (void)hipFree(m->d_ops);
-    m->ops_cap = c.ops.size() + 64;
-    HPS_CHECK(hipMalloc(&m->d_ops, m->ops_cap * sizeof(Op)));
-  }
   hipStream_t s = b->stream;
-  if (!c.ops.empty())
-    HPS_CHECK(hipMemcpyAsync(m->d_ops, c.ops.data(), c.ops.size() * sizeof(Op),
-                             hipMemcpyHostToDevice, s));
   const uint32_t kstride = b->grid_size * (b->blck_size / 4);
   const bool kick_ok = kstride != 0;
   const bool deposit_ok = b->blck_size / 4 != 0;
 
   // The launch list of this call.
-  // kinds 0..6 are the profiling families of the header; 7 is a host step
+  // kinds 0..5 are the profiling families of the header; L_HOST is a host step
   enum { L_SEG = 0, L_PHASE, L_BUILD, L_FIELD, L_SEG_FINAL, L_PHASE_FINAL, L_HOST };
   struct Launch {
     int kind;
@@ -1270,14 +1296,14 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   };
   std::vector<Launch> launches;
   bool host_steps = false;
-  for (const Step& st : c.steps) {
+  auto add_step = [&](const Step& st) {
     if (st.kind == 1) {
       Launch l{};
       l.kind = L_HOST;
       l.new_freq = st.new_freq;
       launches.push_back(l);
       host_steps = true;
-      continue;
+      return;
     }
     if (st.op_end > st.op_begin || st.kick || st.stats != STAT_NONE) {
       Launch l{};
@@ -1323,7 +1349,7 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       g.kind = L_FIELD;
       launches.push_back(g);
     }
-  }
+  };
   auto issue = [&](const Launch& l, hipStream_t st) -> hipError_t {
     hps_spch* sc = m->sc;
     switch (l.kind) {
@@ -1343,76 +1369,134 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   // profiling: HIP events around every launch (eager launches only)
   std::vector<std::pair<int, size_t>> marks;
   size_t nev = 0;
-  const bool eager = m->profile || host_steps || !m->use_graph;
+  // one eager launch; host steps run in order with the launches
+  auto issue_eager = [&](const Launch& l) -> int {
+    if (l.kind == L_HOST) {
+      // the phases of a frequency change are not known at compile time of
+      // later launches' arguments: rebuild them after the host step
+      return change_frequency(b, l.new_freq);
+    }
+    if (m->profile) {
+      while (m->evpool.size() < nev + 2) {
+        hipEvent_t e;
+        HPS_CHECK(hipEventCreate(&e));
+        m->evpool.push_back(e);
+      }
+      marks.push_back({l.kind, nev});
+      HPS_CHECK(hipEventRecord(m->evpool[nev], s));
+    }
+    Launch lc = l;
+    if (l.kind == L_PHASE || l.kind == L_PHASE_FINAL) {
+      lc.ph.current = b->current;
+      lc.ph.freq = b->freq;
+      lc.ph.bins = m->sc->bins;   // a mesh-size step may have reallocated it
+    }
+    HPS_CHECK(issue(lc, s));
+    if (m->profile) {
+      HPS_CHECK(hipEventRecord(m->evpool[nev + 1], s));
+      nev += 2;
+    }
+    return 0;
+  };
+
   HPS_CHECK(hipEventRecord(m->ev0, s));
-  if (eager) {
-    for (const Launch& l : launches) {
-      if (l.kind == L_HOST) {
-        // the phases of a frequency change are not known at compile time of
-        // later launches' arguments: rebuild them after the host step
-        if (change_frequency(b, l.new_freq)) return -1;
-        continue;
-      }
-      if (m->profile) {
-        while (m->evpool.size() < nev + 2) {
-          hipEvent_t e;
-          HPS_CHECK(hipEventCreate(&e));
-          m->evpool.push_back(e);
+  if (stepped) {
+    // compile one element, then issue every closed step; the map table grows
+    // on the device as the program does
+    size_t done = 0;
+    auto flush = [&]() -> int {
+      for (; done < c.steps.size(); ++done) {
+        const Step& st = c.steps[done];
+        if (st.kind == 0 && st.op_end > st.op_begin) {
+          if ((size_t)st.op_end > m->ops_cap) {
+            HPS_CHECK(hipStreamSynchronize(s));
+            (void)hipFree(m->d_ops);
+            m->d_ops = nullptr;
+            m->ops_cap = 2 * (size_t)st.op_end + 64;
+            HPS_CHECK(hipMalloc(&m->d_ops, m->ops_cap * sizeof(Op)));
+          }
+          HPS_CHECK(hipMemcpyAsync(m->d_ops + st.op_begin, c.ops.data() + st.op_begin,
+                                   (size_t)(st.op_end - st.op_begin) * sizeof(Op),
+                                   hipMemcpyHostToDevice, s));
+          // the host vector may move when the program grows
+          HPS_CHECK(hipStreamSynchronize(s));
         }
-        marks.push_back({l.kind, nev});
-        HPS_CHECK(hipEventRecord(m->evpool[nev], s));
+        launches.clear();
+        add_step(st);
+        for (const Launch& l : launches) {
+          if (l.kind == L_PHASE_FINAL && mesh_cutoff_step(m)) return -1;
+          if (issue_eager(l)) return -1;
+        }
       }
-      Launch lc = l;
-      if (l.kind == L_PHASE || l.kind == L_PHASE_FINAL) {
-        lc.ph.current = b->current;
-        lc.ph.freq = b->freq;
-      }
-      HPS_CHECK(issue(lc, s));
-      if (m->profile) {
-        HPS_CHECK(hipEventRecord(m->evpool[nev + 1], s));
-        nev += 2;
-      }
-    }
-  } else {
-    std::vector<uint8_t> sig;
-    const auto put = [&](const void* p, size_t nb) {
-      const uint8_t* q = (const uint8_t*)p;
-      sig.insert(sig.end(), q, q + nb);
+      return 0;
     };
-    for (const Launch& l : launches) {
-      put(&l.kind, sizeof l.kind);
-      if (l.kind == L_SEG || l.kind == L_SEG_FINAL) put(&l.seg, sizeof l.seg);
-      else put(&l.ph, sizeof l.ph);
-    }
-    put(c.ops.data(), c.ops.size() * sizeof(Op));
-    if (m->sc) {
-      const hps_spch* sc = m->sc;
-      const void* ptrs[4] = {sc->dev, sc->fld1, sc->fld2, sc->bins};
-      put(ptrs, sizeof ptrs);
-      put(&sc->cfg.nx, sizeof sc->cfg.nx);
-      put(&sc->cfg.nz, sizeof sc->cfg.nz);
-      put(&sc->max_lds, sizeof sc->max_lds);
-    }
-    if (!m->graph_exec || sig != m->graph_sig) {
-      if (m->graph_exec) { (void)hipGraphExecDestroy(m->graph_exec); m->graph_exec = nullptr; }
-      if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
-      HPS_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
-      hipError_t err = hipSuccess;
-      for (const Launch& l : launches) {
-        err = issue(l, s);
-        if (err != hipSuccess) break;
+    for (uint32_t i = 0; (int32_t)i <= end_index; ++i)
+      if ((int32_t)i >= start_index || m->bl->el[i].kind == HPS_SPCH_COMP) {
+        if (compile_element(c, i) || flush()) return -1;
       }
-      hipGraph_t g = nullptr;
-      hipError_t e2 = hipStreamEndCapture(s, &g);
-      if (err != hipSuccess) return fail(std::string("graph capture: ") + hipGetErrorString(err));
-      HPS_CHECK(e2);
-      m->graph = g;
-      HPS_CHECK(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0));
-      m->graph_sig.swap(sig);
-    } else {
-      m->graph_replays += 1;
+    c.close(STAT_NONE, false, 0);
+    if (flush()) return -1;
+  } else {
+    for (uint32_t i = 0; (int32_t)i <= end_index; ++i)
+      if ((int32_t)i >= start_index || m->bl->el[i].kind == HPS_SPCH_COMP)
+        if (compile_element(c, i)) return -1;
+    c.close(STAT_NONE, false, 0);
+
+    if (c.ops.size() > m->ops_cap) {
+      (void)hipFree(m->d_ops);
+      m->ops_cap = c.ops.size() + 64;
+      HPS_CHECK(hipMalloc(&m->d_ops, m->ops_cap * sizeof(Op)));
     }
-    HPS_CHECK(hipGraphLaunch(m->graph_exec, s));
+    if (!c.ops.empty())
+      HPS_CHECK(hipMemcpyAsync(m->d_ops, c.ops.data(), c.ops.size() * sizeof(Op),
+                               hipMemcpyHostToDevice, s));
+    for (const Step& st : c.steps) add_step(st);
+
+    const bool eager = m->profile || host_steps || !m->use_graph;
+    if (eager) {
+      for (const Launch& l : launches)
+        if (issue_eager(l)) return -1;
+    } else {
+      std::vector<uint8_t> sig;
+      const auto put = [&](const void* p, size_t nb) {
+        const uint8_t* q = (const uint8_t*)p;
+        sig.insert(sig.end(), q, q + nb);
+      };
+      for (const Launch& l : launches) {
+        put(&l.kind, sizeof l.kind);
+        if (l.kind == L_SEG || l.kind == L_SEG_FINAL) put(&l.seg, sizeof l.seg);
+        else put(&l.ph, sizeof l.ph);
+      }
+      put(c.ops.data(), c.ops.size() * sizeof(Op));
+      if (m->sc) {
+        const hps_spch* sc = m->sc;
+        const void* ptrs[4] = {sc->dev, sc->fld1, sc->fld2, sc->bins};
+        put(ptrs, sizeof ptrs);
+        put(&sc->cfg.nx, sizeof sc->cfg.nx);
+        put(&sc->cfg.nz, sizeof sc->cfg.nz);
+        put(&sc->max_lds, sizeof sc->max_lds);
+      }
+      if (!m->graph_exec || sig != m->graph_sig) {
+        if (m->graph_exec) { (void)hipGraphExecDestroy(m->graph_exec); m->graph_exec = nullptr; }
+        if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+        HPS_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
+        hipError_t err = hipSuccess;
+        for (const Launch& l : launches) {
+          err = issue(l, s);
+          if (err != hipSuccess) break;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (err != hipSuccess) return fail(std::string("graph capture: ") + hipGetErrorString(err));
+        HPS_CHECK(e2);
+        m->graph = g;
+        HPS_CHECK(hipGraphInstantiate(&m->graph_exec, g, nullptr, nullptr, 0));
+        m->graph_sig.swap(sig);
+      } else {
+        m->graph_replays += 1;
+      }
+      HPS_CHECK(hipGraphLaunch(m->graph_exec, s));
+    }
   }
   HPS_CHECK(hipEventRecord(m->ev1, s));
   HPS_CHECK(hipEventSynchronize(m->ev1));
@@ -1421,7 +1505,6 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   m->last_ms = ms;
   for (const auto& mk : marks) {
     float t = 0;
-    (void)nev;
     HPS_CHECK(hipEventElapsedTime(&t, m->evpool[mk.second], m->evpool[mk.second + 1]));
     m->prof_ms[mk.first] += t;
     m->prof_n[mk.first] += 1;

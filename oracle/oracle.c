@@ -1554,7 +1554,21 @@ int orc_spch_start(orc_spch* s, orc_beam* b, double r_length)
   uint32_t adj_bunch;
   if (s->s.adj_bunch_w == 0.0 || ke < s->s.adj_bunch_w) adj_bunch = s->s.adj_bunch;
   else adj_bunch = 0;
-  if (s->s.mesh_w != 0.0) return set_err("mesh_size_cutoff_w is not implemented");
+  /* above the mesh cutoff energy: half the mesh in each direction and four
+   * times the kick interval (space_charge.cpp:107-117); SetMeshSize
+   * reallocates the tables zeroed (space_charge.cpp:61-73) */
+  if (s->s.mesh_w != 0.0) {
+    if (s->s.nx_init < 2 || s->s.nz_init < 2)
+      return set_err("mesh_size_cutoff_w needs a mesh of at least 2 x 2");
+    if (ke > s->s.mesh_w && (s->s.nx != s->s.nx_init / 2 || s->s.nz != s->s.nz_init / 2))
+      orc_spch_set_mesh_size(s, s->s.nx_init / 2, s->s.nz_init / 2);
+    else if (ke <= s->s.mesh_w && (s->s.nx != s->s.nx_init || s->s.nz != s->s.nz_init))
+      orc_spch_set_mesh_size(s, s->s.nx_init, s->s.nz_init);
+    if (ke > s->s.mesh_w && s->s.interval != s->s.interval_init * 4)
+      s->s.interval = s->s.interval_init * 4;
+    else if (ke <= s->s.mesh_w && s->s.interval != s->s.interval_init)
+      s->s.interval = s->s.interval_init;
+  }
   /* SetParameters, space_charge.cu:77-86 */
   s->freq = b->freq; s->beta = beta; s->frac = s->s.fraction; s->adj_eff = adj_bunch;
   s->s.kick_count += 1;

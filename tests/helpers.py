@@ -19,7 +19,13 @@ def workload(idx=0, num=4096, periods=4, current=None, sc=True, seed=None):
     return wl
 
 
-def run_oracle(wl, elems=None, dist=None, adj=0, adj_w=0.0):
+def _configure(sc, opts):
+    """SpaceCharge setters by name, e.g. {"mesh_size_cutoff_w": 1.2}."""
+    for k, v in (opts or {}).items():
+        getattr(sc, "set_" + k)(v)
+
+
+def run_oracle(wl, elems=None, dist=None, adj=0, adj_w=0.0, sc_opts=None):
     from oracle import oracle as O
     b = wl.beam
     elems = elems if elems is not None else cfg.build_lattice(wl.lattice)
@@ -30,12 +36,14 @@ def run_oracle(wl, elems=None, dist=None, adj=0, adj_w=0.0):
     sc = O.SpaceCharge(wl.nr, wl.nz, wl.interval, adj) if wl.space_charge else None
     if sc is not None and adj_w:
         sc.set_adj_bunch_cutoff_w(adj_w)
+    if sc is not None:
+        _configure(sc, sc_opts)
     sim = O.Simulator(beam, elems, sc)
     sim.simulate()
     return beam, sc, sim
 
 
-def run_gpu(wl, elems=None, dist=None, adj=0, adj_w=0.0):
+def run_gpu(wl, elems=None, dist=None, adj=0, adj_w=0.0, sc_opts=None):
     from hpsim_amd import HPSim
     b = wl.beam
     elems = elems if elems is not None else cfg.build_lattice(wl.lattice)
@@ -48,6 +56,8 @@ def run_gpu(wl, elems=None, dist=None, adj=0, adj_w=0.0):
           if wl.space_charge else None)
     if sc is not None and adj_w:
         sc.set_adj_bunch_cutoff_w(adj_w)
+    if sc is not None:
+        _configure(sc, sc_opts)
     sim = HPSim.Simulator(beam, bl, sc)
     sim.simulate()
     return beam, sc, sim

@@ -269,3 +269,24 @@ def test_adjacent_bunch_space_charge_bit_exact(num, adj, adj_w):
     if num > 2:
         for name, g, o in zip(("greens", "field", "bins"), gsc.tables(), osc.tables()):
             assert np.array_equal(g.view(np.uint64), o.view(np.uint64)), name
+
+
+@pytest.mark.parametrize("remesh", [None, 1e9])
+def test_mesh_size_cutoff_bit_exact(remesh):
+    """mesh_size_cutoff_w: once the beam passes 1.27 MeV the mesh halves and
+    the drift kick interval quadruples, kick point by kick point.  With
+    remeshing switched off the fresh (zeroed) half-size tables are what the
+    later kicks read, as after the reference's SetMeshSize."""
+    wl = H.workload(0, num=4096, periods=6, sc=True)
+    wl.interval = 0.01
+    opts = {"mesh_size_cutoff_w": 1.27}
+    if remesh is not None:
+        opts["remesh_threshold"] = remesh
+    ob, osc, _ = H.run_oracle(wl, sc_opts=opts)
+    gb, gsc, _ = H.run_gpu(wl, sc_opts=opts)
+    err = H.compare_arrays(gb.arrays(), ob.arrays())
+    assert err is None, err
+    gs, os_ = gsc.state(), osc.state()
+    for k in ("remesh_count", "kick_count", "num_on_mesh", "dr", "dz", "nx", "nz", "interval"):
+        assert gs[k] == os_[k], (k, gs[k], os_[k])
+    assert gs["nx"] == wl.nr // 2 and gs["interval"] == 0.04

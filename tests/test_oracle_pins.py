@@ -317,6 +317,43 @@ def test_adjacent_bunch_mesh_and_energy_cutoff(oracle_lib):
     assert H.compare_arrays(b3.arrays(), b4.arrays()) is None
 
 
+def test_mesh_size_cutoff_follows_beam_energy(oracle_lib):
+    """mesh_size_cutoff_w (space_charge.cpp:107-117): at each kick, a beam
+    whose good mean energy is above the cutoff gets half the mesh in each
+    direction and four times the kick interval; a drift reads the interval
+    when it starts (simulation_engine.cu:128-130).  Observed element by
+    element: the kick count of every drift, the mesh after every element."""
+    O = oracle_lib
+    n, mw, iv = 2048, 1.27, 0.01
+    wl = cfg.config(0, num=n, periods=6)
+    els = cfg.build_lattice(wl.lattice)
+    b = O.Beam(n, 939.294, -1.0, 0.01)
+    b.set_distribution(*cfg.make_distribution(wl.beam))
+    b.set_frequency(201.25)
+    b.set_ref_w(0.75)
+    sc = O.SpaceCharge(32, 128, iv)
+    sc.set_mesh_size_cutoff_w(mw)
+    sim = O.Simulator(b, els, sc)
+    ke = None           # good mean energy at the last kick
+    crossed = False
+    for i, e in enumerate(els):
+        before = sc.state()
+        sim.simulate(i, i)
+        after = sc.state()
+        kicks = after["kick_count"] - before["kick_count"]
+        if e.kind == "Drift":
+            want_iv = iv * 4 if (ke is not None and ke > mw) else iv
+            assert before["interval"] == want_iv
+            assert kicks == math.ceil(e.length / want_iv)
+        else:
+            assert kicks == 1
+        ke = b.stats()["w_avg_good"]
+        big = ke > mw
+        crossed |= big
+        assert (after["nx"], after["nz"]) == ((16, 64) if big else (32, 128))
+    assert crossed and ke > mw
+
+
 def test_field_sum_brute_force(L):
     """Canonical field sum vs a plain double loop of charge x Green's value."""
     nr, nz = 4, 8
