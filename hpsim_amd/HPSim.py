@@ -488,6 +488,32 @@ def probe_math(fn, x, y=None):
     return out
 
 
+def set_db_epics(pv_name, value, dbconnection, beamline):
+    """set_db_epics(pv, value, DBConnection, BeamLine), hpsim_module.cpp:52-82:
+    writes the PV into the database (whose triggers recompute the model
+    fields) and updates the elements that follow the PV."""
+    from . import pv
+    pv.set_db_epics(pv_name, value, dbconnection, beamline)
+
+
+def get_db_epics(pv_name, dbconnection):
+    """get_db_epics(pv, DBConnection) -> value text or None, hpsim_module.cpp:267-307."""
+    from . import pv
+    return pv.get_db_epics(pv_name, dbconnection)
+
+
+def set_db_model(record_name, field_name, value, dbconnection, beamline):
+    """set_db_model(record, field, value, DBConnection, BeamLine), hpsim_module.cpp:89-265."""
+    from . import pv
+    pv.set_db_model(record_name, field_name, value, dbconnection, beamline)
+
+
+def get_db_model(record_name, field_name, dbconnection):
+    """get_db_model(record, field, DBConnection) -> value text or None, hpsim_module.cpp:309-366."""
+    from . import pv
+    return pv.get_db_model(record_name, field_name, dbconnection)
+
+
 def set_gpu(n):
     """set_gpu(n), hpsim_module.cpp / init.cpp:12-18."""
     check(_lib().hps_set_device(int(n)))

@@ -5,6 +5,7 @@ import fails, because every tracking call must run on the GPU kernels.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 
@@ -153,7 +154,17 @@ def lib():
         L = C.CDLL(LIB_PATH)
         _declare(L)
         _lib = L
+        atexit.register(_at_exit)
     return _lib
+
+
+def _at_exit():
+    """Objects still alive when the interpreter shuts down (e.g. held by a
+    traceback) are not destroyed through the engine: the process exit
+    releases their device memory, and the HIP runtime may already be
+    finishing."""
+    global _lib
+    _lib = None
 
 
 class HPSimError(RuntimeError):

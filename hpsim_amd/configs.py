@@ -138,8 +138,8 @@ GAP_FIELDS = ("length", "frequency", "cell_length_over_beta_lambda", "amplitude"
 class Element:
     """One beamline element (reference beamline_element.h).  ``kind`` is the
     reference type string: Drift, Quad, RFGap-DTL, RFGap-CCL, ApertureC,
-    ApertureR, SpchComp, Diagnostics, Rotation, Steerer (the engine's maps),
-    or Buncher / Dipole (read from lattice databases, not tracked yet)."""
+    ApertureR, SpchComp, Diagnostics, Rotation, Steerer, Buncher, Dipole.
+    Buncher and Dipole model fields are in ``params``."""
     name: str
     kind: str
     length: float = 0.0

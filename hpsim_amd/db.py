@@ -4,7 +4,10 @@ The reference keeps its lattices in SQLite files (one per beamline section,
 e.g. tbtd.db, dtl.db, trst.db, ccl.db) and builds the beamline by walking
 each database's ``linac`` view (src/beamline/src/init.cpp:20-429).  This
 module does the same with Python's sqlite3 and returns element records
-(hpsim_amd.configs.Element) for hpsim_amd.HPSim.BeamLine.
+(hpsim_amd.configs.Element) for hpsim_amd.HPSim.BeamLine.  Like the
+reference (GetDataFromDB + std::atof), every field is read as SQLite's text
+form of the value (15 significant digits for a REAL) and converted to double
+from that text.
 
 The databases' triggers use SQL math functions that the reference loads
 from a compiled SQLite extension (DBConnection::LoadLib,
@@ -151,7 +154,7 @@ def beamline_elements(dbc: DBConnection):
             table = None
 
             def col(tbl, c):
-                return _f(_one(conn, f"select {_ident(c)} from {d}.{_ident(tbl)} "
+                return _f(_one(conn, f"select cast({_ident(c)} as text) from {d}.{_ident(tbl)} "
                                      "where view_index = ?", (view_index,)))
 
             if mtype == "quad":
@@ -166,9 +169,11 @@ def beamline_elements(dbc: DBConnection):
             elif mtype in ("dtl_gap", "ccl_gap"):
                 table = "rf_gap"
                 key = "cell_id" if mtype == "dtl_gap" else "tank_id"
+                cols = ", ".join(f"cast(t1.{c} as text)" for c in (
+                    "beta_g", "beta_min", "ta0", "ta1", "ta2", "ta3", "ta4", "ta5", "sa1", "sa2",
+                    "sa3", "sa4", "sa5"))
                 ttf = conn.execute(
-                    f"select t1.beta_g, t1.beta_min, t1.ta0, t1.ta1, t1.ta2, t1.ta3, t1.ta4, "
-                    f"t1.ta5, t1.sa1, t1.sa2, t1.sa3, t1.sa4, t1.sa5 from {d}.transit_time_factor t1 "
+                    f"select {cols} from {d}.transit_time_factor t1 "
                     f"join {d}.rf_gap t2 on t1.module_id = t2.module_id and t1.tank_id = t2.{key} "
                     "where t2.view_index = ?", (view_index,)).fetchone()
                 if ttf is None:
@@ -181,7 +186,8 @@ def beamline_elements(dbc: DBConnection):
                 g["length"] = col(table, "length_model")
                 g["frequency"] = col(table, "frequency_model")
                 g["cell_length_over_beta_lambda"] = _f(_one(
-                    conn, f"select cell_length_betalambda_design from {d}.rf_module where id = "
+                    conn, f"select cast(cell_length_betalambda_design as text) from {d}.rf_module "
+                          "where id = "
                           f"(select module_id from {d}.rf_gap where view_index = ?)",
                     (view_index,)))
                 g["amplitude"] = col(table, "amplitude_model")

@@ -18,6 +18,7 @@ import math
 import numpy as np
 
 from . import HPSim
+from . import lcsutil as lcs
 
 _COORDINATES = ["x", "xp", "y", "yp", "phi", "w"]
 _EMITTANCE = ["x", "y", "z"]
@@ -451,6 +452,36 @@ def get_element_list(start_elem_name=None, end_elem_name=None, elem_type=None):
     (hpsim.py:1849-1883)."""
     t = _ELEM_TYPES.get(elem_type, elem_type) if elem_type else ""
     return BeamLine.beamline.get_element_names(start_elem_name or "", end_elem_name or "", t)
+
+
+def set_db_epics(pv_name, value):
+    """Change an EPICS PV value in the database and the beamline model
+    (hpsim.py:1783-1793); DBConnection and BeamLine must exist."""
+    HPSim.set_db_epics(pv_name, str(value), DBConnection.dbconnection, BeamLine.beamline)
+
+
+def get_db_epics(pv_name):
+    """An EPICS PV value from the database (hpsim.py:1795-1805): a float,
+    except for PVs of LCS channel type 'L'."""
+    value = HPSim.get_db_epics(pv_name, DBConnection.dbconnection)
+    if lcs.get_pv_type(pv_name) != "L":
+        value = float(value)
+    return value
+
+
+def set_db_model(table_name, field_name, value):
+    """Change a model field of a database record (hpsim.py:1807-1817)."""
+    HPSim.set_db_model(table_name, field_name, str(value), DBConnection.dbconnection,
+                       BeamLine.beamline)
+
+
+def get_db_model(elem_name, field_name):
+    """A model field of a database record (hpsim.py:1819-1832): a float, except
+    for the text fields id, name and model_type."""
+    value = HPSim.get_db_model(elem_name, field_name, DBConnection.dbconnection)
+    if field_name not in ("id", "name", "model_type"):
+        value = float(value)
+    return value
 
 
 def get_beamline_direction(start_elem, stop_elem):
