@@ -476,6 +476,65 @@ class Simulator:
         return v.value
 
 
+class Batch:
+    """Several Simulators run as one launch sequence (hps_batch_*): the PV
+    scan of pytest/sim-lbeg-scan-pv.py:140-160, one simulate per machine
+    setting, with every kernel launch covering all settings.  Results are
+    those of separate Simulator.simulate calls.  The simulators' lattices
+    must share one layout (element values may differ) and one SCHEFF mesh
+    size."""
+
+    _FAMILIES = ("segment", "phase", "sc_build", "field", "segment_final", "phase_final")
+
+    def __init__(self, simulators=()):
+        h = C.c_void_p()
+        check(_lib().hps_batch_create(C.byref(h)))
+        self._h = h
+        self.simulators = []
+        for m in simulators:
+            self.add(m)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and N._lib is not None:
+            N._lib.hps_batch_destroy(h)
+            self._h = None
+
+    def add(self, simulator):
+        if not isinstance(simulator, Simulator):
+            raise TypeError("a Batch holds Simulators")
+        check(_lib().hps_batch_add(self._h, simulator._h))
+        self.simulators.append(simulator)
+
+    def __len__(self):
+        return len(self.simulators)
+
+    def simulate(self, *args):
+        """simulate() or simulate(start_element, end_element) of every member."""
+        if len(args) == 0:
+            i0, i1 = -1, -1
+        elif len(args) == 2:
+            bl = self.simulators[0].beamline
+            i0, i1 = (a if isinstance(a, int) else bl.index(a) for a in args)
+        else:
+            raise TypeError("simulate takes either 0 or 2 arguments")
+        check(_lib().hps_batch_simulate(self._h, i0, i1))
+
+    def set_profiling(self, on=True):
+        check(_lib().hps_batch_set_profiling(self._h, 1 if on else 0))
+
+    def profile(self):
+        ms = (C.c_double * len(self._FAMILIES))()
+        n = (C.c_uint32 * len(self._FAMILIES))()
+        check(_lib().hps_batch_get_profile(self._h, ms, n))
+        return {self._FAMILIES[k]: (ms[k], n[k]) for k in range(len(self._FAMILIES))}
+
+    def last_time_ms(self):
+        v = C.c_double()
+        check(_lib().hps_batch_last_time_ms(self._h, C.byref(v)))
+        return v.value
+
+
 def probe_math(fn, x, y=None):
     """Evaluate one of the engine's device math routines over x (for tests)."""
     codes = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "sinh": 4, "cosh": 5, "sqrt": 6,

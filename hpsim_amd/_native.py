@@ -77,6 +77,9 @@ EXPORTS = (
     "hps_sim_create", "hps_sim_destroy", "hps_sim_simulate", "hps_sim_reset",
     "hps_sim_set_space_charge", "hps_sim_last_time_ms", "hps_sim_set_profiling",
     "hps_sim_get_profile", "hps_sim_set_graph", "hps_sim_graph_replays", "hps_probe_math",
+    "hps_batch_create", "hps_batch_destroy", "hps_batch_add", "hps_batch_size",
+    "hps_batch_simulate", "hps_batch_last_time_ms", "hps_batch_set_profiling",
+    "hps_batch_get_profile",
 )
 
 _lib = None
@@ -137,6 +140,14 @@ def _declare(L):
         "hps_sim_set_graph": (C.c_int, [P, C.c_int]),
         "hps_sim_graph_replays": (C.c_int, [P, up]),
         "hps_probe_math": (C.c_int, [C.c_int, dp, dp, dp, u32]),
+        "hps_batch_create": (C.c_int, [PP]),
+        "hps_batch_destroy": (C.c_int, [P]),
+        "hps_batch_add": (C.c_int, [P, P]),
+        "hps_batch_size": (u32, [P]),
+        "hps_batch_simulate": (C.c_int, [P, i32, i32]),
+        "hps_batch_last_time_ms": (C.c_int, [P, dp]),
+        "hps_batch_set_profiling": (C.c_int, [P, C.c_int]),
+        "hps_batch_get_profile": (C.c_int, [P, dp, up]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -288,3 +288,54 @@ def config(idx: int, num: int | None = None, periods: int | None = None) -> Work
                           scale_gradient_with_bg=True)
         return Workload("plain-fp64-1048576x200", b, lat)
     raise ValueError(f"unknown single-beam configuration {idx}")
+
+
+@dataclass
+class BatchWorkload:
+    """Batched scan: one lattice layout, per-beam settings and sizes."""
+    name: str
+    beams: list            # BeamSpec per member
+    lattices: list         # element list per member
+    nr: int = 32
+    nz: int = 128
+    interval: float = 0.05
+    space_charge: bool = True
+
+
+def perturb_lattice(elems, seed, grad=0.05, volt=0.02, phase_deg=3.0):
+    """A copy of elems with every quad gradient scaled by U(1 - grad, 1 + grad),
+    every gap amplitude by U(1 - volt, 1 + volt) and every gap RF phase
+    (phase_shift) moved by U(-phase_deg, phase_deg) degrees, from Philox
+    uniforms of stream 16 + k for the k-th setting of each element."""
+    import copy
+    n = len(elems)
+    u = uniform(3 * n, 16, seed).reshape(n, 3)
+    out = []
+    for i, e in enumerate(elems):
+        e = copy.deepcopy(e)
+        if e.kind == "Quad":
+            e.gradient *= 1.0 + grad * (2.0 * u[i, 0] - 1.0)
+        elif e.kind.startswith("RFGap"):
+            e.gap = dict(e.gap)
+            e.gap["amplitude"] *= 1.0 + volt * (2.0 * u[i, 1] - 1.0)
+            e.gap["phase_shift"] += math.radians(phase_deg) * (2.0 * u[i, 2] - 1.0)
+        out.append(e)
+    return out
+
+
+def batched_scan(nbeams=256, nmin=16384, nmax=1048576, periods=40, seed=7):
+    """The batched-scan configuration: nbeams beams of the 65,536-particle
+    configuration's distribution law with log-uniform seeded particle counts
+    in [nmin, nmax], each on the 40-period lattice with its own seeded
+    perturbations (quad gradients +-5 %, gap voltages +-2 %, RF phases
+    +-3 degrees)."""
+    base = config(0, periods=periods)
+    lattice = build_lattice(base.lattice)
+    u = uniform(nbeams, 15, seed)
+    nums = np.exp(np.log(nmin) + (np.log(nmax) - np.log(nmin)) * u).astype(np.int64)
+    beams, lats = [], []
+    for k in range(nbeams):
+        b = BeamSpec(num=int(nums[k]), seed=seed * 100003 + k + 1)
+        beams.append(b)
+        lats.append(perturb_lattice(lattice, seed * 7919 + k + 1))
+    return BatchWorkload(f"batched-scan-{nbeams}x{periods}", beams, lats)

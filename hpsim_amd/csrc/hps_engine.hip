@@ -1513,6 +1513,376 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
   return 0;
 }
 
+// ---- Batch: several simulations, one grid per launch ----
+
+struct hps_batch {
+  std::vector<hps_sim*> sims;
+  std::vector<uint8_t> host_rec;      // launch records + block maps of the last call
+  uint8_t* d_rec = nullptr;
+  size_t rec_cap = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+  bool profile = false;
+  std::vector<hipEvent_t> evpool;
+  double prof_ms[HPS_PROFILE_FAMILIES] = {};
+  uint32_t prof_n[HPS_PROFILE_FAMILIES] = {};
+};
+
+extern "C" int hps_batch_create(hps_batch** out)
+{
+  if (!out) return fail("null output handle");
+  hps_batch* bt = new hps_batch();
+  if (hipEventCreate(&bt->ev0) != hipSuccess || hipEventCreate(&bt->ev1) != hipSuccess) {
+    delete bt;
+    return fail("event creation failed");
+  }
+  *out = bt;
+  return 0;
+}
+
+extern "C" int hps_batch_destroy(hps_batch* bt)
+{
+  if (!bt) return 0;
+  if (!bt->sims.empty()) (void)hipStreamSynchronize(bt->sims[0]->b->stream);
+  for (hipEvent_t e : bt->evpool) (void)hipEventDestroy(e);
+  (void)hipFree(bt->d_rec);
+  (void)hipEventDestroy(bt->ev0);
+  (void)hipEventDestroy(bt->ev1);
+  delete bt;
+  return 0;
+}
+
+extern "C" int hps_batch_add(hps_batch* bt, hps_sim* m)
+{
+  if (!bt || !m) return fail("null argument");
+  if (bt->sims.size() >= (1u << (32 - kBatchLocalBits)) - 1) return fail("batch is full");
+  for (hps_sim* o : bt->sims) {
+    if (o == m) return fail("simulator is already in the batch");
+    if (o->b == m->b) return fail("two simulators of a batch share one beam");
+    if (m->sc && o->sc == m->sc) return fail("two simulators of a batch share one SpaceCharge");
+  }
+  if (!bt->sims.empty()) {
+    const hps_beam* b0 = bt->sims[0]->b;
+    if (m->b->device != b0->device) return fail("the simulators of a batch must share one device");
+  }
+  bt->sims.push_back(m);
+  return 0;
+}
+
+extern "C" uint32_t hps_batch_size(const hps_batch* bt) { return bt ? (uint32_t)bt->sims.size() : 0; }
+
+extern "C" int hps_batch_set_profiling(hps_batch* bt, int on)
+{
+  if (!bt) return fail("null batch");
+  bt->profile = on != 0;
+  for (int k = 0; k < HPS_PROFILE_FAMILIES; ++k) { bt->prof_ms[k] = 0; bt->prof_n[k] = 0; }
+  return 0;
+}
+
+extern "C" int hps_batch_get_profile(hps_batch* bt, double* ms, uint32_t* launches)
+{
+  if (!bt || !ms || !launches) return fail("null argument");
+  for (int k = 0; k < HPS_PROFILE_FAMILIES; ++k) { ms[k] = bt->prof_ms[k]; launches[k] = bt->prof_n[k]; }
+  return 0;
+}
+
+extern "C" int hps_batch_last_time_ms(hps_batch* bt, double* out)
+{
+  if (!bt || !out) return fail("null argument");
+  *out = bt->last_ms;
+  return 0;
+}
+
+// simulate(start, end) of every simulation of the batch, with results
+// identical to separate hps_sim_simulate calls.  Every simulation is compiled
+// on its own; the programs must have the same structure (the same maps in
+// the same segments and kick points: the same lattice layout, element
+// values may differ), the same SCHEFF mesh and no mesh-size cutoff.  Each
+// launch then covers all simulations: the segment and phase kernels over
+// the concatenated chunks, one workgroup per simulation for the
+// cross-chunk steps, the rebuild and field grids side by side.
+extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
+{
+  if (!bt) return fail("null batch");
+  const size_t B = bt->sims.size();
+  if (B == 0) return 0;
+  hps_sim* m0 = bt->sims[0];
+  const uint32_t nel = (uint32_t)m0->bl->el.size();
+  if (nel == 0) return 0;
+  const int32_t start_index = start < 0 ? 0 : start;
+  const int32_t end_index = end < 0 ? (int32_t)nel - 1 : end;
+  if (end_index >= (int32_t)nel || start_index >= (int32_t)nel) return fail("element index out of range");
+  const bool sc_on = m0->sc_on && m0->sc;
+  for (hps_sim* m : bt->sims) {
+    if (m->bl->el.size() != nel) return fail("the beamlines of a batch must have the same elements");
+    if ((m->sc_on && m->sc) != sc_on) return fail("space charge must be on for all or none of a batch");
+    if (m->b->stream != m0->b->stream) return fail("the simulators of a batch must share one device");
+    if (sc_on) {
+      const hps_spch_state& cf = m->sc->cfg;
+      if (cf.mesh_w != 0.0) return fail("mesh_size_cutoff_w is not supported in a batch");
+      if (cf.nx != m0->sc->cfg.nx || cf.nz != m0->sc->cfg.nz)
+        return fail("the SCHEFF meshes of a batch must have the same size");
+      if ((size_t)cf.nx * cf.nz * 8 + kBlockLdsReserve > m->sc->max_lds)
+        return fail("the SCHEFF mesh has more bins than one workgroup's LDS histogram holds");
+    }
+  }
+  // compile every simulation (restoring the simulators on a structure error)
+  struct Saved { int32_t prev_end; uint32_t ccl; double dw, wl; };
+  std::vector<Saved> saved(B);
+  for (size_t k = 0; k < B; ++k) {
+    hps_sim* m = bt->sims[k];
+    saved[k] = {m->prev_end, m->ccl_cell_num, m->design_w, m->wavelen};
+  }
+  auto restore = [&]() {
+    for (size_t k = 0; k < B; ++k) {
+      hps_sim* m = bt->sims[k];
+      m->prev_end = saved[k].prev_end; m->ccl_cell_num = saved[k].ccl;
+      m->design_w = saved[k].dw; m->wavelen = saved[k].wl;
+    }
+  };
+  std::vector<Compiler> cs(B);
+  for (size_t k = 0; k < B; ++k) {
+    hps_sim* m = bt->sims[k];
+    hps_beam* b = m->b;
+    if (start_index < m->prev_end) hps_sim_reset(m);
+    m->prev_end = end_index;
+    Compiler& c = cs[k];
+    c.m = m;
+    c.frac = m->sc ? m->sc->cfg.fraction : 1.0;
+    c.freq = b->freq;
+    c.quad_gap_run = b->blck_size / 2 != 0;
+    c.dipole_run = b->blck_size / 4 != 0;
+    c.sc_stats = b->n > 2;
+    for (uint32_t i = 0; (int32_t)i <= end_index; ++i)
+      if ((int32_t)i >= start_index || m->bl->el[i].kind == HPS_SPCH_COMP)
+        if (compile_element(c, i)) { restore(); return -1; }
+    c.close(STAT_NONE, false, 0);
+  }
+  const Compiler& c0 = cs[0];
+  for (size_t k = 1; k < B; ++k) {
+    const Compiler& c = cs[k];
+    bool same = c.steps.size() == c0.steps.size() && c.ops.size() == c0.ops.size() &&
+                c.sc_stats == c0.sc_stats && c.quad_gap_run == c0.quad_gap_run &&
+                c.dipole_run == c0.dipole_run;
+    for (size_t i = 0; same && i < c.steps.size(); ++i) {
+      const Step &a = c.steps[i], &o = c0.steps[i];
+      same = a.kind == o.kind && a.op_begin == o.op_begin && a.op_end == o.op_end &&
+             a.kick == o.kick && a.stats == o.stats && a.kick_point == o.kick_point;
+    }
+    for (size_t i = 0; same && i < c.ops.size(); ++i) same = c.ops[i].kind == c0.ops[i].kind;
+    if (!same) {
+      restore();
+      return fail("the simulations of a batch compile to different programs (lattice layout, "
+                  "element kinds or beam sizes differ)");
+    }
+  }
+  hipStream_t s = m0->b->stream;
+  for (size_t k = 0; k < B; ++k) {
+    hps_sim* m = bt->sims[k];
+    const Compiler& c = cs[k];
+    if (c.ops.size() > m->ops_cap) {
+      HPS_CHECK(hipStreamSynchronize(s));
+      (void)hipFree(m->d_ops);
+      m->d_ops = nullptr;
+      m->ops_cap = c.ops.size() + 64;
+      HPS_CHECK(hipMalloc(&m->d_ops, m->ops_cap * sizeof(Op)));
+    }
+    if (!c.ops.empty())
+      HPS_CHECK(hipMemcpyAsync(m->d_ops, c.ops.data(), c.ops.size() * sizeof(Op),
+                               hipMemcpyHostToDevice, s));
+  }
+
+  // record layout: chunk map, rebuild map, rebuild and field records, then
+  // per launch the records of its step
+  const uint32_t nr = sc_on ? m0->sc->cfg.nx : 0, nz = sc_on ? m0->sc->cfg.nz : 0;
+  std::vector<uint32_t> chunk_map, build_map;
+  for (size_t k = 0; k < B; ++k) {
+    const uint32_t nch = bt->sims[k]->b->pr.nch;
+    for (uint32_t c = 0; c < nch; ++c) chunk_map.push_back(((uint32_t)k << kBatchLocalBits) | c);
+  }
+  std::vector<BuildArgs> build(B);
+  std::vector<FieldArgs> field(B);
+  if (sc_on) {
+    const uint32_t gb = greens_blocks(nr, nz);
+    for (size_t k = 0; k < B; ++k) {
+      hps_sim* m = bt->sims[k];
+      BuildArgs& a = build[k];
+      a.sc = m->sc->dev; a.fld1 = m->sc->fld1; a.b = m->b->d; a.st = m->b->st; a.bins = m->sc->bins;
+      a.greens_blocks = gb;
+      a.dep_blocks = m->b->blck_size / 4 != 0 ? deposit_blocks(m->b->n) : 0;
+      for (uint32_t c = 0; c < gb + a.dep_blocks; ++c)
+        build_map.push_back(((uint32_t)k << kBatchLocalBits) | c);
+      FieldArgs& f = field[k];
+      f.sc = m->sc->dev; f.bins = m->sc->bins; f.fld1 = m->sc->fld1; f.fld2 = m->sc->fld2;
+    }
+  }
+  auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  size_t off = 0;
+  const size_t off_chunk = off; off = align(off + chunk_map.size() * 4);
+  const size_t off_bmap = off; off = align(off + build_map.size() * 4);
+  const size_t off_build = off; off = align(off + B * sizeof(BuildArgs));
+  const size_t off_field = off; off = align(off + B * sizeof(FieldArgs));
+  size_t nseg = 0, nkp = 0;
+  for (const Step& st : c0.steps) {
+    if (st.kind != 0) continue;
+    if (st.op_end > st.op_begin || st.kick || st.stats != STAT_NONE) ++nseg;
+    if (st.kick_point) ++nkp;
+  }
+  const size_t off_steps = off;
+  off = align(off + nseg * B * sizeof(SegArgs) + nkp * B * sizeof(PhaseArgs) + 256);
+  if (off > bt->rec_cap) {
+    HPS_CHECK(hipStreamSynchronize(s));
+    (void)hipFree(bt->d_rec);
+    bt->d_rec = nullptr;
+    bt->rec_cap = off + off / 4;
+    HPS_CHECK(hipMalloc(&bt->d_rec, bt->rec_cap));
+  }
+  std::vector<uint8_t>& H = bt->host_rec;
+  H.assign(off, 0);
+  std::memcpy(H.data() + off_chunk, chunk_map.data(), chunk_map.size() * 4);
+  std::memcpy(H.data() + off_bmap, build_map.data(), build_map.size() * 4);
+  std::memcpy(H.data() + off_build, build.data(), B * sizeof(BuildArgs));
+  std::memcpy(H.data() + off_field, field.data(), B * sizeof(FieldArgs));
+  const uint32_t* d_chunk_map = (const uint32_t*)(bt->d_rec + off_chunk);
+  const uint32_t* d_build_map = (const uint32_t*)(bt->d_rec + off_bmap);
+  const BuildArgs* d_build = (const BuildArgs*)(bt->d_rec + off_build);
+  const FieldArgs* d_field = (const FieldArgs*)(bt->d_rec + off_field);
+  const uint32_t total_chunks = (uint32_t)chunk_map.size();
+  const uint32_t build_blocks = (uint32_t)build_map.size();
+
+  enum { L_SEG = 0, L_PHASE, L_BUILD, L_FIELD, L_SEG_FINAL, L_PHASE_FINAL };
+  struct BLaunch {
+    int kind;
+    size_t rec;        // byte offset of the records
+    int op_class;
+  };
+  std::vector<BLaunch> launches;
+  size_t cursor = off_steps;
+  size_t uploaded = 0;         // bytes of H already on the device
+  std::vector<std::pair<int, size_t>> marks;
+  size_t nev = 0;
+  auto flush = [&]() -> int {
+    if (cursor > uploaded) {
+      HPS_CHECK(hipMemcpyAsync(bt->d_rec + uploaded, H.data() + uploaded, cursor - uploaded,
+                               hipMemcpyHostToDevice, s));
+      uploaded = cursor;
+    }
+    for (const BLaunch& l : launches) {
+      if (bt->profile) {
+        while (bt->evpool.size() < nev + 2) {
+          hipEvent_t e;
+          HPS_CHECK(hipEventCreate(&e));
+          bt->evpool.push_back(e);
+        }
+        marks.push_back({l.kind, nev});
+        HPS_CHECK(hipEventRecord(bt->evpool[nev], s));
+      }
+      const void* rec = bt->d_rec + l.rec;
+      switch (l.kind) {
+        case L_SEG:
+          HPS_CHECK(launch_segment_batch((const SegArgs*)rec, d_chunk_map, total_chunks,
+                                         l.op_class, s));
+          break;
+        case L_SEG_FINAL:
+          HPS_CHECK(launch_seg_final_batch((const SegArgs*)rec, (uint32_t)B, s));
+          break;
+        case L_PHASE:
+          HPS_CHECK(launch_phase_batch((const PhaseArgs*)rec, d_chunk_map, total_chunks, s));
+          break;
+        case L_PHASE_FINAL:
+          HPS_CHECK(launch_phase_final_batch((const PhaseArgs*)rec, (uint32_t)B, s));
+          break;
+        case L_BUILD:
+          HPS_CHECK(launch_sc_build_batch(d_build, d_build_map, build_blocks, nr, nz, s));
+          break;
+        default:
+          HPS_CHECK(launch_field_batch(d_field, (uint32_t)B, nr, nz, m0->sc->max_lds, s));
+          break;
+      }
+      if (bt->profile) {
+        HPS_CHECK(hipEventRecord(bt->evpool[nev + 1], s));
+        nev += 2;
+      }
+    }
+    launches.clear();
+    return 0;
+  };
+
+  HPS_CHECK(hipEventRecord(bt->ev0, s));
+  // the records of the host beam state (current, frequency) are built after
+  // every host step that changes it
+  for (size_t i = 0; i < c0.steps.size(); ++i) {
+    const Step& st0 = c0.steps[i];
+    if (st0.kind == 1) {
+      if (flush()) return -1;
+      for (size_t k = 0; k < B; ++k)
+        if (change_frequency(bt->sims[k]->b, cs[k].steps[i].new_freq)) return -1;
+      continue;
+    }
+    if (st0.op_end > st0.op_begin || st0.kick || st0.stats != STAT_NONE) {
+      SegArgs* recs = (SegArgs*)(H.data() + cursor);
+      for (size_t k = 0; k < B; ++k) {
+        hps_sim* m = bt->sims[k];
+        hps_beam* b = m->b;
+        const Step& st = cs[k].steps[i];
+        const uint32_t kstride = b->grid_size * (b->blck_size / 4);
+        SegArgs& a = recs[k];
+        a.b = b->d;
+        a.st = b->st;
+        a.pr = b->pr;
+        a.ops = m->d_ops;
+        a.op_begin = st.op_begin;
+        a.op_end = st.op_end;
+        a.kick = (st.kick && kstride != 0 && m->sc) ? 1 : 0;
+        a.kick_mask = kstride ? kstride - 1 : 0;
+        a.kick_len = st.kick_len;
+        a.sc = m->sc ? m->sc->dev : nullptr;
+        a.fld2 = m->sc ? m->sc->fld2 : nullptr;
+        a.stats = st.stats;
+        a.rmax = (st.stats == STAT_SC && m->sc && m->sc->cfg.adj_bunch != 0) ? 1 : 0;
+        a.op_class = OPC_LIGHT;
+        for (int32_t o = st.op_begin; o < st.op_end; ++o)
+          a.op_class = std::max(a.op_class, op_class_of(cs[k].ops[o].kind));
+      }
+      launches.push_back({L_SEG, cursor, recs[0].op_class});
+      if (st0.stats != STAT_NONE) launches.push_back({L_SEG_FINAL, cursor, 0});
+      cursor += B * sizeof(SegArgs);
+    }
+    if (st0.kick_point) {
+      PhaseArgs* recs = (PhaseArgs*)(H.data() + cursor);
+      for (size_t k = 0; k < B; ++k) {
+        hps_sim* m = bt->sims[k];
+        hps_beam* b = m->b;
+        PhaseArgs& p = recs[k];
+        p.b = b->d; p.st = b->st; p.pr = b->pr; p.sc = m->sc->dev; p.bins = m->sc->bins;
+        p.do_sums = b->n > 2 ? 1 : 0;
+        p.rmax = m->sc->cfg.adj_bunch == 0 ? 0 : (cs[k].sc_stats ? 1 : 2);
+        p.frac = cs[k].steps[i].frac; p.current = b->current; p.freq = b->freq;
+      }
+      if (c0.sc_stats) launches.push_back({L_PHASE, cursor, 0});
+      launches.push_back({L_PHASE_FINAL, cursor, 0});
+      launches.push_back({L_BUILD, 0, 0});
+      launches.push_back({L_FIELD, 0, 0});
+      cursor += B * sizeof(PhaseArgs);
+    }
+  }
+  if (flush()) return -1;
+  HPS_CHECK(hipEventRecord(bt->ev1, s));
+  HPS_CHECK(hipEventSynchronize(bt->ev1));
+  float ms = 0;
+  HPS_CHECK(hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
+  bt->last_ms = ms;
+  for (hps_sim* m : bt->sims) m->last_ms = ms;
+  for (const auto& mk : marks) {
+    float t = 0;
+    HPS_CHECK(hipEventElapsedTime(&t, bt->evpool[mk.second], bt->evpool[mk.second + 1]));
+    bt->prof_ms[mk.first] += t;
+    bt->prof_n[mk.first] += 1;
+  }
+  HPS_CHECK(hipGetLastError());
+  return 0;
+}
+
 extern "C" int hps_probe_math(int fn, const double* x, const double* y, double* out, uint32_t n)
 {
   if (!x || !out) return fail("null argument");

@@ -81,6 +81,29 @@ struct PhaseArgs {
   double freq;             // beam frequency (MHz)
 };
 
+// Arguments of the SCHEFF rebuild and field launches.
+struct BuildArgs {
+  ScDev* sc;
+  double* fld1;
+  BeamArrays b;
+  const hps_beam_stats* st;
+  int64_t* bins;
+  uint32_t greens_blocks;   // leading workgroups that fill the Green's table
+  uint32_t dep_blocks;      // deposit workgroups after them (0: empty deposit)
+};
+
+struct FieldArgs {
+  const ScDev* sc;
+  const int64_t* bins;
+  const double* fld1;
+  double* fld2;
+};
+
+// Batched launches cover the blocks of several simulations in one grid; a
+// block map entry packs the simulation (high bits) and its local block.
+constexpr uint32_t kBatchLocalBits = 20;
+constexpr uint32_t kBatchLocalMask = (1u << kBatchLocalBits) - 1;
+
 // Launchers (hps_kernels.hip).
 hipError_t launch_segment(const SegArgs& a, hipStream_t s);
 hipError_t launch_seg_final(const SegArgs& a, hipStream_t s);
@@ -100,6 +123,20 @@ hipError_t launch_sc_build(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, Be
                            hipStream_t s);
 hipError_t launch_field(const ScDev* sc, const int64_t* bins, const double* fld1, double* fld2,
                         uint32_t nr, uint32_t nz, size_t max_lds, hipStream_t s);
+uint32_t greens_blocks(uint32_t nr, uint32_t nz);
+
+// Batched forms: recs holds one argument record per simulation (the segment
+// and phase kernels also a block map over their chunks).
+hipError_t launch_segment_batch(const SegArgs* recs, const uint32_t* map, uint32_t nblocks,
+                                int op_class, hipStream_t s);
+hipError_t launch_seg_final_batch(const SegArgs* recs, uint32_t nmem, hipStream_t s);
+hipError_t launch_phase_batch(const PhaseArgs* recs, const uint32_t* map, uint32_t nblocks,
+                              hipStream_t s);
+hipError_t launch_phase_final_batch(const PhaseArgs* recs, uint32_t nmem, hipStream_t s);
+hipError_t launch_sc_build_batch(const BuildArgs* recs, const uint32_t* map, uint32_t nblocks,
+                                 uint32_t nr, uint32_t nz, hipStream_t s);
+hipError_t launch_field_batch(const FieldArgs* recs, uint32_t nmem, uint32_t nr, uint32_t nz,
+                              size_t max_lds, hipStream_t s);
 
 // Generic canonical sums for the getters: over the filtered particles
 // (filter 0: loss == 0, 1: loss + lloss == 0, 2: all) of a, a*a, b, b*b, a*b,

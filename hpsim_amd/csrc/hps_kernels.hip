@@ -381,12 +381,10 @@ __device__ __forceinline__ Particle load_particle(const BeamArrays& b, uint32_t 
 // The group size trades registers (waves per SIMD) against independent
 // chains per wave.
 template <int kClass, int kGroup>
-__global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
+__device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSmem& sm)
 {
   static_assert(kPerLane % kGroup == 0, "group must divide the particles per lane");
-  __shared__ RedSmem sm;
   const int t = threadIdx.x;
-  const uint32_t c = blockIdx.x;
   const BeamArrays& b = a.b;
   KickConst k{};
   if (a.kick) k = kick_const(a.sc, a.st, a.kick_len);
@@ -480,12 +478,31 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
   }
 }
 
+template <int kClass, int kGroup>
+__global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
+{
+  __shared__ RedSmem sm;
+  segment_body<kClass, kGroup>(a, blockIdx.x, sm);
+}
+
+// The same over the chunks of several simulations: map[block] packs the
+// simulation (high bits) and its chunk (low kBatchLocalBits bits).
+template <int kClass, int kGroup>
+__global__ __launch_bounds__(kBlock) void k_segment_batch(const SegArgs* __restrict__ recs,
+                                                          const uint32_t* __restrict__ map)
+{
+  __shared__ RedSmem sm;
+  const uint32_t e = map[blockIdx.x];
+  const SegArgs& a = recs[e >> kBatchLocalBits];
+  segment_body<kClass, kGroup>(a, e & kBatchLocalMask, sm);
+}
+
 // Cross-chunk step of the k_segment statistics in one workgroup, launched
 // after k_segment so that kernel order, not a device-wide fence per
 // workgroup, makes the partials visible: the aperture loss count and
 // centroid (UpdateLoss + UpdateAvgXY), or the longitudinal-loss update and
 // good moments of the space-charge statistics.
-__global__ __launch_bounds__(kBlock) void k_seg_final(SegArgs a)
+__device__ __forceinline__ void seg_final_body(const SegArgs& a)
 {
   __shared__ RedSmem sm;
   __shared__ uint32_t rescan_list[64];
@@ -578,16 +595,24 @@ __global__ __launch_bounds__(kBlock) void k_seg_final(SegArgs a)
 }
 
 
+__global__ __launch_bounds__(kBlock) void k_seg_final(SegArgs a) { seg_final_body(a); }
+
+// one workgroup per simulation of a batch
+__global__ __launch_bounds__(kBlock) void k_seg_final_batch(const SegArgs* recs)
+{
+  const SegArgs a = recs[blockIdx.x];
+  seg_final_body(a);
+}
+
 // --- k_phase -------------------------------------------------------------------
 
 // Per-chunk sums of the relative phase (UpdateRelativePhi(true) then the
 // phi_r moments of Beam::UpdateStatForSpaceCharge, beam.cpp:1003-1016):
 //   0 phi_r over loss == 0, 1 phi_r and 2 phi_r^2 over good particles.
-__global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
+__device__ __forceinline__ void phase_body(const PhaseArgs& a, uint32_t c)
 {
   __shared__ RedSmem sm;
   const int t = threadIdx.x;
-  const uint32_t c = blockIdx.x;
   const BeamArrays& b = a.b;
   const Partials& pr = a.pr;
   const double ref = a.st->phi_avg_good;
@@ -615,12 +640,21 @@ __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a)
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a) { phase_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(kBlock) void k_phase_batch(const PhaseArgs* recs, const uint32_t* map)
+{
+  const uint32_t e = map[blockIdx.x];
+  const PhaseArgs a = recs[e >> kBatchLocalBits];
+  phase_body(a, e & kBatchLocalMask);
+}
+
 // One workgroup after k_phase: the phi_r moments, then the Scheff::Start
 // control logic of space_charge.cpp:84-138 (mesh_size_cutoff_w = 0) on the
 // device, and the cleared charge histogram
 // when it decides to remesh.  Thread 0 reads every input into registers
 // before it writes any output.
-__global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
+__device__ __forceinline__ void phase_final_body(const PhaseArgs& a)
 {
   __shared__ RedSmem sm;
   __shared__ int remesh;
@@ -749,6 +783,14 @@ __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a)
   __syncthreads();
   if (remesh)
     for (uint32_t i = t; i < nbins; i += kBlock) a.bins[i] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a) { phase_final_body(a); }
+
+__global__ __launch_bounds__(kBlock) void k_phase_final_batch(const PhaseArgs* recs)
+{
+  const PhaseArgs a = recs[blockIdx.x];
+  phase_final_body(a);
 }
 
 // --- SCHEFF rebuild ----------------------------------------------------------------
@@ -913,32 +955,48 @@ __device__ __forceinline__ void deposit_block(const BeamArrays& b, const hps_bea
 // next dep_blocks workgroups deposit the charge.  Both read only the control
 // step's results, so they run side by side; all exit when there is no
 // remesh.
-__global__ __launch_bounds__(kDepositBlock) void k_sc_build(ScDev* sc, double* fld1, uint32_t nr,
-                                                            uint32_t nz, BeamArrays b,
-                                                            const hps_beam_stats* st,
-                                                            int64_t* bins,
-                                                            uint32_t greens_blocks,
-                                                            uint32_t dep_blocks)
+__device__ __forceinline__ void sc_build_body(const BuildArgs& a, uint32_t nr, uint32_t nz,
+                                              uint32_t blk, unsigned long long* hist,
+                                              uint32_t* ured)
+{
+  if (!a.sc->remesh) return;
+  if (blk < a.greens_blocks) {
+    greens_entry(a.sc, a.fld1, nr, nz, a.b.n, a.b.mass, blk * kDepositBlock + threadIdx.x);
+    return;
+  }
+  deposit_block(a.b, a.st, a.sc, a.bins, blk - a.greens_blocks, a.dep_blocks, hist, ured);
+}
+
+__global__ __launch_bounds__(kDepositBlock) void k_sc_build(BuildArgs a, uint32_t nr, uint32_t nz)
 {
   extern __shared__ unsigned long long hist[];
   __shared__ uint32_t ured[kDepositBlock];
-  if (!sc->remesh) return;
-  if (blockIdx.x < greens_blocks) {
-    greens_entry(sc, fld1, nr, nz, b.n, b.mass, blockIdx.x * kDepositBlock + threadIdx.x);
-    return;
-  }
-  deposit_block(b, st, sc, bins, blockIdx.x - greens_blocks, dep_blocks, hist, ured);
+  sc_build_body(a, nr, nz, blockIdx.x, hist, ured);
+}
+
+__global__ __launch_bounds__(kDepositBlock) void k_sc_build_batch(const BuildArgs* recs,
+                                                                  const uint32_t* map,
+                                                                  uint32_t nr, uint32_t nz)
+{
+  extern __shared__ unsigned long long hist[];
+  __shared__ uint32_t ured[kDepositBlock];
+  const uint32_t e = map[blockIdx.x];
+  const BuildArgs a = recs[e >> kBatchLocalBits];
+  sc_build_body(a, nr, nz, e & kBatchLocalMask, hist, ured);
 }
 
 // Field table E(zp, rp) = sum over bins of charge * Green's value,
 // UpdateFldTbl2Kernel (space_charge_kernel.cu:230-342).  One wavefront per
 // field point: lane l sums bins l, l+64, ... with fma, then a halving tree.
-__global__ __launch_bounds__(kBlock) void k_field(const ScDev* sc, const int64_t* bins,
-                                                  const double* fld1, double* fld2, uint32_t nr,
-                                                  uint32_t nz)
+__device__ __forceinline__ void field_body(const FieldArgs& f, uint32_t nr, uint32_t nz,
+                                           uint32_t blk)
 {
+  const ScDev* sc = f.sc;
+  const int64_t* bins = f.bins;
+  const double* fld1 = f.fld1;
+  double* fld2 = f.fld2;
   if (!sc->remesh) return;
-  const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint32_t wave = (blk * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const uint32_t nout = (nz + 1) * (nr + 1);
   if (wave >= nout) return;
@@ -973,6 +1031,19 @@ __global__ __launch_bounds__(kBlock) void k_field(const ScDev* sc, const int64_t
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_field(FieldArgs f, uint32_t nr, uint32_t nz)
+{
+  field_body(f, nr, nz, blockIdx.x);
+}
+
+// per simulation of a batch: blocks [m * per, (m + 1) * per)
+__global__ __launch_bounds__(kBlock) void k_field_batch(const FieldArgs* recs, uint32_t per,
+                                                        uint32_t nr, uint32_t nz)
+{
+  const FieldArgs f = recs[blockIdx.x / per];
+  field_body(f, nr, nz, blockIdx.x % per);
+}
+
 // k_field with the charge bins (as FP64) and the Green's slice of one source
 // radius rp staged in LDS.  Workgroup = (rp, zpb consecutive field rows zp);
 // each wavefront computes two field points zp, zp + 1 with the same lane
@@ -980,17 +1051,19 @@ __global__ __launch_bounds__(kBlock) void k_field(const ScDev* sc, const int64_t
 // giving four independent FMA chains per lane.  Needs 24 * nr * nz bytes
 // of LDS.
 constexpr int kFieldOpw = 2;
-__global__ __launch_bounds__(1024) void k_field_lds(const ScDev* sc, const int64_t* bins,
-                                                    const double* fld1, double* fld2,
-                                                    uint32_t nr, uint32_t nz, uint32_t zpb)
+__device__ __forceinline__ void field_lds_body(const FieldArgs& f, uint32_t nr, uint32_t nz,
+                                               uint32_t zpb, uint32_t blk, double* lds)
 {
-  extern __shared__ double lds[];
+  const ScDev* sc = f.sc;
+  const int64_t* bins = f.bins;
+  const double* fld1 = f.fld1;
+  double* fld2 = f.fld2;
   if (!sc->remesh) return;
   const uint32_t fld_sz = nr * nz;
   double2* lg = reinterpret_cast<double2*>(lds);   // [nz][nr] (Er, Ez), 16-byte aligned
   double* lb = lds + 2 * (size_t)fld_sz;             // [fld_sz] charge
-  const uint32_t rp = blockIdx.x % (nr + 1);
-  const uint32_t zgrp = blockIdx.x / (nr + 1);
+  const uint32_t rp = blk % (nr + 1);
+  const uint32_t zgrp = blk / (nr + 1);
   const double2* g = reinterpret_cast<const double2*>(fld1) + (size_t)rp * nz * nr;
   for (uint32_t i = threadIdx.x; i < fld_sz; i += blockDim.x) {
     lb[i] = (double)bins[i] * 0x1p-36;
@@ -1046,6 +1119,21 @@ __global__ __launch_bounds__(1024) void k_field_lds(const ScDev* sc, const int64
       fld2[2 * o1 + 1] = ty1;
     }
   }
+}
+
+__global__ __launch_bounds__(1024) void k_field_lds(FieldArgs f, uint32_t nr, uint32_t nz,
+                                                    uint32_t zpb)
+{
+  extern __shared__ double lds[];
+  field_lds_body(f, nr, nz, zpb, blockIdx.x, lds);
+}
+
+__global__ __launch_bounds__(1024) void k_field_lds_batch(const FieldArgs* recs, uint32_t per,
+                                                          uint32_t nr, uint32_t nz, uint32_t zpb)
+{
+  extern __shared__ double lds[];
+  const FieldArgs f = recs[blockIdx.x / per];
+  field_lds_body(f, nr, nz, zpb, blockIdx.x % per, lds);
 }
 
 // --- getter reductions ---------------------------------------------------------
@@ -1185,9 +1273,30 @@ hipError_t launch_segment(const SegArgs& a, hipStream_t s)
   return hipGetLastError();
 }
 
+hipError_t launch_segment_batch(const SegArgs* recs, const uint32_t* map, uint32_t nblocks,
+                                int op_class, hipStream_t s)
+{
+  if (op_class >= OPC_GAP)
+    hipLaunchKernelGGL((k_segment_batch<OPC_GAP, kSegGroupGap>), dim3(nblocks), dim3(kBlock), 0,
+                       s, recs, map);
+  else if (op_class == OPC_QUAD)
+    hipLaunchKernelGGL((k_segment_batch<OPC_QUAD, kSegGroupQuad>), dim3(nblocks), dim3(kBlock),
+                       0, s, recs, map);
+  else
+    hipLaunchKernelGGL((k_segment_batch<OPC_LIGHT, kSegGroupLight>), dim3(nblocks),
+                       dim3(kBlock), 0, s, recs, map);
+  return hipGetLastError();
+}
+
 hipError_t launch_seg_final(const SegArgs& a, hipStream_t s)
 {
   hipLaunchKernelGGL(k_seg_final, dim3(1), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_final_batch(const SegArgs* recs, uint32_t nmem, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_seg_final_batch, dim3(nmem), dim3(kBlock), 0, s, recs);
   return hipGetLastError();
 }
 
@@ -1204,6 +1313,19 @@ hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s)
   return hipGetLastError();
 }
 
+hipError_t launch_phase_batch(const PhaseArgs* recs, const uint32_t* map, uint32_t nblocks,
+                              hipStream_t s)
+{
+  hipLaunchKernelGGL(k_phase_batch, dim3(nblocks), dim3(kBlock), 0, s, recs, map);
+  return hipGetLastError();
+}
+
+hipError_t launch_phase_final_batch(const PhaseArgs* recs, uint32_t nmem, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_phase_final_batch, dim3(nmem), dim3(kBlock), 0, s, recs);
+  return hipGetLastError();
+}
+
 uint32_t deposit_blocks(uint32_t n)
 {
   uint32_t grid = (n + 4 * kDepositBlock - 1) / (4 * kDepositBlock);
@@ -1212,38 +1334,93 @@ uint32_t deposit_blocks(uint32_t n)
   return grid;
 }
 
+uint32_t greens_blocks(uint32_t nr, uint32_t nz)
+{
+  return ((nr + 1) * nz * nr + kDepositBlock - 1) / kDepositBlock;
+}
+
 hipError_t launch_sc_build(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, BeamArrays b,
                            const hps_beam_stats* st, int64_t* bins, bool deposit,
                            hipStream_t s)
 {
-  const uint32_t greens_blocks = ((nr + 1) * nz * nr + kDepositBlock - 1) / kDepositBlock;
-  const uint32_t dep = deposit ? deposit_blocks(b.n) : 0;
+  BuildArgs a;
+  a.sc = sc; a.fld1 = fld1; a.b = b; a.st = st; a.bins = bins;
+  a.greens_blocks = greens_blocks(nr, nz);
+  a.dep_blocks = deposit ? deposit_blocks(b.n) : 0;
   const size_t lds = (size_t)nr * nz * sizeof(unsigned long long);
-  hipLaunchKernelGGL(k_sc_build, dim3(greens_blocks + dep), dim3(kDepositBlock), lds, s, sc,
-                     fld1, nr, nz, b, st, bins, greens_blocks, dep);
+  hipLaunchKernelGGL(k_sc_build, dim3(a.greens_blocks + a.dep_blocks), dim3(kDepositBlock), lds,
+                     s, a, nr, nz);
   return hipGetLastError();
 }
 
-hipError_t launch_field(const ScDev* sc, const int64_t* bins, const double* fld1, double* fld2,
-                        uint32_t nr, uint32_t nz, size_t max_lds, hipStream_t s)
+hipError_t launch_sc_build_batch(const BuildArgs* recs, const uint32_t* map, uint32_t nblocks,
+                                 uint32_t nr, uint32_t nz, hipStream_t s)
 {
-  const size_t lds = (size_t)24 * nr * nz;
-  if (lds <= max_lds) {
-    // about one workgroup per CU: split the nz + 1 field rows of each rp
-    // into groups of at most 32 rows (16 wavefronts)
+  const size_t lds = (size_t)nr * nz * sizeof(unsigned long long);
+  hipLaunchKernelGGL(k_sc_build_batch, dim3(nblocks), dim3(kDepositBlock), lds, s, recs, map, nr,
+                     nz);
+  return hipGetLastError();
+}
+
+namespace {
+
+// Launch shape of the field sum: LDS-staged when the bins and one Green's
+// slice fit, about one workgroup per CU (the nz + 1 field rows of each rp in
+// groups of at most 32 rows, 16 wavefronts); else one wavefront per point.
+struct FieldShape {
+  bool lds;
+  uint32_t blocks, threads, zpb;
+  size_t lds_bytes;
+};
+
+FieldShape field_shape(uint32_t nr, uint32_t nz, size_t max_lds)
+{
+  FieldShape f{};
+  f.lds_bytes = (size_t)24 * nr * nz;
+  f.lds = f.lds_bytes <= max_lds;
+  if (f.lds) {
     uint32_t groups = 256 / (nr + 1);
     if (groups < 1) groups = 1;
     uint32_t zpb = (nz + 1 + groups - 1) / groups;
     if (zpb > 16 * kFieldOpw) zpb = 16 * kFieldOpw;
     groups = (nz + 1 + zpb - 1) / zpb;
-    const uint32_t waves = (zpb + kFieldOpw - 1) / kFieldOpw;
-    hipLaunchKernelGGL(k_field_lds, dim3((nr + 1) * groups), dim3(waves * 64), lds, s, sc, bins,
-                       fld1, fld2, nr, nz, zpb);
-    return hipGetLastError();
+    f.zpb = zpb;
+    f.blocks = (nr + 1) * groups;
+    f.threads = ((zpb + kFieldOpw - 1) / kFieldOpw) * 64;
+  } else {
+    f.blocks = ((nz + 1) * (nr + 1) + 3) / 4;
+    f.threads = kBlock;
+    f.lds_bytes = 0;
   }
-  const uint32_t nout = (nz + 1) * (nr + 1);
-  const uint32_t grid = (nout + 3) / 4;
-  hipLaunchKernelGGL(k_field, dim3(grid), dim3(kBlock), 0, s, sc, bins, fld1, fld2, nr, nz);
+  return f;
+}
+
+}  // namespace
+
+hipError_t launch_field(const ScDev* sc, const int64_t* bins, const double* fld1, double* fld2,
+                        uint32_t nr, uint32_t nz, size_t max_lds, hipStream_t s)
+{
+  FieldArgs a;
+  a.sc = sc; a.bins = bins; a.fld1 = fld1; a.fld2 = fld2;
+  const FieldShape f = field_shape(nr, nz, max_lds);
+  if (f.lds)
+    hipLaunchKernelGGL(k_field_lds, dim3(f.blocks), dim3(f.threads), f.lds_bytes, s, a, nr, nz,
+                       f.zpb);
+  else
+    hipLaunchKernelGGL(k_field, dim3(f.blocks), dim3(f.threads), 0, s, a, nr, nz);
+  return hipGetLastError();
+}
+
+hipError_t launch_field_batch(const FieldArgs* recs, uint32_t nmem, uint32_t nr, uint32_t nz,
+                              size_t max_lds, hipStream_t s)
+{
+  const FieldShape f = field_shape(nr, nz, max_lds);
+  if (f.lds)
+    hipLaunchKernelGGL(k_field_lds_batch, dim3(f.blocks * nmem), dim3(f.threads), f.lds_bytes, s,
+                       recs, f.blocks, nr, nz, f.zpb);
+  else
+    hipLaunchKernelGGL(k_field_batch, dim3(f.blocks * nmem), dim3(f.threads), 0, s, recs,
+                       f.blocks, nr, nz);
   return hipGetLastError();
 }
 

@@ -219,6 +219,30 @@ int hps_sim_graph_replays(hps_sim* m, uint32_t* out);
  * simulation_engine.cpp:84-92). */
 int hps_sim_last_time_ms(hps_sim* m, double* out);
 
+/* ---- Batch: several simulations in one launch sequence ---- */
+/* The PV-scan loop of the reference, pytest/sim-lbeg-scan-pv.py:140-160
+ * (restore_initial_beam + set_db_epics + simulate per setting, one after the
+ * other), run as one program: every simulator of the batch (its own beam,
+ * beamline settings and SpaceCharge) is compiled as by hps_sim_simulate,
+ * and each kernel launch covers all of them.  Results are identical to
+ * separate hps_sim_simulate calls.  The programs must have the same
+ * structure (same element layout and kinds; element values and beam sizes
+ * may differ, beams of at least 3 particles), the same SCHEFF mesh size,
+ * no mesh_size_cutoff_w.  A simulator belongs to the batch by reference
+ * (destroy the batch first). */
+typedef struct hps_batch hps_batch;
+int hps_batch_create(hps_batch** out);
+int hps_batch_destroy(hps_batch* bt);
+int hps_batch_add(hps_batch* bt, hps_sim* m);
+uint32_t hps_batch_size(const hps_batch* bt);
+/* simulate(start, end) of every simulator (same index rules as hps_sim_simulate) */
+int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end);
+int hps_batch_last_time_ms(hps_batch* bt, double* out);
+/* per-kernel-family device time of the batched launches (families as
+ * hps_sim_set_profiling) */
+int hps_batch_set_profiling(hps_batch* bt, int on);
+int hps_batch_get_profile(hps_batch* bt, double* ms, uint32_t* launches);
+
 /* ---- elementary-function probes (host-callable, run on the device) ---- */
 /* Evaluates fn over n inputs on the GPU: 0 exp, 1 log, 2 sin, 3 cos, 4 sinh,
  * 5 cosh, 6 sqrt, 7 fmod(x, 2pi), 8 x/y, 9 Bessel I0, 10 Bessel I1,

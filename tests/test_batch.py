@@ -1,0 +1,134 @@
+"""Batched launches (hps_batch_*): several simulators, each with its own beam
+size and lattice settings, run as one launch sequence; every member must
+match its own oracle run bit for bit."""
+import numpy as np
+import pytest
+
+from hpsim_amd import configs as cfg
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(nums, periods=3, seed=11, freq=None):
+    base = cfg.config(0, periods=periods)
+    lattice = cfg.build_lattice(base.lattice)
+    out = []
+    for k, n in enumerate(nums):
+        spec = cfg.BeamSpec(num=n, seed=seed * 1000 + k)
+        out.append((spec, cfg.perturb_lattice(lattice, seed + k)))
+    return base, out
+
+
+def _gpu(members, nr=32, nz=128, interval=0.05, adj=0, ranges=((None, None),), sc=True):
+    from hpsim_amd import HPSim
+    objs = []
+    for spec, els in members:
+        b = HPSim.Beam(mass=spec.mass, charge=spec.charge, current=spec.current, num=spec.num)
+        b.set_distribution(*cfg.make_distribution(spec))
+        b.set_frequency(spec.freq)
+        b.set_ref_w(spec.w0)
+        s = HPSim.SpaceCharge(nr=nr, nz=nz, interval=interval, adj_bunch=adj) if sc else None
+        m = HPSim.Simulator(b, HPSim.BeamLine(elements=els), s)
+        objs.append((b, s, m))
+    batch = HPSim.Batch([m for _, _, m in objs])
+    for r in ranges:
+        if r == (None, None):
+            batch.simulate()
+        else:
+            batch.simulate(*r)
+    return objs, batch
+
+
+def _oracle(members, nr=32, nz=128, interval=0.05, adj=0, ranges=((None, None),), sc=True):
+    from oracle import oracle as O
+    out = []
+    for spec, els in members:
+        b = O.Beam(spec.num, spec.mass, spec.charge, spec.current)
+        b.set_distribution(*cfg.make_distribution(spec))
+        b.set_frequency(spec.freq)
+        b.set_ref_w(spec.w0)
+        s = O.SpaceCharge(nr, nz, interval, adj) if sc else None
+        m = O.Simulator(b, els, s)
+        for r in ranges:
+            m.simulate(*r)
+        out.append((b, s))
+    return out
+
+
+def _check(objs, ref):
+    for k, ((gb, gs, _), (ob, os_)) in enumerate(zip(objs, ref)):
+        err = H.compare_arrays(gb.arrays(), ob.arrays(), f"member {k}: ")
+        assert err is None, err
+        if gs is not None:
+            g, o = gs.state(), os_.state()
+            for f in ("remesh_count", "kick_count", "num_on_mesh", "dr", "dz"):
+                assert g[f] == o[f], (k, f, g[f], o[f])
+
+
+def test_batch_matches_separate_oracle_runs():
+    """Five beams of different sizes, each on its own perturbed lattice."""
+    _, members = _members([1000, 3000, 4096, 7000, 2500])
+    objs, batch = _gpu(members)
+    _check(objs, _oracle(members))
+    assert batch.last_time_ms() > 0
+
+
+def test_batch_partial_ranges_and_repeat():
+    """simulate(start, end) twice (the second call continues the first), then
+    the whole lattice again (a reset, as for a single simulator)."""
+    _, members = _members([2048, 5000, 3333], periods=4)
+    ranges = ((0, 6), (7, 15), (None, None))
+    objs, _ = _gpu(members, ranges=ranges)
+    _check(objs, _oracle(members, ranges=ranges))
+
+
+def test_batch_without_space_charge_and_small_mesh():
+    _, members = _members([1500, 4096])
+    objs, _ = _gpu(members, sc=False)
+    _check(objs, _oracle(members, sc=False))
+    objs, _ = _gpu(members, nr=7, nz=33, adj=2)
+    _check(objs, _oracle(members, nr=7, nz=33, adj=2))
+
+
+def test_batch_with_frequency_change_on_the_ccl(tmp_path):
+    """Beams at 201.25 MHz entering the 805 MHz CCL: the frequency change is a
+    host step between batched launches."""
+    import os
+    import shutil
+    from hpsim_amd import db
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ccl.db")
+    shutil.copyfile(src, tmp_path / "ccl.db")
+    c = db.DBConnection(str(tmp_path / "ccl.db"))
+    c.clear_model_index()
+    els = db.beamline_elements(c)[:120]
+    c.close()
+    members = []
+    for k, n in enumerate((2048, 3000)):
+        spec = cfg.BeamSpec(num=n, seed=50 + k, w0=99.9636448918256, phi0=-2.295,
+                            sig_phi=0.05, sig_w=0.02)
+        members.append((spec, cfg.perturb_lattice(els, 77 + k)))
+    objs, _ = _gpu(members, nr=16, nz=64)
+    _check(objs, _oracle(members, nr=16, nz=64))
+
+
+def test_batch_rejects_different_layouts_and_keeps_simulators_usable():
+    from hpsim_amd import HPSim
+    _, members = _members([2000, 2000])
+    members[1] = (members[1][0], members[1][1][:-4])      # one period shorter
+    objs = []
+    for spec, els in members:
+        b = HPSim.Beam(mass=spec.mass, charge=spec.charge, current=spec.current, num=spec.num)
+        b.set_distribution(*cfg.make_distribution(spec))
+        b.set_frequency(spec.freq)
+        b.set_ref_w(spec.w0)
+        s = HPSim.SpaceCharge(nr=32, nz=128, interval=0.05)
+        objs.append((b, s, HPSim.Simulator(b, HPSim.BeamLine(elements=els), s)))
+    batch = HPSim.Batch([m for _, _, m in objs])
+    with pytest.raises(Exception, match="same elements|different programs"):
+        batch.simulate()
+    with pytest.raises(Exception, match="already in the batch"):
+        batch.add(objs[0][2])
+    objs[1][2].simulate()
+    ref = _oracle(members[1:])
+    assert H.compare_arrays(objs[1][0].arrays(), ref[0][0].arrays()) is None
