@@ -63,6 +63,11 @@ def parse():
                    help="lattice periods of the all-core CPU-baseline sample")
     p.add_argument("--cpu-serial-periods", type=int, default=10,
                    help="lattice periods of the one-core CPU-baseline sample")
+    p.add_argument("--workload", default="plain", choices=("plain", "small", "batched-scan"),
+                   help="plain: the 1,048,576-particle single beam (default); small: the "
+                        "65,536-particle single beam on 40 periods; batched-scan: 256 beams "
+                        "of 16K-1M particles with per-beam lattice settings in one batch")
+    p.add_argument("--beams", type=int, default=256, help="beams of the batched scan")
     return p.parse_args()
 
 
@@ -92,7 +97,7 @@ def _oracle_run(wl, periods, openmp):
     return b.num * len(elems) / dt, O.lib().orc_threads(), dt, len(elems)
 
 
-def cpu_baseline(wl, periods, serial_periods):
+def cpu_baseline(wl, periods, serial_periods, idx=1):
     """OpenMP oracle on all host cores (OMP_NUM_THREADS) over `periods`
     periods, plus the serial oracle on `serial_periods` periods in a child
     process (one process holds one oracle build)."""
@@ -106,13 +111,128 @@ def cpu_baseline(wl, periods, serial_periods):
                      f"the same lattice, SCHEFF on, OpenMP oracle, {threads} threads, {dt:.1f} s"}
     code = ("import json, sys; sys.path.insert(0, %r); import bench; "
             "from hpsim_amd import configs as cfg; "
-            "print(json.dumps(bench._oracle_run(cfg.config(1, num=%d, periods=%d), %d, False)))"
-            % (ROOT, wl.beam.num, wl.lattice.periods, serial_periods))
+            "print(json.dumps(bench._oracle_run(cfg.config(%d, num=%d, periods=%d), %d, False)))"
+            % (ROOT, idx, wl.beam.num, wl.lattice.periods, serial_periods))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True)
     srate, _, sdt, snel = json.loads(r.stdout.strip().splitlines()[-1])
     out["serial"] = {"value": srate, "cores": 1,
                      "sample": f"first {snel} elements, serial oracle, {sdt:.1f} s"}
     return out
+
+
+def _member_dist(spec):
+    from hpsim_amd import configs as cfg
+    return cfg.make_distribution(spec)
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batched_cpu_baseline(w, budget_s=20.0):
+    """OpenMP oracle over whole members of the batch, in order, until about
+    budget_s seconds of tracking; pushes/s of that sample (simulate() time
+    only)."""
+    from oracle import oracle as O
+    O.use_openmp()
+    done = pushes = 0
+    busy = 0.0
+    for spec, els in zip(w.beams, w.lattices):
+        beam = O.Beam(spec.num, spec.mass, spec.charge, spec.current)
+        beam.set_distribution(*_member_dist(spec))
+        beam.set_frequency(spec.freq)
+        beam.set_ref_w(spec.w0)
+        sim = O.Simulator(beam, els, O.SpaceCharge(w.nr, w.nz, w.interval))
+        t0 = time.perf_counter()
+        sim.simulate()
+        busy += time.perf_counter() - t0
+        pushes += spec.num * len(els)
+        done += 1
+        if busy > budget_s:
+            break
+    return {"value": pushes / busy, "unit": "particle-element pushes/s",
+            "cores": O.lib().orc_threads(), "kind": "port",
+            "sample": f"first {done} of {len(w.beams)} beams of the batch "
+                      f"({pushes // len(w.lattices[0])} particles x {len(w.lattices[0])} elements), "
+                      f"OpenMP oracle, one beam after the other, {busy:.1f} s"}
+
+
+def main_batched(args):
+    import multiprocessing as mp
+
+    from hpsim_amd import HPSim
+    from hpsim_amd import configs as cfg
+
+    w = cfg.batched_scan(nbeams=args.beams, periods=args.periods or 40)
+    total = sum(b.num for b in w.beams)
+    nel = len(w.lattices[0])
+    _log(f"{len(w.beams)} beams, {total} particles, {nel} elements: sampling distributions")
+    with mp.get_context("spawn").Pool(min(16, os.cpu_count() or 1)) as pool:
+        dists = pool.map(_member_dist, w.beams, chunksize=1)
+    _log("uploading beams")
+    sims = []
+    for spec, els, d in zip(w.beams, w.lattices, dists):
+        beam = HPSim.Beam(mass=spec.mass, charge=spec.charge, current=spec.current, num=spec.num)
+        beam.set_distribution(*d)
+        beam.set_frequency(spec.freq)
+        beam.set_ref_w(spec.w0)
+        beam.save_initial_beam()
+        sc = HPSim.SpaceCharge(nr=w.nr, nz=w.nz, interval=w.interval)
+        sims.append(HPSim.Simulator(beam, HPSim.BeamLine(elements=els), sc))
+    del dists
+    batch = HPSim.Batch(sims)
+
+    def step():
+        for m in sims:
+            m.beam.restore_initial_beam()
+        batch.simulate()
+
+    for k in range(args.warmup):
+        step()
+        _log(f"warm-up step {k + 1} done")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step()
+        _log(f"timed step {k + 1} done")
+    dt = time.perf_counter() - t0
+    batch.set_profiling(True)
+    step()
+    prof = batch.profile()
+    batch.set_profiling(False)
+    seg_ms, seg_n = prof["segment"]
+    avg_s = seg_ms / 1e3 / max(seg_n, 1)
+    achieved = SEG_BYTES_PER_PARTICLE * total / avg_s / 1e9
+    transmitted = sum(m.beam.get_size() - m.beam.get_loss_num("t") for m in sims)
+    out = {
+        "metric": "particle-element pushes/s incl. space charge",
+        "value": total * nel * args.steps / dt,
+        "unit": "particle-element pushes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox-seeded truncated Gaussian beams, generated lattice with "
+                "seeded per-beam perturbations)",
+        "config": {"workload": w.name, "beams": len(w.beams), "particles": total,
+                   "particles_min": min(b.num for b in w.beams),
+                   "particles_max": max(b.num for b in w.beams), "elements": nel,
+                   "mesh": f"{w.nr}x{w.nz}", "transmitted": transmitted},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_segment_batch", "avg_launch_us": avg_s * 1e6,
+                     "launches": seg_n, "bytes_per_launch": SEG_BYTES_PER_PARTICLE * total,
+                     "timing": "HIP events around every launch of one extra step"},
+        "kernel_ms_per_step": {k: v[0] for k, v in prof.items()},
+        "launch": "eager batched",
+    }
+    if not args.no_cpu_baseline:
+        _log("cpu baseline")
+        out["cpu_baseline"] = batched_cpu_baseline(w)
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -121,10 +241,12 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent beams "
               "shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched-scan":
+        return main_batched(args)
     from hpsim_amd import HPSim
     from hpsim_amd import configs as cfg
 
-    wl = cfg.config(1, num=args.num, periods=args.periods)
+    wl = cfg.config(0 if args.workload == "small" else 1, num=args.num, periods=args.periods)
     b = wl.beam
     elems = cfg.build_lattice(wl.lattice)
     dist = cfg.make_distribution(b)
@@ -190,7 +312,8 @@ def main():
         "launch": "eager" if args.eager else "hip-graph",
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_periods, args.cpu_serial_periods)
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_periods, args.cpu_serial_periods,
+                                           0 if args.workload == "small" else 1)
     print(json.dumps(out), flush=True)
 
 
