@@ -276,6 +276,7 @@ struct hps_spch {
   double* fld1 = nullptr;
   double* fld2 = nullptr;
   int64_t* bins = nullptr;
+  double* part = nullptr;    // split partial sums of the field (kFieldMaxSplit tables)
   size_t max_lds = 0;        // LDS bytes per workgroup on this device
 };
 
@@ -780,12 +781,13 @@ constexpr uint32_t kMaxAdjBunch = 1024;
 
 static int spch_alloc_tables(hps_spch* s)
 {
-  (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins);
-  s->fld1 = nullptr; s->fld2 = nullptr; s->bins = nullptr;
+  (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins); (void)hipFree(s->part);
+  s->fld1 = nullptr; s->fld2 = nullptr; s->bins = nullptr; s->part = nullptr;
   const uint32_t nr = s->cfg.nx, nz = s->cfg.nz;
   const size_t n1 = (size_t)(nr + 1) * nr * nz * 2, n2 = (size_t)(nz + 1) * (nr + 1) * 2;
   HPS_CHECK(hipMalloc(&s->fld1, n1 * 8));
   HPS_CHECK(hipMalloc(&s->fld2, n2 * 8));
+  HPS_CHECK(hipMalloc(&s->part, kFieldMaxSplit * n2 * 8));
   HPS_CHECK(hipMalloc(&s->bins, (size_t)nr * nz * 8));
   HPS_CHECK(hipMemset(s->fld1, 0, n1 * 8));
   HPS_CHECK(hipMemset(s->fld2, 0, n2 * 8));
@@ -841,6 +843,7 @@ int hps_spch_destroy(hps_spch* s)
   if (!s) return 0;
   (void)hipStreamSynchronize(s->stream);
   (void)hipFree(s->dev); (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins);
+  (void)hipFree(s->part);
   delete s;
   return 0;
 }
@@ -1360,9 +1363,11 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
       case L_BUILD:
         return launch_sc_build(sc->dev, sc->fld1, sc->cfg.nx, sc->cfg.nz, b->d, b->st, sc->bins,
                                deposit_ok, st);
-      default:
-        return launch_field(sc->dev, sc->bins, sc->fld1, sc->fld2, sc->cfg.nx, sc->cfg.nz,
-                            sc->max_lds, st);
+      default: {
+        FieldArgs f;
+        f.sc = sc->dev; f.bins = sc->bins; f.fld1 = sc->fld1; f.fld2 = sc->fld2; f.part = sc->part;
+        return launch_field(f, sc->cfg.nx, sc->cfg.nz, sc->max_lds, st);
+      }
     }
   };
 
@@ -1714,6 +1719,7 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
         build_map.push_back(((uint32_t)k << kBatchLocalBits) | c);
       FieldArgs& f = field[k];
       f.sc = m->sc->dev; f.bins = m->sc->bins; f.fld1 = m->sc->fld1; f.fld2 = m->sc->fld2;
+      f.part = m->sc->part;
     }
   }
   auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };

@@ -97,7 +97,11 @@ struct FieldArgs {
   const int64_t* bins;
   const double* fld1;
   double* fld2;
+  double* part;     // kFieldMaxSplit * (nz+1)*(nr+1) double2: split partial sums
 };
+// The field sum may split its partial sums over up to this many workgroup
+// sets (a lone simulation, to use more CUs); part holds their results.
+constexpr uint32_t kFieldMaxSplit = 8;
 
 // Batched launches cover the blocks of several simulations in one grid; a
 // block map entry packs the simulation (high bits) and its local block.
@@ -121,8 +125,8 @@ uint32_t deposit_blocks(uint32_t n);
 hipError_t launch_sc_build(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, BeamArrays b,
                            const hps_beam_stats* st, int64_t* bins, bool deposit,
                            hipStream_t s);
-hipError_t launch_field(const ScDev* sc, const int64_t* bins, const double* fld1, double* fld2,
-                        uint32_t nr, uint32_t nz, size_t max_lds, hipStream_t s);
+hipError_t launch_field(const FieldArgs& a, uint32_t nr, uint32_t nz, size_t max_lds,
+                        hipStream_t s);
 uint32_t greens_blocks(uint32_t nr, uint32_t nz);
 
 // Batched forms: recs holds one argument record per simulation (the segment

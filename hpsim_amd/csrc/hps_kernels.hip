@@ -986,154 +986,193 @@ __global__ __launch_bounds__(kDepositBlock) void k_sc_build_batch(const BuildArg
 }
 
 // Field table E(zp, rp) = sum over bins of charge * Green's value,
-// UpdateFldTbl2Kernel (space_charge_kernel.cu:230-342).  One wavefront per
-// field point: lane l sums bins l, l+64, ... with fma, then a halving tree.
-__device__ __forceinline__ void field_body(const FieldArgs& f, uint32_t nr, uint32_t nz,
-                                           uint32_t blk)
+// UpdateFldTbl2Kernel (space_charge_kernel.cu:230-342), in the canonical
+// order: P = min(NextPow2(nr), 64) partial sums, partial p takes the source
+// rings rs = p, p + P, ... in ascending order and for each the rows
+// zs = 0 .. nz-1 in ascending order (fma), then a halving tree over the P
+// partials.  Field point (zp, rp) reads the Green's entry at dz = zs - zp
+// (zs >= zp, E_z sign -1) or zp - zs - 1 (zs < zp).
+//
+// The P partials may be split by residue over S workgroup sets (S a power of
+// two dividing P): set s holds the partials p = s, s + S, ...; its halving
+// tree runs the levels o >= S (which pair partials of the same residue) and
+// stores the level-S value; k_field_combine runs the levels o < S.
+//
+// A group of P / S lanes computes K consecutive rows zp0 .. zp0 + K - 1 of
+// one rp: lane j holds partial s + j S of all K points.  Along zs the K
+// points need the Green's entries of K consecutive offsets d = zs - zp, so a
+// K-deep window of (E_r, signed E_z) values stays in registers and each step
+// loads one new entry and one charge.  The window slot of offset d is d mod K
+// (zp0 is a multiple of K), so with the zs loop unrolled by K every slot
+// index is a compile-time constant.  kLds: the workgroup's columns (rings of
+// its residue) of the rp slice of the table and of the bins are staged in
+// LDS first.
+struct FieldGeom {
+  uint32_t P, S, groups_per_wg, wgs_per_set;
+};
+
+template <int K, bool kLds>
+__device__ __forceinline__ void field_win_body(const FieldArgs& f, uint32_t nr, uint32_t nz,
+                                               const FieldGeom& fg, uint32_t blk, double* lds)
 {
   const ScDev* sc = f.sc;
-  const int64_t* bins = f.bins;
-  const double* fld1 = f.fld1;
-  double* fld2 = f.fld2;
   if (!sc->remesh) return;
-  const uint32_t wave = (blk * kBlock + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  const uint32_t nout = (nz + 1) * (nr + 1);
-  if (wave >= nout) return;
-  const uint32_t zp = wave / (nr + 1), rp = wave % (nr + 1);
+  const uint32_t S = fg.S, G = fg.P / S;
+  const uint32_t per_rp = S * fg.wgs_per_set;
+  const uint32_t rp = blk / per_rp;
+  const uint32_t set = (blk % per_rp) / fg.wgs_per_set;
+  const uint32_t wg = blk % fg.wgs_per_set;
   const uint32_t fld_sz = nr * nz;
-  const uint32_t dq = 64 / nr, dm = 64 % nr;
-  uint32_t zs = (uint32_t)lane / nr, rs = (uint32_t)lane % nr;
-  double tx = 0.0, ty = 0.0;
-  const double* g = fld1 + 2 * (size_t)rp * nz * nr;
-  for (uint32_t bi = lane; bi < fld_sz; bi += 64) {
-    const double bin = (double)bins[bi] * 0x1p-36;
-    double sign = 1.0;
-    uint32_t dz;
-    if (zs >= zp) { dz = zs - zp; sign = -1.0; }
-    else dz = zp - zs - 1;
-    const size_t fi = (size_t)dz * nr + rs;
-    tx = ::fma(bin, g[2 * fi], tx);
-    ty = ::fma(sign * bin, g[2 * fi + 1], ty);
-    rs += dm; zs += dq;
-    if (rs >= nr) { rs -= nr; zs += 1; }
+  // columns of this set: rings rs = set + c S, c < nc
+  const uint32_t nc = set < nr ? (nr - set + S - 1) / S : 0;
+  const double2* gsrc = reinterpret_cast<const double2*>(f.fld1) + (size_t)rp * fld_sz;
+  const double2* g = gsrc;
+  const double* lb = nullptr;
+  uint32_t gstride = nr;
+  if (kLds) {
+    double2* lg = reinterpret_cast<double2*>(lds);   // [nz][nc] (E_r, E_z)
+    double* lq = lds + 2 * (size_t)nz * nc;           // [nz][nc] charge
+    for (uint32_t i = threadIdx.x; i < nz * nc; i += blockDim.x) {
+      const uint32_t z = i / nc, c = i % nc;
+      const uint32_t src = z * nr + set + c * S;
+      lg[i] = gsrc[src];
+      lq[i] = (double)f.bins[src] * 0x1p-36;
+    }
+    __syncthreads();
+    g = lg;
+    lb = lq;
+    gstride = nc;
   }
+  const uint32_t j = threadIdx.x % G;
+  const uint32_t block = wg * fg.groups_per_wg + threadIdx.x / G;
+  const int zp0 = (int)(block * K);
+  if (zp0 > (int)nz) return;
+  const int inz = (int)nz;
+  // column of ring rs in the (staged or global) table
+  auto col = [&](uint32_t rs) -> uint32_t { return kLds ? (rs - set) / S : rs; };
+  // window entry for offset d of ring rs; offsets of the zs loop always
+  // index the table (|d| < nz for rows zp <= nz), only the prefill of rows
+  // past nz needs the bound
+  auto entry = [&](int d, uint32_t c) -> double2 {
+    const int t = d >= 0 ? d : -d - 1;
+    double2 v = g[(uint32_t)t * gstride + c];
+    if (d >= 0) v.y = -v.y;
+    return v;
+  };
+  auto entry_bounded = [&](int d, uint32_t c) -> double2 {
+    const int t = d >= 0 ? d : -d - 1;
+    return t < inz ? entry(d, c) : make_double2(0.0, 0.0);
+  };
+  auto charge = [&](int zs, uint32_t c) -> double {
+    const uint32_t bi = (uint32_t)zs * gstride + c;
+    return kLds ? lb[bi] : (double)f.bins[bi] * 0x1p-36;
+  };
+  double tx[K], ty[K];
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    const double ox = __shfl_down(tx, o, 64);
-    const double oy = __shfl_down(ty, o, 64);
-    tx = tx + ox;
-    ty = ty + oy;
-  }
-  if (lane == 0) {
-    fld2[2 * (size_t)wave] = tx;
-    fld2[2 * (size_t)wave + 1] = ty;
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_field(FieldArgs f, uint32_t nr, uint32_t nz)
-{
-  field_body(f, nr, nz, blockIdx.x);
-}
-
-// per simulation of a batch: blocks [m * per, (m + 1) * per)
-__global__ __launch_bounds__(kBlock) void k_field_batch(const FieldArgs* recs, uint32_t per,
-                                                        uint32_t nr, uint32_t nz)
-{
-  const FieldArgs f = recs[blockIdx.x / per];
-  field_body(f, nr, nz, blockIdx.x % per);
-}
-
-// k_field with the charge bins (as FP64) and the Green's slice of one source
-// radius rp staged in LDS.  Workgroup = (rp, zpb consecutive field rows zp);
-// each wavefront computes two field points zp, zp + 1 with the same lane
-// chains and halving tree as k_field (identical tables), the two points
-// giving four independent FMA chains per lane.  Needs 24 * nr * nz bytes
-// of LDS.
-constexpr int kFieldOpw = 2;
-__device__ __forceinline__ void field_lds_body(const FieldArgs& f, uint32_t nr, uint32_t nz,
-                                               uint32_t zpb, uint32_t blk, double* lds)
-{
-  const ScDev* sc = f.sc;
-  const int64_t* bins = f.bins;
-  const double* fld1 = f.fld1;
-  double* fld2 = f.fld2;
-  if (!sc->remesh) return;
-  const uint32_t fld_sz = nr * nz;
-  double2* lg = reinterpret_cast<double2*>(lds);   // [nz][nr] (Er, Ez), 16-byte aligned
-  double* lb = lds + 2 * (size_t)fld_sz;             // [fld_sz] charge
-  const uint32_t rp = blk % (nr + 1);
-  const uint32_t zgrp = blk / (nr + 1);
-  const double2* g = reinterpret_cast<const double2*>(fld1) + (size_t)rp * nz * nr;
-  for (uint32_t i = threadIdx.x; i < fld_sz; i += blockDim.x) {
-    lb[i] = (double)bins[i] * 0x1p-36;
-    lg[i] = g[i];
-  }
-  __syncthreads();
-  const uint32_t zbeg = zgrp * zpb;
-  const uint32_t zend = zbeg + zpb < nz + 1 ? zbeg + zpb : nz + 1;
-  const uint32_t zp0 = zbeg + (threadIdx.x >> 6) * kFieldOpw;
-  if (zp0 >= zend) return;
-  const bool two = zp0 + 1 < zend;
-  const uint32_t zp1 = two ? zp0 + 1 : zp0;
-  const int lane = threadIdx.x & 63;
-  const uint32_t dq = 64 / nr, dm = 64 % nr;
-  uint32_t zs = (uint32_t)lane / nr, rs = (uint32_t)lane % nr;
-  double tx0 = 0.0, ty0 = 0.0, tx1 = 0.0, ty1 = 0.0;
-#pragma unroll 2
-  for (uint32_t bi = lane; bi < fld_sz; bi += 64) {
-    const double bin = lb[bi];
-    double sign0 = 1.0, sign1 = 1.0;
-    uint32_t dz0, dz1;
-    if (zs >= zp0) { dz0 = zs - zp0; sign0 = -1.0; }
-    else dz0 = zp0 - zs - 1;
-    if (zs >= zp1) { dz1 = zs - zp1; sign1 = -1.0; }
-    else dz1 = zp1 - zs - 1;
-    const double2 g0 = lg[dz0 * nr + rs];
-    const double2 g1 = lg[dz1 * nr + rs];
-    tx0 = ::fma(bin, g0.x, tx0);
-    ty0 = ::fma(sign0 * bin, g0.y, ty0);
-    tx1 = ::fma(bin, g1.x, tx1);
-    ty1 = ::fma(sign1 * bin, g1.y, ty1);
-    rs += dm; zs += dq;
-    if (rs >= nr) { rs -= nr; zs += 1; }
-  }
+  for (int k = 0; k < K; ++k) { tx[k] = 0.0; ty[k] = 0.0; }
+  for (uint32_t rs = set + j * S; rs < nr; rs += fg.P) {
+    const uint32_t c = col(rs);
+    double2 w[K];
+    w[0] = make_double2(0.0, 0.0);
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    const double a0 = __shfl_down(tx0, o, 64);
-    const double b0 = __shfl_down(ty0, o, 64);
-    const double a1 = __shfl_down(tx1, o, 64);
-    const double b1 = __shfl_down(ty1, o, 64);
-    tx0 = tx0 + a0;
-    ty0 = ty0 + b0;
-    tx1 = tx1 + a1;
-    ty1 = ty1 + b1;
+    for (int k = 1; k < K; ++k) w[K - k] = entry_bounded(-zp0 - k, c);
+    int zs0 = 0;
+    for (; zs0 + K <= inz; zs0 += K) {
+#pragma unroll
+      for (int u = 0; u < K; ++u) {
+        w[u] = entry(zs0 + u - zp0, c);
+        const double q = charge(zs0 + u, c);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const double2 v = w[(u - k + K) % K];
+          tx[k] = ::fma(q, v.x, tx[k]);
+          ty[k] = ::fma(q, v.y, ty[k]);
+        }
+      }
+    }
+    if (zs0 < inz) {
+#pragma unroll
+      for (int u = 0; u < K; ++u) {
+        if (zs0 + u < inz) {
+          w[u] = entry(zs0 + u - zp0, c);
+          const double q = charge(zs0 + u, c);
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            const double2 v = w[(u - k + K) % K];
+            tx[k] = ::fma(q, v.x, tx[k]);
+            ty[k] = ::fma(q, v.y, ty[k]);
+          }
+        }
+      }
+    }
   }
-  if (lane == 0) {
-    const size_t o0 = (size_t)zp0 * (nr + 1) + rp;
-    fld2[2 * o0] = tx0;
-    fld2[2 * o0 + 1] = ty0;
-    if (two) {
-      const size_t o1 = (size_t)zp1 * (nr + 1) + rp;
-      fld2[2 * o1] = tx1;
-      fld2[2 * o1 + 1] = ty1;
+  // levels o = P/2 .. S of the tree: lane offsets G/2 .. 1 inside the group
+  for (uint32_t o = G / 2; o >= 1; o >>= 1) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double ox = __shfl_down(tx[k], o, (int)G);
+      const double oy = __shfl_down(ty[k], o, (int)G);
+      tx[k] = tx[k] + ox;
+      ty[k] = ty[k] + oy;
+    }
+  }
+  if (j == 0) {
+    const uint32_t nout = (nz + 1) * (nr + 1);
+    double* dst = S == 1 ? f.fld2 : f.part + 2 * (size_t)set * nout;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int zp = zp0 + k;
+      if (zp <= inz) {
+        const size_t o = (size_t)zp * (nr + 1) + rp;
+        dst[2 * o] = tx[k];
+        dst[2 * o + 1] = ty[k];
+      }
     }
   }
 }
 
-__global__ __launch_bounds__(1024) void k_field_lds(FieldArgs f, uint32_t nr, uint32_t nz,
-                                                    uint32_t zpb)
+template <int K, bool kLds>
+__global__ __launch_bounds__(1024) void k_field_win(FieldArgs f, uint32_t nr, uint32_t nz,
+                                                    FieldGeom fg)
 {
   extern __shared__ double lds[];
-  field_lds_body(f, nr, nz, zpb, blockIdx.x, lds);
+  field_win_body<K, kLds>(f, nr, nz, fg, blockIdx.x, lds);
 }
 
-__global__ __launch_bounds__(1024) void k_field_lds_batch(const FieldArgs* recs, uint32_t per,
-                                                          uint32_t nr, uint32_t nz, uint32_t zpb)
+// per simulation of a batch: blocks [m * per, (m + 1) * per)
+template <int K, bool kLds>
+__global__ __launch_bounds__(1024) void k_field_win_batch(const FieldArgs* recs, uint32_t per,
+                                                          uint32_t nr, uint32_t nz, FieldGeom fg)
 {
   extern __shared__ double lds[];
   const FieldArgs f = recs[blockIdx.x / per];
-  field_lds_body(f, nr, nz, zpb, blockIdx.x % per, lds);
+  field_win_body<K, kLds>(f, nr, nz, fg, blockIdx.x % per, lds);
+}
+
+// Levels o = S/2 .. 1 of the field tree over the split results.
+__global__ __launch_bounds__(kBlock) void k_field_combine(FieldArgs f, uint32_t nout, uint32_t S)
+{
+  if (!f.sc->remesh) return;
+  const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+  if (o >= nout) return;
+  double ax[kFieldMaxSplit], ay[kFieldMaxSplit];
+#pragma unroll
+  for (uint32_t l = 0; l < kFieldMaxSplit; ++l) {
+    ax[l] = l < S ? f.part[2 * ((size_t)l * nout + o)] : 0.0;
+    ay[l] = l < S ? f.part[2 * ((size_t)l * nout + o) + 1] : 0.0;
+  }
+#pragma unroll
+  for (uint32_t h = kFieldMaxSplit / 2; h >= 1; h >>= 1) {
+    if (h < S) {
+#pragma unroll
+      for (uint32_t l = 0; l < h; ++l) {
+        ax[l] = ax[l] + ax[l + h];
+        ay[l] = ay[l] + ay[l + h];
+      }
+    }
+  }
+  f.fld2[2 * (size_t)o] = ax[0];
+  f.fld2[2 * (size_t)o + 1] = ay[0];
 }
 
 // --- getter reductions ---------------------------------------------------------
@@ -1364,64 +1403,95 @@ hipError_t launch_sc_build_batch(const BuildArgs* recs, const uint32_t* map, uin
 
 namespace {
 
-// Launch shape of the field sum: LDS-staged when the bins and one Green's
-// slice fit, about one workgroup per CU (the nz + 1 field rows of each rp in
-// groups of at most 32 rows, 16 wavefronts); else one wavefront per point.
+// Launch shape of the field sum.
 struct FieldShape {
   bool lds;
-  uint32_t blocks, threads, zpb;
+  int K;
+  FieldGeom g;
+  uint32_t threads, blocks;
   size_t lds_bytes;
 };
 
-FieldShape field_shape(uint32_t nr, uint32_t nz, size_t max_lds)
+FieldShape field_shape_k(uint32_t nr, uint32_t nz, size_t max_lds, int K, uint32_t S,
+                         uint32_t max_threads)
 {
   FieldShape f{};
-  f.lds_bytes = (size_t)24 * nr * nz;
+  uint32_t P = 1;
+  while (P < nr && P < 64) P <<= 1;
+  if (S > P) S = P;
+  f.g.P = P;
+  f.g.S = S;
+  f.K = K;
+  const uint32_t G = P / S;
+  const uint32_t nblk = (nz + 1 + K - 1) / K;
+  uint32_t threads = nblk * G;
+  threads = (threads + 63) / 64 * 64;
+  if (threads > max_threads) threads = max_threads;
+  f.threads = threads;
+  f.g.groups_per_wg = threads / G;
+  f.g.wgs_per_set = (nblk + f.g.groups_per_wg - 1) / f.g.groups_per_wg;
+  f.blocks = (nr + 1) * S * f.g.wgs_per_set;
+  const uint32_t nc = (nr + S - 1) / S;   // columns of the widest set
+  f.lds_bytes = (size_t)24 * nz * nc;
   f.lds = f.lds_bytes <= max_lds;
-  if (f.lds) {
-    uint32_t groups = 256 / (nr + 1);
-    if (groups < 1) groups = 1;
-    uint32_t zpb = (nz + 1 + groups - 1) / groups;
-    if (zpb > 16 * kFieldOpw) zpb = 16 * kFieldOpw;
-    groups = (nz + 1 + zpb - 1) / zpb;
-    f.zpb = zpb;
-    f.blocks = (nr + 1) * groups;
-    f.threads = ((zpb + kFieldOpw - 1) / kFieldOpw) * 64;
-  } else {
-    f.blocks = ((nz + 1) * (nr + 1) + 3) / 4;
-    f.threads = kBlock;
-    f.lds_bytes = 0;
-  }
+  if (!f.lds) f.lds_bytes = 0;
   return f;
+}
+
+// A batch has workgroups enough with the deepest window and all partials in
+// one workgroup; a lone simulation splits its partials over sets to use
+// more CUs (each set stages only its rings).
+FieldShape field_shape(uint32_t nr, uint32_t nz, size_t max_lds, uint32_t nsim)
+{
+  if (nsim * (nr + 1) >= 1024) return field_shape_k(nr, nz, max_lds, 8, 1, 1024);
+  return field_shape_k(nr, nz, max_lds, 4, kFieldMaxSplit, 1024);
+}
+
+template <bool kBatch>
+hipError_t launch_field_shape(const FieldShape& f, const FieldArgs* recs, const FieldArgs& one,
+                              uint32_t nsim, uint32_t nr, uint32_t nz, hipStream_t s)
+{
+  const dim3 grid(f.blocks * nsim), block(f.threads);
+#define HPS_FIELD_LAUNCH(KK, LL)                                                              \
+  if (kBatch)                                                                                 \
+    hipLaunchKernelGGL((k_field_win_batch<KK, LL>), grid, block, f.lds_bytes, s, recs,        \
+                       f.blocks, nr, nz, f.g);                                                \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_field_win<KK, LL>), grid, block, f.lds_bytes, s, one, nr, nz, f.g);
+  if (f.K == 8) {
+    if (f.lds) { HPS_FIELD_LAUNCH(8, true) } else { HPS_FIELD_LAUNCH(8, false) }
+  } else if (f.K == 4) {
+    if (f.lds) { HPS_FIELD_LAUNCH(4, true) } else { HPS_FIELD_LAUNCH(4, false) }
+  } else if (f.K == 2) {
+    if (f.lds) { HPS_FIELD_LAUNCH(2, true) } else { HPS_FIELD_LAUNCH(2, false) }
+  } else {
+    if (f.lds) { HPS_FIELD_LAUNCH(1, true) } else { HPS_FIELD_LAUNCH(1, false) }
+  }
+#undef HPS_FIELD_LAUNCH
+  return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_field(const ScDev* sc, const int64_t* bins, const double* fld1, double* fld2,
-                        uint32_t nr, uint32_t nz, size_t max_lds, hipStream_t s)
+hipError_t launch_field(const FieldArgs& a, uint32_t nr, uint32_t nz, size_t max_lds,
+                        hipStream_t s)
 {
-  FieldArgs a;
-  a.sc = sc; a.bins = bins; a.fld1 = fld1; a.fld2 = fld2;
-  const FieldShape f = field_shape(nr, nz, max_lds);
-  if (f.lds)
-    hipLaunchKernelGGL(k_field_lds, dim3(f.blocks), dim3(f.threads), f.lds_bytes, s, a, nr, nz,
-                       f.zpb);
-  else
-    hipLaunchKernelGGL(k_field, dim3(f.blocks), dim3(f.threads), 0, s, a, nr, nz);
+  const FieldShape f = field_shape(nr, nz, max_lds, 1);
+  hipError_t e = launch_field_shape<false>(f, nullptr, a, 1, nr, nz, s);
+  if (e != hipSuccess || f.g.S == 1) return e;
+  const uint32_t nout = (nz + 1) * (nr + 1);
+  hipLaunchKernelGGL(k_field_combine, dim3((nout + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a,
+                     nout, f.g.S);
   return hipGetLastError();
 }
 
 hipError_t launch_field_batch(const FieldArgs* recs, uint32_t nmem, uint32_t nr, uint32_t nz,
                               size_t max_lds, hipStream_t s)
 {
-  const FieldShape f = field_shape(nr, nz, max_lds);
-  if (f.lds)
-    hipLaunchKernelGGL(k_field_lds_batch, dim3(f.blocks * nmem), dim3(f.threads), f.lds_bytes, s,
-                       recs, f.blocks, nr, nz, f.zpb);
-  else
-    hipLaunchKernelGGL(k_field_batch, dim3(f.blocks * nmem), dim3(f.threads), 0, s, recs,
-                       f.blocks, nr, nz);
-  return hipGetLastError();
+  FieldArgs none{};
+  FieldShape f = field_shape(nr, nz, max_lds, nmem);
+  if (f.g.S != 1) f = field_shape_k(nr, nz, max_lds, f.K, 1, f.threads);
+  return launch_field_shape<true>(f, recs, none, nmem, nr, nz, s);
 }
 
 hipError_t launch_sums(BeamArrays b, const double* a, const double* bb, int filter, Partials pr,

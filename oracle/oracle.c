@@ -21,8 +21,11 @@
  *    truncated to multiples of 2^-36, int64 sums) instead of float atomics
  *    (space_charge_kernel.cu:200-218): order independent, deterministic.
  *  - The field sum of UpdateFldTbl2Kernel (space_charge_kernel.cu:230-342)
- *    uses fused multiply-adds in a canonical order: lane l of 64 sums bins
- *    b = l, l+64, ... sequentially, then a halving tree over the 64 lanes.
+ *    uses fused multiply-adds in a canonical order: with P = min(NextPow2(nr),
+ *    64) partial sums, partial p takes the source rings rs = p, p+P, ... in
+ *    ascending order and, for each, the source rows zs = 0 .. nz-1 in
+ *    ascending order; then a halving tree over the P partials (partials
+ *    with no ring stay +0.0).
  *  - Elementary functions come from orc_math.h (project-owned, see there).
  *  - rsqrt(x) is 1.0/sqrt(x); double->int conversions saturate (orc_to_int).
  *  - Lattice conventions for configurations the reference cannot run
@@ -1291,31 +1294,35 @@ void orc_t_green_table(uint32_t nr, uint32_t nz, double dr_, double dz_, double 
 }
 
 /* Field sum of UpdateFldTbl2Kernel(Double), space_charge_kernel.cu:230-568,
- * in the canonical order described at the top of this file. */
+ * in the canonical order described at the top of this file.  The field
+ * point (zp, rp) sees source bin (zs, rs) through the Green's entry at
+ * dz = zs - zp (zs >= zp, E_z sign -1) or zp - zs - 1 (zs < zp). */
 void orc_t_field(uint32_t nr, uint32_t nz, const int64_t* bins, const double* fld1, double* fld2)
 {
   uint32_t fld_sz = nr * nz;
+  uint32_t P = orc_t_nextpow2(nr);
+  if (P > 64) P = 64;
   #pragma omp parallel for collapse(2) schedule(static)
   for (uint32_t zp = 0; zp <= nz; ++zp)
     for (uint32_t rp = 0; rp <= nr; ++rp) {
       double ax[64], ay[64];
-      for (int l = 0; l < 64; ++l) {
+      for (uint32_t p = 0; p < P; ++p) {
         double tx = 0.0, ty = 0.0;
-        for (uint32_t bi = (uint32_t)l; bi < fld_sz; bi += 64) {
-          double bin = (double)bins[bi] * 0x1p-36;
-          uint32_t rs = bi % nr, zs = bi / nr;
-          double sign = 1.0;
-          uint32_t dz;
-          if (zs >= zp) { dz = zs - zp; sign = -1.0; }
-          else dz = zp - zs - 1;
-          size_t fi = (size_t)dz + (size_t)rs * nz + (size_t)rp * fld_sz;
-          tx = fma(bin, fld1[2 * fi], tx);
-          ty = fma(sign * bin, fld1[2 * fi + 1], ty);
-        }
-        ax[l] = tx; ay[l] = ty;
+        for (uint32_t rs = p; rs < nr; rs += P)
+          for (uint32_t zs = 0; zs < nz; ++zs) {
+            double q = (double)bins[zs * nr + rs] * 0x1p-36;
+            uint32_t dz;
+            double sign;
+            if (zs >= zp) { dz = zs - zp; sign = -1.0; }
+            else { dz = zp - zs - 1; sign = 1.0; }
+            size_t fi = (size_t)dz + (size_t)rs * nz + (size_t)rp * fld_sz;
+            tx = fma(q, fld1[2 * fi], tx);
+            ty = fma(q, sign * fld1[2 * fi + 1], ty);
+          }
+        ax[p] = tx; ay[p] = ty;
       }
-      for (int o = 32; o >= 1; o >>= 1)
-        for (int l = 0; l < o; ++l) { ax[l] = ax[l] + ax[l + o]; ay[l] = ay[l] + ay[l + o]; }
+      for (uint32_t o = P / 2; o >= 1; o >>= 1)
+        for (uint32_t l = 0; l < o; ++l) { ax[l] = ax[l] + ax[l + o]; ay[l] = ay[l] + ay[l + o]; }
       size_t f2 = (size_t)zp * (nr + 1) + rp;
       fld2[2 * f2] = ax[0];
       fld2[2 * f2 + 1] = ay[0];

@@ -132,3 +132,17 @@ def test_batch_rejects_different_layouts_and_keeps_simulators_usable():
     objs[1][2].simulate()
     ref = _oracle(members[1:])
     assert H.compare_arrays(objs[1][0].arrays(), ref[0][0].arrays()) is None
+
+
+@pytest.mark.parametrize("nr,nz,nb", [(32, 128, 40), (30, 60, 40), (64, 256, 16)])
+def test_batch_of_many_small_beams(nr, nz, nb):
+    """Many beams: the batched field sum takes its deepest register window
+    (at 30 x 60 with padded partial lanes and a partial last window; at
+    64 x 256 reading the tables from global memory, as they exceed LDS)."""
+    _, members = _members([600 + 37 * k for k in range(nb)], periods=2, seed=23)
+    objs, _ = _gpu(members, nr=nr, nz=nz)
+    _check(objs, _oracle(members, nr=nr, nz=nz))
+    for k in (0, nb // 2, nb - 1):
+        g, o = objs[k][1].tables(), _oracle(members[k:k + 1], nr=nr, nz=nz)[0][1].tables()
+        for name, a, b in zip(("greens", "field", "bins"), g, o):
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (k, name)

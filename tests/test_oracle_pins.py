@@ -354,10 +354,11 @@ def test_mesh_size_cutoff_follows_beam_energy(oracle_lib):
     assert crossed and ke > mw
 
 
-def test_field_sum_brute_force(L):
-    """Canonical field sum vs a plain double loop of charge x Green's value."""
-    nr, nz = 4, 8
-    rng = np.random.default_rng(6)
+@pytest.mark.parametrize("nr,nz", [(4, 8), (7, 5), (70, 3)])
+def test_field_sum_brute_force(L, nr, nz):
+    """Canonical field sum vs a plain double loop of charge x Green's value
+    (mesh widths below, between and above the 64 partial sums)."""
+    rng = np.random.default_rng(6 + nr)
     bins = rng.integers(0, 2 ** 40, nr * nz).astype(np.int64)
     bins[rng.random(nr * nz) < 0.3] = 0
     f1 = rng.standard_normal((nr + 1) * nr * nz * 2)
