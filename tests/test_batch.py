@@ -146,3 +146,40 @@ def test_batch_of_many_small_beams(nr, nz, nb):
         g, o = objs[k][1].tables(), _oracle(members[k:k + 1], nr=nr, nz=nz)[0][1].tables()
         for name, a, b in zip(("greens", "field", "bins"), g, o):
             assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (k, name)
+
+
+def test_pv_scan_as_one_batch_matches_the_scan_loop(tmp_path):
+    """hpsim_amd.scan.pv_scan (the loop of sim-lbeg-scan-pv.py:140-160 as one
+    batch) against the loop on the GPU: per value set_db_epics, then simulate
+    a fresh copy of the initial beam with a fresh SpaceCharge."""
+    import os
+    import shutil
+    from hpsim_amd import HPSim
+    from hpsim_amd import lcsutil as lcs
+    from hpsim_amd.scan import _copy_beam, pv_scan
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dtl.db")
+    shutil.copyfile(src, tmp_path / "dtl.db")
+    dbc = HPSim.DBConnection(str(tmp_path / "dtl.db"))
+    dbc.clear_model_index()
+    bl = HPSim.BeamLine(dbc)
+    spec = cfg.BeamSpec(num=3000, seed=5, w0=0.75, phi0=-2.758, sig_phi=0.05, sig_w=2e-4)
+    beam = HPSim.Beam(mass=spec.mass, charge=spec.charge, current=0.015, num=spec.num)
+    beam.set_distribution(*cfg.make_distribution(spec))
+    beam.set_frequency(201.25)
+    beam.set_ref_w(0.75)
+    beam.save_initial_beam()
+    sc = HPSim.SpaceCharge(nr=16, nz=64, interval=0.05)
+    pv = lcs.get_pv_asp(1)
+    v0 = float(HPSim.get_db_epics(pv, dbc))
+    values = [v0 * f for f in (0.97, 1.0, 1.03)]
+    stop = bl.index([n for n in bl.get_element_names() if n.startswith("01RG")][30])
+    out = pv_scan(pv, values, beam, dbc, bl, sc, 0, stop)
+    assert float(HPSim.get_db_epics(pv, dbc)) == v0
+    for v, got in zip(values, out):
+        b = _copy_beam(beam)
+        HPSim.set_db_epics(pv, str(v), dbc, bl)
+        sim = HPSim.Simulator(b, bl, HPSim.SpaceCharge(nr=16, nz=64, interval=0.05))
+        sim.simulate(0, stop)
+        assert H.compare_arrays(got.arrays(), b.arrays(), f"value {v}: ") is None
+    HPSim.set_db_epics(pv, str(v0), dbc, bl)
+    dbc.close()
