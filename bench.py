@@ -63,10 +63,13 @@ def parse():
                    help="lattice periods of the all-core CPU-baseline sample")
     p.add_argument("--cpu-serial-periods", type=int, default=10,
                    help="lattice periods of the one-core CPU-baseline sample")
-    p.add_argument("--workload", default="plain", choices=("plain", "small", "batched-scan"),
+    p.add_argument("--workload", default="plain",
+                   choices=("plain", "small", "batched-scan", "stress"),
                    help="plain: the 1,048,576-particle single beam (default); small: the "
                         "65,536-particle single beam on 40 periods; batched-scan: 256 beams "
-                        "of 16K-1M particles with per-beam lattice settings in one batch")
+                        "of 16K-1M particles with per-beam lattice settings in one batch; "
+                        "stress: 64 beams of 4,194,304 particles, 400 periods, 64x256 mesh, "
+                        "in one batch")
     p.add_argument("--beams", type=int, default=256, help="beams of the batched scan")
     return p.parse_args()
 
@@ -129,15 +132,17 @@ def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def batched_cpu_baseline(w, budget_s=20.0):
+def batched_cpu_baseline(w, budget_s=20.0, max_elements=None):
     """OpenMP oracle over whole members of the batch, in order, until about
     budget_s seconds of tracking; pushes/s of that sample (simulate() time
-    only)."""
+    only).  max_elements: track only the first elements of each member."""
     from oracle import oracle as O
     O.use_openmp()
     done = pushes = 0
     busy = 0.0
     for spec, els in zip(w.beams, w.lattices):
+        if max_elements:
+            els = els[:max_elements]
         beam = O.Beam(spec.num, spec.mass, spec.charge, spec.current)
         beam.set_distribution(*_member_dist(spec))
         beam.set_frequency(spec.freq)
@@ -150,10 +155,11 @@ def batched_cpu_baseline(w, budget_s=20.0):
         done += 1
         if busy > budget_s:
             break
+    nel = min(len(w.lattices[0]), max_elements or len(w.lattices[0]))
     return {"value": pushes / busy, "unit": "particle-element pushes/s",
             "cores": O.lib().orc_threads(), "kind": "port",
             "sample": f"first {done} of {len(w.beams)} beams of the batch "
-                      f"({pushes // len(w.lattices[0])} particles x {len(w.lattices[0])} elements), "
+                      f"({pushes // nel} particles x first {nel} elements), "
                       f"OpenMP oracle, one beam after the other, {busy:.1f} s"}
 
 
@@ -163,7 +169,11 @@ def main_batched(args):
     from hpsim_amd import HPSim
     from hpsim_amd import configs as cfg
 
-    w = cfg.batched_scan(nbeams=args.beams, periods=args.periods or 40)
+    if args.workload == "stress":
+        w = cfg.combined_stress(nbeams=args.beams if args.beams != 256 else 64,
+                                num=args.num or 4194304, periods=args.periods or 400)
+    else:
+        w = cfg.batched_scan(nbeams=args.beams, periods=args.periods or 40)
     total = sum(b.num for b in w.beams)
     nel = len(w.lattices[0])
     _log(f"{len(w.beams)} beams, {total} particles, {nel} elements: sampling distributions")
@@ -215,8 +225,8 @@ def main_batched(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (Philox-seeded truncated Gaussian beams, generated lattice with "
-                "seeded per-beam perturbations)",
+        "data": ("synthetic (Philox-seeded truncated Gaussian beams, generated lattice"
+                 + (")" if args.workload == "stress" else " with seeded per-beam perturbations)")),
         "config": {"workload": w.name, "beams": len(w.beams), "particles": total,
                    "particles_min": min(b.num for b in w.beams),
                    "particles_max": max(b.num for b in w.beams), "elements": nel,
@@ -228,10 +238,12 @@ def main_batched(args):
                      "timing": "HIP events around every launch of one extra step"},
         "kernel_ms_per_step": {k: v[0] for k, v in prof.items()},
         "launch": "eager batched",
+        "field_shape_autotuned": dict(zip(("shape", "candidate_ms"), batch.tuning())),
     }
     if not args.no_cpu_baseline:
         _log("cpu baseline")
-        out["cpu_baseline"] = batched_cpu_baseline(w)
+        out["cpu_baseline"] = batched_cpu_baseline(
+            w, max_elements=40 if args.workload == "stress" else None)
     print(json.dumps(out), flush=True)
 
 
@@ -241,7 +253,7 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent beams "
               "shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "batched-scan":
+    if args.workload in ("batched-scan", "stress"):
         return main_batched(args)
     from hpsim_amd import HPSim
     from hpsim_amd import configs as cfg

@@ -523,6 +523,18 @@ class Batch:
     def set_profiling(self, on=True):
         check(_lib().hps_batch_set_profiling(self._h, 1 if on else 0))
 
+    def set_autotune(self, on=True):
+        """Time the field-sum launch shapes at the first rebuild (default on)."""
+        check(_lib().hps_batch_set_autotune(self._h, 1 if on else 0))
+
+    def tuning(self):
+        """(chosen field-sum shape or None, candidate times in ms)."""
+        shape = C.c_uint32()
+        ms = (C.c_float * 16)()
+        check(_lib().hps_batch_get_tuning(self._h, C.byref(shape), ms, 16))
+        times = [t for t in ms if t > 0]
+        return (None if shape.value == 0xFFFFFFFF else shape.value), times
+
     def profile(self):
         ms = (C.c_double * len(self._FAMILIES))()
         n = (C.c_uint32 * len(self._FAMILIES))()

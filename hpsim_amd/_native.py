@@ -79,7 +79,7 @@ EXPORTS = (
     "hps_sim_get_profile", "hps_sim_set_graph", "hps_sim_graph_replays", "hps_probe_math",
     "hps_batch_create", "hps_batch_destroy", "hps_batch_add", "hps_batch_size",
     "hps_batch_simulate", "hps_batch_last_time_ms", "hps_batch_set_profiling",
-    "hps_batch_get_profile",
+    "hps_batch_get_profile", "hps_batch_set_autotune", "hps_batch_get_tuning",
 )
 
 _lib = None
@@ -148,6 +148,8 @@ def _declare(L):
         "hps_batch_last_time_ms": (C.c_int, [P, dp]),
         "hps_batch_set_profiling": (C.c_int, [P, C.c_int]),
         "hps_batch_get_profile": (C.c_int, [P, dp, up]),
+        "hps_batch_set_autotune": (C.c_int, [P, C.c_int]),
+        "hps_batch_get_tuning": (C.c_int, [P, up, C.POINTER(C.c_float), u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -339,3 +339,14 @@ def batched_scan(nbeams=256, nmin=16384, nmax=1048576, periods=40, seed=7):
         beams.append(b)
         lats.append(perturb_lattice(lattice, seed * 7919 + k + 1))
     return BatchWorkload(f"batched-scan-{nbeams}x{periods}", beams, lats)
+
+
+def combined_stress(nbeams=64, num=4194304, periods=400, seed=11):
+    """The combined-stress configuration: nbeams beams of num particles on the
+    65,536-particle configuration's lattice extended to `periods` periods,
+    SCHEFF on a 64 x 256 mesh; beams differ by their seeds."""
+    base = config(0, periods=periods)
+    lattice = build_lattice(base.lattice)
+    beams = [BeamSpec(num=num, seed=seed * 100003 + k + 1) for k in range(nbeams)]
+    return BatchWorkload(f"stress-{nbeams}x{num}x{periods}", beams, [lattice] * nbeams,
+                         nr=64, nz=256)

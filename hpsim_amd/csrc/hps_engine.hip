@@ -1522,6 +1522,12 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
 
 struct hps_batch {
   std::vector<hps_sim*> sims;
+  // field-sum launch shape (UINT32_MAX: not chosen yet); autotune times the
+  // candidates at the first rebuild and keeps the fastest
+  bool autotune = true;
+  uint32_t field_shape = UINT32_MAX;
+  uint32_t field_key[3] = {0, 0, 0};   // nr, nz, batch size the shape was chosen for
+  std::vector<float> field_times;      // ms per candidate of the last tuning
   std::vector<uint8_t> host_rec;      // launch records + block maps of the last call
   uint8_t* d_rec = nullptr;
   size_t rec_cap = 0;
@@ -1532,6 +1538,36 @@ struct hps_batch {
   double prof_ms[HPS_PROFILE_FAMILIES] = {};
   uint32_t prof_n[HPS_PROFILE_FAMILIES] = {};
 };
+
+// Times every field-sum launch shape once on the batch's live tables (the
+// rebuild of a kick point the control step marked for remesh, so the kernels
+// do the full work) and keeps the fastest in bt->field_shape.  Needs eager
+// launches; each candidate overwrites the field tables with the same values.
+static int tune_field(hps_batch* bt, const FieldArgs* d_field, uint32_t nmem, uint32_t nr,
+                      uint32_t nz, size_t max_lds, hipStream_t s)
+{
+  const uint32_t n = field_batch_shapes();
+  std::vector<hipEvent_t> ev(2 * n);
+  for (auto& e : ev) HPS_CHECK(hipEventCreate(&e));
+  // one untimed launch first (instruction caches, table residency)
+  HPS_CHECK(launch_field_batch(d_field, nmem, nr, nz, max_lds, field_batch_default_shape(nr, nmem), s));
+  for (uint32_t k = 0; k < n; ++k) {
+    HPS_CHECK(hipEventRecord(ev[2 * k], s));
+    HPS_CHECK(launch_field_batch(d_field, nmem, nr, nz, max_lds, k, s));
+    HPS_CHECK(hipEventRecord(ev[2 * k + 1], s));
+  }
+  HPS_CHECK(hipEventSynchronize(ev[2 * n - 1]));
+  bt->field_times.assign(n, 0.0f);
+  uint32_t best = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    HPS_CHECK(hipEventElapsedTime(&bt->field_times[k], ev[2 * k], ev[2 * k + 1]));
+    if (bt->field_times[k] < bt->field_times[best]) best = k;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  bt->field_shape = best;
+  bt->field_key[0] = nr; bt->field_key[1] = nz; bt->field_key[2] = nmem;
+  return 0;
+}
 
 extern "C" int hps_batch_create(hps_batch** out)
 {
@@ -1588,6 +1624,23 @@ extern "C" int hps_batch_get_profile(hps_batch* bt, double* ms, uint32_t* launch
 {
   if (!bt || !ms || !launches) return fail("null argument");
   for (int k = 0; k < HPS_PROFILE_FAMILIES; ++k) { ms[k] = bt->prof_ms[k]; launches[k] = bt->prof_n[k]; }
+  return 0;
+}
+
+extern "C" int hps_batch_set_autotune(hps_batch* bt, int on)
+{
+  if (!bt) return fail("null batch");
+  bt->autotune = on != 0;
+  bt->field_shape = UINT32_MAX;
+  bt->field_key[0] = bt->field_key[1] = bt->field_key[2] = 0;
+  return 0;
+}
+
+extern "C" int hps_batch_get_tuning(hps_batch* bt, uint32_t* shape, float* ms, uint32_t n)
+{
+  if (!bt || !shape) return fail("null argument");
+  *shape = bt->field_shape;
+  for (uint32_t k = 0; ms && k < n && k < bt->field_times.size(); ++k) ms[k] = bt->field_times[k];
   return 0;
 }
 
@@ -1700,6 +1753,8 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
   // record layout: chunk map, rebuild map, rebuild and field records, then
   // per launch the records of its step
   const uint32_t nr = sc_on ? m0->sc->cfg.nx : 0, nz = sc_on ? m0->sc->cfg.nz : 0;
+  if (bt->field_key[0] != nr || bt->field_key[1] != nz || bt->field_key[2] != (uint32_t)B)
+    bt->field_shape = bt->autotune ? UINT32_MAX : field_batch_default_shape(nr, (uint32_t)B);
   std::vector<uint32_t> chunk_map, build_map;
   for (size_t k = 0; k < B; ++k) {
     const uint32_t nch = bt->sims[k]->b->pr.nch;
@@ -1801,9 +1856,16 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
         case L_BUILD:
           HPS_CHECK(launch_sc_build_batch(d_build, d_build_map, build_blocks, nr, nz, s));
           break;
-        default:
-          HPS_CHECK(launch_field_batch(d_field, (uint32_t)B, nr, nz, m0->sc->max_lds, s));
+        default: {
+          if (bt->autotune && bt->field_shape == UINT32_MAX) {
+            // first rebuild of this batch shape: time every field launch shape
+            // on the live tables (each gives the same bits) and keep the fastest
+            if (tune_field(bt, d_field, (uint32_t)B, nr, nz, m0->sc->max_lds, s)) return -1;
+          }
+          HPS_CHECK(launch_field_batch(d_field, (uint32_t)B, nr, nz, m0->sc->max_lds,
+                                       bt->field_shape, s));
           break;
+        }
       }
       if (bt->profile) {
         HPS_CHECK(hipEventRecord(bt->evpool[nev + 1], s));

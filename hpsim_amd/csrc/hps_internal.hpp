@@ -139,8 +139,13 @@ hipError_t launch_phase_batch(const PhaseArgs* recs, const uint32_t* map, uint32
 hipError_t launch_phase_final_batch(const PhaseArgs* recs, uint32_t nmem, hipStream_t s);
 hipError_t launch_sc_build_batch(const BuildArgs* recs, const uint32_t* map, uint32_t nblocks,
                                  uint32_t nr, uint32_t nz, hipStream_t s);
+// Field sum of a batch in launch shape `shape` < field_batch_shapes() (any
+// larger value: the default for the batch size); every shape gives the same
+// bits, so a caller may time them and keep the fastest.
+uint32_t field_batch_shapes();
+uint32_t field_batch_default_shape(uint32_t nr, uint32_t nmem);
 hipError_t launch_field_batch(const FieldArgs* recs, uint32_t nmem, uint32_t nr, uint32_t nz,
-                              size_t max_lds, hipStream_t s);
+                              size_t max_lds, uint32_t shape, hipStream_t s);
 
 // Generic canonical sums for the getters: over the filtered particles
 // (filter 0: loss == 0, 1: loss + lloss == 0, 2: all) of a, a*a, b, b*b, a*b,

@@ -1485,13 +1485,35 @@ hipError_t launch_field(const FieldArgs& a, uint32_t nr, uint32_t nz, size_t max
   return hipGetLastError();
 }
 
+namespace {
+
+// Field-sum shapes a batch can use (all partials in one workgroup set): window
+// depth x workgroup size.  Every shape computes the same canonical sums.
+constexpr int kFieldBatchK[] = {8, 4, 2};
+constexpr uint32_t kFieldBatchThreads[] = {1024, 512, 256};
+constexpr uint32_t kFieldBatchShapes = 9;
+
+FieldShape field_batch_shape(uint32_t nr, uint32_t nz, size_t max_lds, uint32_t idx)
+{
+  return field_shape_k(nr, nz, max_lds, kFieldBatchK[idx / 3], 1, kFieldBatchThreads[idx % 3]);
+}
+
+}  // namespace
+
+uint32_t field_batch_shapes() { return kFieldBatchShapes; }
+
+uint32_t field_batch_default_shape(uint32_t nr, uint32_t nmem)
+{
+  return nmem * (nr + 1) >= 1024 ? 0u : 3u;   // K 8 or 4, 1024 threads
+}
+
 hipError_t launch_field_batch(const FieldArgs* recs, uint32_t nmem, uint32_t nr, uint32_t nz,
-                              size_t max_lds, hipStream_t s)
+                              size_t max_lds, uint32_t shape, hipStream_t s)
 {
   FieldArgs none{};
-  FieldShape f = field_shape(nr, nz, max_lds, nmem);
-  if (f.g.S != 1) f = field_shape_k(nr, nz, max_lds, f.K, 1, f.threads);
-  return launch_field_shape<true>(f, recs, none, nmem, nr, nz, s);
+  if (shape >= kFieldBatchShapes) shape = field_batch_default_shape(nr, nmem);
+  return launch_field_shape<true>(field_batch_shape(nr, nz, max_lds, shape), recs, none, nmem,
+                                  nr, nz, s);
 }
 
 hipError_t launch_sums(BeamArrays b, const double* a, const double* bb, int filter, Partials pr,

@@ -238,6 +238,13 @@ uint32_t hps_batch_size(const hps_batch* bt);
 /* simulate(start, end) of every simulator (same index rules as hps_sim_simulate) */
 int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end);
 int hps_batch_last_time_ms(hps_batch* bt, double* out);
+/* Field-sum launch shape autotuning (default on): the first rebuild of a
+ * batch (per mesh size and batch size) times every launch shape of the
+ * field sum on the live tables and keeps the fastest; all shapes give the
+ * same bits.  get_tuning: the chosen shape (UINT32_MAX: none yet) and up to
+ * n candidate times in ms.  No reference counterpart. */
+int hps_batch_set_autotune(hps_batch* bt, int on);
+int hps_batch_get_tuning(hps_batch* bt, uint32_t* shape, float* ms, uint32_t n);
 /* per-kernel-family device time of the batched launches (families as
  * hps_sim_set_profiling) */
 int hps_batch_set_profiling(hps_batch* bt, int on);

@@ -140,8 +140,10 @@ def test_batch_of_many_small_beams(nr, nz, nb):
     (at 30 x 60 with padded partial lanes and a partial last window; at
     64 x 256 reading the tables from global memory, as they exceed LDS)."""
     _, members = _members([600 + 37 * k for k in range(nb)], periods=2, seed=23)
-    objs, _ = _gpu(members, nr=nr, nz=nz)
+    objs, batch = _gpu(members, nr=nr, nz=nz)
     _check(objs, _oracle(members, nr=nr, nz=nz))
+    shape, times = batch.tuning()          # the field shapes were timed and one kept
+    assert shape is not None and len(times) == 9 and times[shape] == min(times)
     for k in (0, nb // 2, nb - 1):
         g, o = objs[k][1].tables(), _oracle(members[k:k + 1], nr=nr, nz=nz)[0][1].tables()
         for name, a, b in zip(("greens", "field", "bins"), g, o):
