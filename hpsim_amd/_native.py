@@ -80,6 +80,9 @@ EXPORTS = (
     "hps_batch_create", "hps_batch_destroy", "hps_batch_add", "hps_batch_size",
     "hps_batch_simulate", "hps_batch_last_time_ms", "hps_batch_set_profiling",
     "hps_batch_get_profile", "hps_batch_set_autotune", "hps_batch_get_tuning",
+    "hps_nn_create", "hps_nn_destroy", "hps_nn_num_params", "hps_nn_get_params",
+    "hps_nn_set_params", "hps_nn_set_data", "hps_nn_train", "hps_nn_predict",
+    "hps_nn_loss_grad", "hps_nn_layers",
 )
 
 _lib = None
@@ -90,6 +93,7 @@ def _declare(L):
     PP = C.POINTER(C.c_void_p)
     dp = C.POINTER(C.c_double)
     up = C.POINTER(C.c_uint32)
+    fp = C.POINTER(C.c_float)
     i32 = C.c_int32
     u32 = C.c_uint32
     d = C.c_double
@@ -149,6 +153,16 @@ def _declare(L):
         "hps_batch_set_profiling": (C.c_int, [P, C.c_int]),
         "hps_batch_get_profile": (C.c_int, [P, dp, up]),
         "hps_batch_set_autotune": (C.c_int, [P, C.c_int]),
+        "hps_nn_create": (C.c_int, [PP, u32, u32, u32, u32, u32]),
+        "hps_nn_destroy": (C.c_int, [P]),
+        "hps_nn_num_params": (u32, [P]),
+        "hps_nn_get_params": (C.c_int, [P, fp]),
+        "hps_nn_set_params": (C.c_int, [P, fp]),
+        "hps_nn_set_data": (C.c_int, [P, fp, fp, u32]),
+        "hps_nn_train": (C.c_int, [P, u32, u32, C.c_float, u32, fp]),
+        "hps_nn_predict": (C.c_int, [P, fp, u32, fp]),
+        "hps_nn_loss_grad": (C.c_int, [P, fp, fp, u32, fp, fp]),
+        "hps_nn_layers": (C.c_int, [P, C.POINTER(C.c_int32), u32, up]),
         "hps_batch_get_tuning": (C.c_int, [P, up, C.POINTER(C.c_float), u32]),
     }
     for name, (res, args) in sig.items():

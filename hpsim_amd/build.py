@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libhpsim_amd.so")
-SOURCES = ["hps_kernels.hip", "hps_engine.hip"]
+SOURCES = ["hps_kernels.hip", "hps_engine.hip", "hps_nn.hip", "hps_surrogate.hip"]
 ARCH = os.environ.get("HPSIM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -49,6 +49,15 @@ uint32_t next_pow2(uint32_t x)   // utility.cpp:7-16
 
 hipStream_t g_stream[16];
 
+}  // namespace
+
+namespace hps {
+// the error text of hps_last_error(), for the other translation units
+int fail_msg(const std::string& m) { return fail(m); }
+}  // namespace hps
+
+namespace {
+
 int stream_for(int dev, hipStream_t* out)
 {
   if (dev < 0 || dev >= 16) return fail("device index out of range");

@@ -250,6 +250,33 @@ int hps_batch_get_tuning(hps_batch* bt, uint32_t* shape, float* ms, uint32_t n);
 int hps_batch_set_profiling(hps_batch* bt, int on);
 int hps_batch_get_profile(hps_batch* bt, double* ms, uint32_t* launches);
 
+/* ---- SCHEFF field surrogate network ---- */
+/* A dilated convolutional network on the (nz+1) x (nr+1) field mesh (input
+ * [H][W][4]: charge / total charge, log(dz*gamma/dr), z/nz - 0.5, r/nr;
+ * output [H][W][2]: E_r, E_z in units of c1 * Q / dr^2), BF16 weights, FP32
+ * accumulation on MFMA, trained with Adam by the library's own forward and
+ * backward kernels.  No reference counterpart: the reference always solves
+ * SCHEFF (space_charge.cu:140-210). */
+typedef struct hps_nn hps_nn;
+int hps_nn_create(hps_nn** out, uint32_t H, uint32_t W, uint32_t width, uint32_t max_batch,
+                  uint32_t seed);
+int hps_nn_destroy(hps_nn* net);
+uint32_t hps_nn_num_params(const hps_nn* net);
+int hps_nn_get_params(hps_nn* net, float* host);
+int hps_nn_set_params(hps_nn* net, const float* host);
+/* training set: x [count][H][W][4], y [count][H][W][2] (host, copied to the device) */
+int hps_nn_set_data(hps_nn* net, const float* x, const float* y, uint32_t count);
+/* `steps` Adam steps on random batches; losses[k] (optional) the mean relative
+ * L2 loss of step k's batch */
+int hps_nn_train(hps_nn* net, uint32_t steps, uint32_t batch, float lr, uint32_t seed,
+                 float* losses);
+int hps_nn_predict(hps_nn* net, const float* x, uint32_t count, float* out);
+/* per-sample losses and the gradient of their mean (for gradient checks) */
+int hps_nn_loss_grad(hps_nn* net, const float* x, const float* y, uint32_t count, float* loss,
+                     float* grad);
+/* layer table rows {Cin, Cout, taps, dilation, relu, residual, weight offset, bias offset} */
+int hps_nn_layers(const hps_nn* net, int32_t* table, uint32_t max_layers, uint32_t* nlayers);
+
 /* ---- elementary-function probes (host-callable, run on the device) ---- */
 /* Evaluates fn over n inputs on the GPU: 0 exp, 1 log, 2 sin, 3 cos, 4 sinh,
  * 5 cosh, 6 sqrt, 7 fmod(x, 2pi), 8 x/y, 9 Bessel I0, 10 Bessel I1,

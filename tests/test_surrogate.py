@@ -1,0 +1,92 @@
+"""The learned SCHEFF field surrogate: the float64 reference network
+(oracle/nn_ref.py) pinned by finite differences; the GPU network (BF16
+weights, FP32 accumulation on MFMA) checked against it, forward and
+gradients; training on fields from the engine's own SCHEFF solve."""
+import numpy as np
+import pytest
+
+from oracle import nn_ref
+
+
+def _table(width, cin=4, cout_last=16):
+    """The layer table of hps_surrogate.hip's architecture."""
+    rows, off = [], 0
+    for (ci, co, taps, dil, relu, resid) in ([(cin, width, 9, 1, 1, 0)] +
+                                            [(width, width, 9, d, 1, 1) for d in (2, 4, 8, 16, 32)] +
+                                            [(width, cout_last, 1, 1, 0, 0)]):
+        rows.append((ci, co, taps, dil, relu, resid, off, off + co * taps * ci))
+        off += co * taps * ci + co
+    return rows, off
+
+
+def test_reference_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    table, n = _table(4, cout_last=2)
+    p = rng.normal(0, 0.3, n)
+    x = rng.normal(0, 1, (2, 9, 7, 4))
+    y = rng.normal(0, 1, (2, 9, 7, 2))
+    loss, g = nn_ref.loss_and_grad(p, table, x, y)
+    for i in rng.choice(n, 40, replace=False):
+        h = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (nn_ref.loss_and_grad(pp, table, x, y)[0].mean() -
+              nn_ref.loss_and_grad(pm, table, x, y)[0].mean()) / (2 * h)
+        assert g[i] == pytest.approx(fd, rel=1e-5, abs=1e-8)
+
+
+@pytest.mark.gpu
+def test_gpu_network_matches_reference_forward_and_gradient():
+    from hpsim_amd.surrogate import Network
+    H, W, width, B = 21, 11, 32, 3
+    net = Network(H, W, width=width, max_batch=4, seed=5)
+    table = [tuple(r) for r in net.layers()]
+    assert table == _table(width)[0]
+    rng = np.random.default_rng(8)
+    p = net.params()
+    p[:] = rng.normal(0, 0.08, p.size).astype(np.float32)
+    net.set_params(p)
+    p = net.params()
+    x = rng.normal(0, 1, (B, H, W, 4)).astype(np.float32)
+    y = rng.normal(0, 1, (B, H, W, 2)).astype(np.float32)
+    ref_pred, _ = nn_ref.forward(p.astype(np.float64), table, x)
+    pred = net.predict(x)
+    # BF16 operands (8 significant bits) with FP32 accumulation
+    rel = np.linalg.norm(pred - ref_pred[..., :2]) / np.linalg.norm(ref_pred[..., :2])
+    assert rel < 2e-2, rel
+    loss, g = net.loss_grad(x, y)
+    rloss, rg = nn_ref.loss_and_grad(p.astype(np.float64), table, x, y)
+    assert np.allclose(loss, rloss, rtol=2e-2)
+    cos = float(g @ rg / (np.linalg.norm(g) * np.linalg.norm(rg)))
+    assert cos > 0.999, cos
+    assert np.linalg.norm(g - rg) / np.linalg.norm(rg) < 4e-2
+    # every layer's weights and biases receive their own gradient
+    for (ci, co, taps, dil, relu, resid, wo, bo) in table:
+        gw, rw = g[wo:bo], rg[wo:bo]
+        assert np.linalg.norm(gw - rw) / np.linalg.norm(rw) < 6e-2, (wo, bo)
+
+
+@pytest.mark.gpu
+def test_gpu_training_fits_a_fixed_linear_operator():
+    """Training with the library's Adam on x -> y = (a smooth linear operator
+    of channel 0) drives the loss far below that of the untrained network."""
+    from hpsim_amd.surrogate import Network
+    H, W = 33, 17
+    rng = np.random.default_rng(2)
+    n = 256
+    x = np.zeros((n, H, W, 4), np.float32)
+    x[..., 0] = rng.random((n, H, W)).astype(np.float32)
+    x[..., 2] = (np.arange(H) / H - 0.5)[None, :, None]
+    x[..., 3] = (np.arange(W) / W)[None, None, :]
+    # target: two fixed blurs of the charge
+    k = np.exp(-0.5 * (np.arange(-4, 5) / 2.0) ** 2)
+    k /= k.sum()
+    blur = np.apply_along_axis(lambda v: np.convolve(v, k, mode="same"), 1, x[..., 0])
+    blur = np.apply_along_axis(lambda v: np.convolve(v, k, mode="same"), 2, blur)
+    y = np.stack([blur, np.gradient(blur, axis=1)], -1).astype(np.float32)
+    net = Network(H, W, width=32, max_batch=16, seed=3)
+    net.set_data(x, y)
+    losses = net.train(300, batch=16, lr=2e-3, seed=4)
+    assert np.isfinite(losses).all()
+    assert losses[-20:].mean() < 0.1 * losses[:5].mean()
