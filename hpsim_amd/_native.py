@@ -82,7 +82,7 @@ EXPORTS = (
     "hps_batch_get_profile", "hps_batch_set_autotune", "hps_batch_get_tuning",
     "hps_nn_create", "hps_nn_destroy", "hps_nn_num_params", "hps_nn_get_params",
     "hps_nn_set_params", "hps_nn_set_data", "hps_nn_train", "hps_nn_predict",
-    "hps_nn_loss_grad", "hps_nn_layers",
+    "hps_nn_loss_grad", "hps_nn_layers", "hps_nn_set_output_scale", "hps_nn_get_output_scale",
 )
 
 _lib = None
@@ -163,6 +163,8 @@ def _declare(L):
         "hps_nn_predict": (C.c_int, [P, fp, u32, fp]),
         "hps_nn_loss_grad": (C.c_int, [P, fp, fp, u32, fp, fp]),
         "hps_nn_layers": (C.c_int, [P, C.POINTER(C.c_int32), u32, up]),
+        "hps_nn_set_output_scale": (C.c_int, [P, C.c_float]),
+        "hps_nn_get_output_scale": (C.c_int, [P, fp]),
         "hps_batch_get_tuning": (C.c_int, [P, up, C.POINTER(C.c_float), u32]),
     }
     for name, (res, args) in sig.items():

@@ -53,6 +53,7 @@ struct NetImpl {
   int ndata = 0;
   int32_t* idx = nullptr;
   int step = 0;
+  float out_scale = 1.0f;                  // the field is the output times this
   hipStream_t s = nullptr;
 };
 
@@ -293,7 +294,8 @@ int hps_nn_train(hps_nn* net, uint32_t steps, uint32_t batch, float lr, uint32_t
   return 0;
 }
 
-// prediction (channels 0, 1) of `count` samples: x [count][H][W][4] -> out [count][H][W][2]
+// prediction (channels 0, 1, times the output scale) of `count` samples:
+// x [count][H][W][4] -> out [count][H][W][2]
 int hps_nn_predict(hps_nn* net, const float* x, uint32_t count, float* out)
 {
   if (!net || !x || !out) return fail_msg("null argument");
@@ -307,8 +309,8 @@ int hps_nn_predict(hps_nn* net, const float* x, uint32_t count, float* out)
     NN_CHECK(hipMemcpyAsync(full.data(), n->out.back(), B * hw * kOutC * 4, hipMemcpyDeviceToHost, n->s));
     NN_CHECK(hipStreamSynchronize(n->s));
     for (size_t i = 0; i < B * hw; ++i) {
-      out[(c0 * hw + i) * 2] = full[i * kOutC];
-      out[(c0 * hw + i) * 2 + 1] = full[i * kOutC + 1];
+      out[(c0 * hw + i) * 2] = full[i * kOutC] * n->out_scale;
+      out[(c0 * hw + i) * 2 + 1] = full[i * kOutC + 1] * n->out_scale;
     }
   }
   return 0;
@@ -329,6 +331,20 @@ int hps_nn_loss_grad(hps_nn* net, const float* x, const float* y, uint32_t count
   NN_CHECK(hipMemcpyAsync(loss, n->loss, count * 4, hipMemcpyDeviceToHost, n->s));
   NN_CHECK(hipMemcpyAsync(grad, n->g, n->nparam * 4, hipMemcpyDeviceToHost, n->s));
   NN_CHECK(hipStreamSynchronize(n->s));
+  return 0;
+}
+
+int hps_nn_set_output_scale(hps_nn* net, float v)
+{
+  if (!net || !(v > 0.0f)) return fail_msg("the output scale must be positive");
+  net->impl.out_scale = v;
+  return 0;
+}
+
+int hps_nn_get_output_scale(const hps_nn* net, float* v)
+{
+  if (!net || !v) return fail_msg("null argument");
+  *v = net->impl.out_scale;
   return 0;
 }
 

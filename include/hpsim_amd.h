@@ -274,6 +274,10 @@ int hps_nn_predict(hps_nn* net, const float* x, uint32_t count, float* out);
 /* per-sample losses and the gradient of their mean (for gradient checks) */
 int hps_nn_loss_grad(hps_nn* net, const float* x, const float* y, uint32_t count, float* loss,
                      float* grad);
+/* The field is the network output times this scale (training targets are
+ * divided by it); hps_nn_predict applies it. */
+int hps_nn_set_output_scale(hps_nn* net, float v);
+int hps_nn_get_output_scale(const hps_nn* net, float* v);
 /* layer table rows {Cin, Cout, taps, dilation, relu, residual, weight offset, bias offset} */
 int hps_nn_layers(const hps_nn* net, int32_t* table, uint32_t max_layers, uint32_t* nlayers);
 
