@@ -182,6 +182,14 @@ struct Op {
   GapUniform gu;          // GAP1 / GAP2 element constants (gap_uniform)
 };
 
+// An op read through the constant address space: the op table does not
+// change while a kernel runs, so its fields (uniform across the workgroup)
+// load with scalar loads into SGPRs instead of vector loads into VGPRs,
+// which the compiler cannot prove safe for a generic pointer once the kernel
+// has stored particles.  The maps take their element arguments by template
+// type for this reason.
+typedef const __attribute__((address_space(4))) Op OpConst;
+
 // Bessi0c, simulation_kernel.cu:15-36
 __device__ __forceinline__ double bessi0c(double r_x)
 {
@@ -256,8 +264,8 @@ __device__ __forceinline__ void bessel_all(double r_x, double& b0, double& b1, d
 }
 
 // GetTransitTimeFactors, simulation_kernel.cu:290-308
-__device__ __forceinline__ void ttf(double r_beta, const hps_gap_param& g, double& t,
-                                    double& tp, double& sp)
+template <class G>
+__device__ __forceinline__ void ttf(double r_beta, const G& g, double& t, double& tp, double& sp)
 {
   double beta = r_beta;
   if (r_beta > 1.1 * g.fit_beta_center) beta = g.fit_beta_center;
@@ -340,8 +348,9 @@ __device__ __forceinline__ void half_quad(Particle& p, double length, double ape
 }
 
 // SimulateRFGapFirstHalfKernel body, simulation_kernel.cu:239-285
-__device__ __forceinline__ void gap_first_half(Particle& p, const hps_gap_param& gap,
-                                               const GapUniform& u, bool ccl, double mass)
+template <class G, class U>
+__device__ __forceinline__ void gap_first_half(Particle& p, const G& gap, const U& u, bool ccl,
+                                               double mass)
 {
   const double x = p.x, y = p.y, xp = p.xp, yp = p.yp, phi = p.phi, w = p.w;
   p.x = x + u.dd1 * xp;
@@ -366,8 +375,9 @@ __device__ __forceinline__ void gap_first_half(Particle& p, const hps_gap_param&
 }
 
 // SimulateRFGapSecondHalfKernel body, simulation_kernel.cu:326-493
-__device__ __forceinline__ void gap_second_half(Particle& p, const hps_gap_param& gap,
-                                                const GapUniform& u, double wavelen, bool ccl,
+template <class G, class U>
+__device__ __forceinline__ void gap_second_half(Particle& p, const G& gap, const U& u,
+                                                double wavelen, bool ccl,
                                                 double mass)
 {
   double x = p.x, y = p.y, xp = p.xp, yp = p.yp;
@@ -479,7 +489,8 @@ __device__ __forceinline__ void steerer(Particle& p, double blh, double blv, dou
 
 // Apply one map step to an alive particle.
 // SimulateBuncherKernel body, simulation_kernel.cu:807-874 (rsqrt as 1/sqrt).
-__device__ __forceinline__ void buncher(Particle& p, const double* a, uint32_t elem, double mass,
+template <class D>
+__device__ __forceinline__ void buncher(Particle& p, D* a, uint32_t elem, double mass,
                                         double charge)
 {
   const double w_ref = a[0], freq = a[1], voltage = a[2], phase = a[3], aper = a[4];
@@ -525,7 +536,8 @@ struct DipoleHalf {
   double r11, r12, r16, r21, r22, r26, r33, r34, r43, r44, r51, r52, r56, beta, dl;
 };
 
-__device__ __forceinline__ DipoleHalf dipole_half(const double* a, double mass)
+template <class D>
+__device__ __forceinline__ DipoleHalf dipole_half(D* a, double mass)
 {
   DipoleHalf o;
   const double rho = a[0], angle = a[1], findx = a[7], design_w = a[8];
@@ -561,7 +573,8 @@ __device__ __forceinline__ DipoleHalf dipole_half(const double* a, double mass)
   return o;
 }
 
-__device__ __forceinline__ void dipole_edge(const double* a, double b, double& r21, double& r43)
+template <class D>
+__device__ __forceinline__ void dipole_edge(D* a, double b, double& r21, double& r43)
 {
   const double h = 1.0 / a[0];
   const double c1 = 2.0 * a[4] * a[5] * h;
@@ -573,7 +586,8 @@ __device__ __forceinline__ void dipole_edge(const double* a, double b, double& r
   r43 = hps::tan(b - psi) * h;
 }
 
-__device__ __forceinline__ void dipole_first_half(Particle& p, const double* a, double mass)
+template <class D>
+__device__ __forceinline__ void dipole_first_half(Particle& p, D* a, double mass)
 {
   double er21, er43;
   dipole_edge(a, a[2], er21, er43);
@@ -592,7 +606,8 @@ __device__ __forceinline__ void dipole_first_half(Particle& p, const double* a, 
   p.phi += TWOPI * (-(d.r51 * x + d.r52 * xp + d.r56 * dpop) + d.dl) / (d.beta * wavelen);
 }
 
-__device__ __forceinline__ void dipole_second_half(Particle& p, const double* a, double mass)
+template <class D>
+__device__ __forceinline__ void dipole_second_half(Particle& p, D* a, double mass)
 {
   double er21, er43;
   dipole_edge(a, a[3], er21, er43);
@@ -613,8 +628,8 @@ __device__ __forceinline__ void dipole_second_half(Particle& p, const double* a,
   p.yp = yp - er43 * y_new;
 }
 
-template <int kClass>
-__device__ __forceinline__ void apply_op(const Op& op, Particle& p, uint32_t index, double mass,
+template <int kClass, class OpT>
+__device__ __forceinline__ void apply_op(const OpT& op, Particle& p, uint32_t index, double mass,
                                          double charge, double x_avg, double y_avg)
 {
   switch (op.kind) {

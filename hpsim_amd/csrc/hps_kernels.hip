@@ -159,9 +159,15 @@ struct KickConst {
 // the parts fixed by the control step come from ScDev (kick_prepare in
 // k_phase_final); the element length, the on-mesh count (k_deposit) and the
 // phase references (which a host frequency step may change) are read here.
-__device__ __forceinline__ KickConst kick_const(const ScDev* sc, const hps_beam_stats* st,
+__device__ __forceinline__ KickConst kick_const(const ScDev* scg, const hps_beam_stats* stg,
                                                 double r_length)
 {
+  // read through the constant address space (scalar loads: neither record
+  // changes while k_segment runs)
+  typedef const __attribute__((address_space(4))) ScDev CScDev;
+  typedef const __attribute__((address_space(4))) hps_beam_stats CStats;
+  CScDev* sc = (CScDev*)scg;
+  CStats* st = (CStats*)stg;
   KickConst k;
   k.phimc = st->phi_avg_r;
   k.xbar = sc->k_xbar;
@@ -389,14 +395,17 @@ __device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSm
   KickConst k{};
   if (a.kick) k = kick_const(a.sc, a.st, a.kick_len);
   const double xa = a.st->x_avg, ya = a.st->y_avg;
-  double acc[kScQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  uint32_t cnt = 0;
-  double mx = -__builtin_huge_val();
-  double r2m = 0.0;
+  // per-lane accumulators in LDS rows (lane t owns column t): 0..5 sums,
+  // 6 largest r^2, ured count, mred max phi.  Held in LDS rather than
+  // registers, they leave the registers to the maps: the RF-gap class fits
+  // three waves per SIMD instead of two (163 vs 177 VGPRs).
+  double* const acc = sm.red + t;
+#pragma unroll
+  for (int q = 0; q < kScQ + 1; ++q) acc[q * kBlock] = 0.0;
+  sm.ured[t] = 0;
+  sm.mred[t] = -__builtin_huge_val();
 #pragma unroll 1
   for (int j0 = 0; j0 < (int)kPerLane; j0 += kGroup) {
-    // kGroup particles of this lane in flight: the maps of one element run
-    // over the whole group, giving the scheduler independent chains
     Particle p[kGroup];
     bool alive_in[kGroup];
 #pragma unroll
@@ -414,7 +423,7 @@ __device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSm
       }
     }
     for (int o = a.op_begin; o < a.op_end; ++o) {
-      const Op& op = a.ops[o];
+      const OpConst& op = ((const OpConst*)a.ops)[o];
 #pragma unroll
       for (int g = 0; g < kGroup; ++g) {
         const uint32_t i = c * kChunk + (j0 + g) * kBlock + t;
@@ -432,37 +441,43 @@ __device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSm
         if (q.loss != 0) b.loss[i] = q.loss;
       }
       if (a.stats == STAT_SC) {
-        if (in && q.lloss == 0) mx = ::fmax(mx, q.phi);
+        if (in && q.lloss == 0) sm.mred[t] = ::fmax(sm.mred[t], q.phi);
         const bool gd = in && is_good(q.loss, q.lloss);
         const double vphi = gd ? q.phi : 0.0, vx = gd ? q.x : 0.0, vy = gd ? q.y : 0.0,
                      vw = gd ? q.w : 0.0;
         acc[0] = acc[0] + vphi;
-        acc[1] = acc[1] + vx;
-        acc[2] = acc[2] + vx * vx;
-        acc[3] = acc[3] + vy;
-        acc[4] = acc[4] + vy * vy;
-        acc[5] = acc[5] + vw;
-        cnt += gd ? 1u : 0u;
+        acc[kBlock] = acc[kBlock] + vx;
+        acc[2 * kBlock] = acc[2 * kBlock] + vx * vx;
+        acc[3 * kBlock] = acc[3 * kBlock] + vy;
+        acc[4 * kBlock] = acc[4 * kBlock] + vy * vy;
+        acc[5 * kBlock] = acc[5 * kBlock] + vw;
+        sm.ured[t] += gd ? 1u : 0u;
         if (a.rmax && in && q.loss == 0) {
-          // MaxR2PhiReduceKernel's first pass (beam_kernel.cu:826-843)
           const double tx = q.x - xa, ty = q.y - ya;
           const double r2 = tx * tx + ty * ty;
-          r2m = r2m < r2 ? r2 : r2m;
+          const double r2m = acc[6 * kBlock];
+          acc[6 * kBlock] = r2m < r2 ? r2 : r2m;
         }
       } else if (a.stats == STAT_APERTURE) {
         const bool ok = in && q.loss == 0;
         acc[0] = acc[0] + (ok ? q.x : 0.0);
-        acc[1] = acc[1] + (ok ? q.y : 0.0);
-        cnt += (in && q.loss != 0) ? 1u : 0u;
+        acc[kBlock] = acc[kBlock] + (ok ? q.y : 0.0);
+        sm.ured[t] += (in && q.loss != 0) ? 1u : 0u;
       }
     }
   }
   const Partials& pr = a.pr;
   if (a.stats == STAT_SC) {
-    block_reduce<kScQ>(acc, cnt, mx, sm.red, sm.ured, sm.mred, t);
+    double v[kScQ];
+#pragma unroll
+    for (int q = 0; q < kScQ; ++q) v[q] = acc[q * kBlock];
+    const double r2m = acc[6 * kBlock];
+    uint32_t cnt = sm.ured[t];
+    double mx = sm.mred[t];
+    block_reduce<kScQ>(v, cnt, mx, sm.red, sm.ured, sm.mred, t);
     if (t == 0) {
 #pragma unroll
-      for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = acc[q];
+      for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = v[q];
       pr.cnt[c] = cnt;
       pr.pmax[c] = mx;
     }
@@ -471,7 +486,8 @@ __device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSm
       if (t == 0) pr.rmax2[c] = m;
     }
   } else if (a.stats == STAT_APERTURE) {
-    double ap[2] = {acc[0], acc[1]};
+    double ap[2] = {acc[0], acc[kBlock]};
+    uint32_t cnt = sm.ured[t];
     double m0 = 0.0;
     block_reduce<2>(ap, cnt, m0, sm.red, sm.ured, sm.mred, t);
     if (t == 0) { pr.sums[c] = ap[0]; pr.sums[pr.mpad + c] = ap[1]; pr.cnt[c] = cnt; }
