@@ -147,6 +147,23 @@ __device__ __forceinline__ bool is_good(uint32_t loss, uint32_t lloss)
   return (uint32_t)(loss + lloss) == 0u;
 }
 
+// Loads and stores through global-address-space pointers.  Pointers that a
+// kernel reads from device memory (the per-simulation records of the
+// batched launches) are generic to the compiler, which then emits flat
+// accesses (64-bit VGPR addresses, waits on both memory counters); every
+// buffer of the engine is global memory.
+template <class T>
+__device__ __forceinline__ T gld(const T* p, size_t i)
+{
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+
+template <class T>
+__device__ __forceinline__ void gst(T* p, size_t i, T v)
+{
+  ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+
 // --- SCHEFF kick (ApplySpchKickKernel, space_charge_kernel.cu:572-706) -----
 
 struct KickConst {
@@ -276,10 +293,12 @@ __device__ __forceinline__ void kick(Particle& p, const KickConst& k, const doub
     uint32_t m = l + (k.nx + 1);
     if (m >= tbl_sz) m = tbl_sz - 1;
     if (l >= tbl_sz) l = tbl_sz - 1;
-    cbgr = k.er_ratio * k.c3 * (d * (a * tbl[2 * l] + b * tbl[2 * (l - 1)]) +
-           c * (a * tbl[2 * m] + b * tbl[2 * (m - 1)]));
-    cbgzs = k.ez_ratio * k.c3 * (d * (a * tbl[2 * l + 1] + b * tbl[2 * (l - 1) + 1]) +
-            c * (a * tbl[2 * m + 1] + b * tbl[2 * (m - 1) + 1]));
+    const double tl0 = gld(tbl, 2 * l), tl1 = gld(tbl, 2 * l + 1);
+    const double tk0 = gld(tbl, 2 * (l - 1)), tk1 = gld(tbl, 2 * (l - 1) + 1);
+    const double tm0 = gld(tbl, 2 * m), tm1 = gld(tbl, 2 * m + 1);
+    const double tn0 = gld(tbl, 2 * (m - 1)), tn1 = gld(tbl, 2 * (m - 1) + 1);
+    cbgr = k.er_ratio * k.c3 * (d * (a * tl0 + b * tk0) + c * (a * tm0 + b * tn0));
+    cbgzs = k.ez_ratio * k.c3 * (d * (a * tl1 + b * tk1) + c * (a * tm1 + b * tn1));
   }
   bgx += cbgr * xovr;
   bgy += cbgr * yovr;
@@ -367,10 +386,10 @@ __device__ __forceinline__ Particle load_particle(const BeamArrays& b, uint32_t 
   p.loss = 1; p.lloss = 0;
   p.x = p.xp = p.y = p.yp = p.phi = p.w = 0.0;
   if (i < b.n) {
-    p.loss = b.loss[i];
-    p.lloss = b.lloss[i];
-    p.x = b.x[i]; p.xp = b.xp[i]; p.y = b.y[i]; p.yp = b.yp[i];
-    p.phi = b.phi[i]; p.w = b.w[i];
+    p.loss = gld(b.loss, i);
+    p.lloss = gld(b.lloss, i);
+    p.x = gld(b.x, i); p.xp = gld(b.xp, i); p.y = gld(b.y, i); p.yp = gld(b.yp, i);
+    p.phi = gld(b.phi, i); p.w = gld(b.w, i);
   }
   return p;
 }
@@ -436,9 +455,9 @@ __device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSm
       const bool in = i < b.n;
       const Particle& q = p[g];
       if (alive_in[g]) {
-        b.x[i] = q.x; b.xp[i] = q.xp; b.y[i] = q.y; b.yp[i] = q.yp;
-        b.phi[i] = q.phi; b.w[i] = q.w;
-        if (q.loss != 0) b.loss[i] = q.loss;
+        gst(b.x, i, q.x); gst(b.xp, i, q.xp); gst(b.y, i, q.y); gst(b.yp, i, q.yp);
+        gst(b.phi, i, q.phi); gst(b.w, i, q.w);
+        if (q.loss != 0) gst(b.loss, i, q.loss);
       }
       if (a.stats == STAT_SC) {
         if (in && q.lloss == 0) sm.mred[t] = ::fmax(sm.mred[t], q.phi);
