@@ -329,6 +329,8 @@ int hps_beam_create(hps_beam** out, uint32_t num, double mass, double charge, do
 {
   if (!out) return fail("null output handle");
   if (num == 0) return fail("Beam needs num > 0 particles");
+  // the cross-chunk reductions fold at most 4096 chunks of 1024 particles
+  if (num > 4096u * kChunk) return fail("a beam holds at most 4,194,304 particles");
   hps_beam* b = new hps_beam();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) { delete b; return fail("no HIP device"); }
@@ -349,7 +351,6 @@ int hps_beam_create(hps_beam** out, uint32_t num, double mass, double charge, do
        hipMemset(b->st, 0, sizeof(hps_beam_stats)) == hipSuccess;
   b->pr.nch = (num + kChunk - 1) / kChunk;
   b->pr.mpad = next_pow2(b->pr.nch);
-  if (b->pr.mpad > 4096) { delete b; return fail("a beam holds at most 4,194,304 particles"); }
   const size_t nsum = (size_t)kMaxQ * b->pr.mpad + b->pr.mpad / 2 + 256;
   ok = ok && hipMalloc(&b->pr.sums, nsum * sizeof(double)) == hipSuccess &&
        hipMemset(b->pr.sums, 0, nsum * sizeof(double)) == hipSuccess;
