@@ -328,38 +328,67 @@ struct RedSmem {
   uint32_t ured[kBlock];
 };
 
-// Recomputes the statistics partials of chunk c from memory (after k_seg_final
-// set new lloss flags in it); the same lane order as the accumulation in
-// k_segment.
-__device__ void sc_chunk_stats(const BeamArrays& b, uint32_t c, int t, RedSmem& sm,
-                               const Partials& pr)
+// Sets the longitudinal-loss flag of every particle of chunk c beyond the
+// threshold thr and recomputes the chunk's statistics partials with the new
+// flags, in one pass over the chunk, on one wavefront: lane L stands for the
+// canonical lanes L, L+64, L+128 and L+192 of k_segment's 256-lane order
+// (each the same sequential sum over j), then folds them as levels 128 and 64
+// of the halving tree before the shuffle levels 32..1 -- the same tree as
+// block_reduce, so the partials are identical.  The waves of a workgroup
+// rescan different chunks at the same time.
+__device__ void chunk_flag_stats_wave(const BeamArrays& b, uint32_t c, double thr, int lane,
+                                      const Partials& pr)
 {
-  double acc[kScQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  constexpr int kSlots = kBlock / 64;
+  double acc[kSlots][kScQ];
   uint32_t cnt = 0;
   double mx = -__builtin_huge_val();
 #pragma unroll
-  for (int j = 0; j < (int)kPerLane; ++j) {
-    const uint32_t i = c * kChunk + j * kBlock + t;
-    double phi = 0.0, x = 0.0, y = 0.0, w = 0.0;
-    uint32_t loss = 1, lloss = 0;
-    if (i < b.n) {
-      phi = b.phi[i]; x = b.x[i]; y = b.y[i]; w = b.w[i]; loss = b.loss[i]; lloss = b.lloss[i];
-      if (lloss == 0) mx = ::fmax(mx, phi);
-    }
-    const bool g = (i < b.n) && is_good(loss, lloss);
-    const double vphi = g ? phi : 0.0, vx = g ? x : 0.0, vy = g ? y : 0.0, vw = g ? w : 0.0;
-    acc[0] = acc[0] + vphi;
-    acc[1] = acc[1] + vx;
-    acc[2] = acc[2] + vx * vx;
-    acc[3] = acc[3] + vy;
-    acc[4] = acc[4] + vy * vy;
-    acc[5] = acc[5] + vw;
-    cnt += g ? 1u : 0u;
-  }
-  block_reduce<kScQ>(acc, cnt, mx, sm.red, sm.ured, sm.mred, t);
-  if (t == 0) {
+  for (int sl = 0; sl < kSlots; ++sl) {
 #pragma unroll
-    for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = acc[q];
+    for (int q = 0; q < kScQ; ++q) acc[sl][q] = 0.0;
+#pragma unroll
+    for (int j = 0; j < (int)kPerLane; ++j) {
+      const uint32_t i = c * kChunk + j * kBlock + sl * 64 + lane;
+      double phi = 0.0, x = 0.0, y = 0.0, w = 0.0;
+      uint32_t loss = 1, lloss = 0;
+      if (i < b.n) {
+        phi = b.phi[i]; x = b.x[i]; y = b.y[i]; w = b.w[i]; loss = b.loss[i]; lloss = b.lloss[i];
+        if (phi > thr) {
+          if (lloss != 1) b.lloss[i] = 1;
+          lloss = 1;
+        }
+        if (lloss == 0) mx = ::fmax(mx, phi);
+      }
+      const bool g = (i < b.n) && is_good(loss, lloss);
+      const double vphi = g ? phi : 0.0, vx = g ? x : 0.0, vy = g ? y : 0.0, vw = g ? w : 0.0;
+      acc[sl][0] = acc[sl][0] + vphi;
+      acc[sl][1] = acc[sl][1] + vx;
+      acc[sl][2] = acc[sl][2] + vx * vx;
+      acc[sl][3] = acc[sl][3] + vy;
+      acc[sl][4] = acc[sl][4] + vy * vy;
+      acc[sl][5] = acc[sl][5] + vw;
+      cnt += g ? 1u : 0u;
+    }
+  }
+  static_assert(kSlots == 4, "the fold below is written for 256 canonical lanes");
+  double v[kScQ];
+#pragma unroll
+  for (int q = 0; q < kScQ; ++q) {
+    const double l0 = acc[0][q] + acc[2][q];   // level 128: lane L += lane L+128
+    const double l1 = acc[1][q] + acc[3][q];   //            lane L+64 += lane L+192
+    v[q] = l0 + l1;                            // level 64
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+    for (int q = 0; q < kScQ; ++q) v[q] = v[q] + __shfl_down(v[q], o, 64);
+    cnt += __shfl_down(cnt, o, 64);
+    mx = ::fmax(mx, __shfl_down(mx, o, 64));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = v[q];
     pr.cnt[c] = cnt;
     pr.pmax[c] = mx;
   }
@@ -601,17 +630,9 @@ __device__ __forceinline__ void seg_final_body(const SegArgs& a)
     if (nfound == 0) break;
     any = true;
     const uint32_t take = nfound < 64 ? nfound : 64;
-    for (uint32_t r = 0; r < take; ++r) {
-      const uint32_t cc = rescan_list[r];
-#pragma unroll
-      for (int j = 0; j < (int)kPerLane; ++j) {
-        const uint32_t i = cc * kChunk + j * kBlock + t;
-        if (i < b.n && b.phi[i] > thr) b.lloss[i] = 1;
-      }
-      __syncthreads();
-      sc_chunk_stats(b, cc, t, sm, pr);
-      __syncthreads();
-    }
+    for (uint32_t r = (uint32_t)t / 64; r < take; r += kBlock / 64)
+      chunk_flag_stats_wave(b, rescan_list[r], thr, t & 63, pr);
+    __syncthreads();
   }
   uint32_t g2 = g1;
   if (any)

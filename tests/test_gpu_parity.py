@@ -134,11 +134,14 @@ def test_graph_replay_same_simulator():
     assert sim.graph_replays() == 2
 
 
-def test_longitudinal_loss_rescan_bit_exact():
+@pytest.mark.parametrize("num", [5000, 150000])
+def test_longitudinal_loss_rescan_bit_exact(num):
     """A 70-degree phase spread pushes particles out of the bucket, so the
-    lloss flags and the chunk rescan path are exercised."""
+    lloss flags and the chunk rescan path are exercised: 5 chunks (one per
+    wavefront of the rescanning workgroup, one wave taking two) and 147
+    chunks (more than the 64 one rescan round lists)."""
     import math
-    wl = H.workload(0, num=5000, periods=6, sc=True)
+    wl = H.workload(0, num=num, periods=6, sc=True)
     wl.beam.sig_phi = math.radians(70.0)
     ob, osc, _ = H.run_oracle(wl)
     gb, gsc, _ = H.run_gpu(wl)
