@@ -839,8 +839,14 @@ int hps_spch_create(hps_spch** out, uint32_t nr, uint32_t nz, double interval, u
       lds > 0)
     s->max_lds = (size_t)lds;
   if (hipMalloc(&s->dev, sizeof(ScDev)) != hipSuccess ||
-      hipMemset(s->dev, 0, sizeof(ScDev)) != hipSuccess || spch_alloc_tables(s) || spch_push_cfg(s)) {
+      hipMemset(s->dev, 0, sizeof(ScDev)) != hipSuccess) {
     (void)hipFree(s->dev);
+    delete s;
+    return fail("device allocation failed for the SpaceCharge");
+  }
+  if (spch_alloc_tables(s) || spch_push_cfg(s)) {   // these set the error message
+    (void)hipFree(s->dev); (void)hipFree(s->fld1); (void)hipFree(s->fld2); (void)hipFree(s->bins);
+    (void)hipFree(s->part);
     delete s;
     return -1;
   }
@@ -934,7 +940,12 @@ int hps_sim_create(hps_sim** out, hps_beam* b, hps_beamline* bl, hps_spch* s)
   if (!out || !b || !bl) return fail("Simulator needs a beam and a beamline");
   hps_sim* m = new hps_sim();
   m->b = b; m->bl = bl; m->sc = s; m->sc_on = s != nullptr;
-  if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess) {
+  if (hipEventCreate(&m->ev0) != hipSuccess) {
+    delete m;
+    return fail("event creation failed");
+  }
+  if (hipEventCreate(&m->ev1) != hipSuccess) {
+    (void)hipEventDestroy(m->ev0);
     delete m;
     return fail("event creation failed");
   }
