@@ -445,8 +445,8 @@ __device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSm
   const double xa = a.st->x_avg, ya = a.st->y_avg;
   // per-lane accumulators in LDS rows (lane t owns column t): 0..5 sums,
   // 6 largest r^2, ured count, mred max phi.  Held in LDS rather than
-  // registers, they leave the registers to the maps: the RF-gap class fits
-  // three waves per SIMD instead of two (163 vs 177 VGPRs).
+  // registers across the particle loop, they leave the registers to the
+  // maps (14 fewer VGPRs in the RF-gap class).
   double* const acc = sm.red + t;
 #pragma unroll
   for (int q = 0; q < kScQ + 1; ++q) acc[q * kBlock] = 0.0;
