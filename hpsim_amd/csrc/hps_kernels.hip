@@ -440,8 +440,6 @@ __device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSm
   static_assert(kPerLane % kGroup == 0, "group must divide the particles per lane");
   const int t = threadIdx.x;
   const BeamArrays& b = a.b;
-  KickConst k{};
-  if (a.kick) k = kick_const(a.sc, a.st, a.kick_len);
   const double xa = a.st->x_avg, ya = a.st->y_avg;
   // per-lane accumulators in LDS rows (lane t owns column t): 0..5 sums,
   // 6 largest r^2, ured count, mred max phi.  Held in LDS rather than
@@ -463,6 +461,12 @@ __device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSm
       alive_in[g] = i < b.n && p[g].loss == 0;
     }
     if (a.kick) {
+      // the kick constants are read again for every particle group (scalar
+      // loads from the scalar cache) rather than held in SGPRs across the maps
+      const ScDev* scp = a.sc;
+      const hps_beam_stats* stp = a.st;
+      asm volatile("" : "+s"(scp), "+s"(stp));
+      const KickConst k = kick_const(scp, stp, a.kick_len);
 #pragma unroll
       for (int g = 0; g < kGroup; ++g) {
         const uint32_t i = c * kChunk + (j0 + g) * kBlock + t;
