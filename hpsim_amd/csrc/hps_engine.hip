@@ -1790,7 +1790,7 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
       BuildArgs& a = build[k];
       a.sc = m->sc->dev; a.fld1 = m->sc->fld1; a.b = m->b->d; a.st = m->b->st; a.bins = m->sc->bins;
       a.greens_blocks = gb;
-      a.dep_blocks = m->b->blck_size / 4 != 0 ? deposit_blocks(m->b->n) : 0;
+      a.dep_blocks = m->b->blck_size / 4 != 0 ? deposit_blocks(m->b->n, 4) : 0;
       for (uint32_t c = 0; c < gb + a.dep_blocks; ++c)
         build_map.push_back(((uint32_t)k << kBatchLocalBits) | c);
       FieldArgs& f = field[k];

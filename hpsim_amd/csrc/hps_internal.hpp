@@ -119,7 +119,9 @@ hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s);
 constexpr uint32_t kDepositBlock = 1024;
 constexpr uint32_t kDepositMaxBlocks = 256;
 constexpr size_t kBlockLdsReserve = 4 * kDepositBlock;
-uint32_t deposit_blocks(uint32_t n);
+// deposit workgroups for n particles, per_thread particles per thread (at most
+// kDepositMaxBlocks workgroups)
+uint32_t deposit_blocks(uint32_t n, uint32_t per_thread);
 // Green's table + charge deposit (deposit = false for beams whose reference
 // deposit launch is empty).
 hipError_t launch_sc_build(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, BeamArrays b,
