@@ -565,6 +565,149 @@ __global__ __launch_bounds__(kBlock) void k_segment_batch(const SegArgs* __restr
   segment_body<kClass, kGroup>(a, e & kBatchLocalMask, sm);
 }
 
+// The same push for beams too small to fill the GPU with k_segment's 256-thread
+// workgroups: one chunk per 1024-thread workgroup, one particle per thread.
+// Thread j*256 + l pushes particle i = chunk*1024 + j*256 + l (the particle
+// canonical lane l takes as its j-th) and stores its statistics
+// contributions in LDS; lane l (< 256) then adds its four in the canonical
+// order j = 0..3 from +0.0, and the 256 lane sums go through block_reduce's
+// halving tree -- the chunk partials are k_segment's bit for bit.
+constexpr uint32_t kWideBlock = kChunk;
+// at most this many chunks (k_segment would run at most one wavefront per SIMD)
+constexpr uint32_t kWideMaxChunks = 256;
+struct WideSmem {
+  double q[kScQ][kWideBlock];   // contributions, then lane sums in [q][0, 256)
+  double mx[kWideBlock];        // phi of particles not longitudinally flagged, else -inf
+  double r2[kWideBlock];        // r^2 about (x_avg, y_avg), loss == 0
+  uint32_t cnt[kWideBlock];     // good (STAT_SC) or lost (STAT_APERTURE) flags
+};
+
+template <int kClass>
+__device__ __forceinline__ void segment_body_wide(const SegArgs& a, uint32_t c, WideSmem& w)
+{
+  const int t = threadIdx.x;
+  const BeamArrays& b = a.b;
+  const uint32_t i = c * kChunk + (uint32_t)t;
+  Particle p = load_particle(b, i);
+  const bool alive_in = i < b.n && p.loss == 0;
+  if (a.kick && alive_in && p.lloss == 0 && (i & a.kick_mask) != 0) {
+    const KickConst k = kick_const(a.sc, a.st, a.kick_len);
+    kick(p, k, a.fld2, b.mass);
+  }
+  const double xa = a.st->x_avg, ya = a.st->y_avg;
+  for (int o = a.op_begin; o < a.op_end; ++o) {
+    const OpConst& op = ((const OpConst*)a.ops)[o];
+    if (p.loss == 0) apply_op<kClass>(op, p, i, b.mass, b.charge, xa, ya);
+  }
+  const bool in = i < b.n;
+  if (alive_in) {
+    gst(b.x, i, p.x); gst(b.xp, i, p.xp); gst(b.y, i, p.y); gst(b.yp, i, p.yp);
+    gst(b.phi, i, p.phi); gst(b.w, i, p.w);
+    if (p.loss != 0) gst(b.loss, i, p.loss);
+  }
+  if (a.stats == STAT_NONE) return;
+  const Partials& pr = a.pr;
+  if (a.stats == STAT_SC) {
+    const bool gd = in && is_good(p.loss, p.lloss);
+    const double vx = gd ? p.x : 0.0, vy = gd ? p.y : 0.0;
+    w.q[0][t] = gd ? p.phi : 0.0;
+    w.q[1][t] = vx;
+    w.q[2][t] = vx * vx;
+    w.q[3][t] = vy;
+    w.q[4][t] = vy * vy;
+    w.q[5][t] = gd ? p.w : 0.0;
+    w.cnt[t] = gd ? 1u : 0u;
+    w.mx[t] = (in && p.lloss == 0) ? p.phi : -__builtin_huge_val();
+    double r2 = 0.0;
+    if (a.rmax && in && p.loss == 0) {
+      // MaxR2PhiReduceKernel's first pass (beam_kernel.cu:826-843)
+      const double tx = p.x - xa, ty = p.y - ya;
+      r2 = tx * tx + ty * ty;
+    }
+    w.r2[t] = r2;
+  } else {
+    const bool ok = in && p.loss == 0;
+    w.q[0][t] = ok ? p.x : 0.0;
+    w.q[1][t] = ok ? p.y : 0.0;
+    w.cnt[t] = (in && p.loss != 0) ? 1u : 0u;
+    w.mx[t] = -__builtin_huge_val();
+    w.r2[t] = 0.0;
+  }
+  __syncthreads();
+  const int nq = a.stats == STAT_SC ? kScQ : 2;
+  // lane sums in the canonical order over j, kept in [q][0, 256)
+  if (t < (int)kBlock) {
+    for (int q = 0; q < nq; ++q) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < (int)kPerLane; ++j) acc = acc + w.q[q][j * kBlock + t];
+      w.q[q][t] = acc;
+    }
+    uint32_t n = 0;
+    double m = -__builtin_huge_val(), r = 0.0;
+#pragma unroll
+    for (int j = 0; j < (int)kPerLane; ++j) {
+      n += w.cnt[j * kBlock + t];
+      m = ::fmax(m, w.mx[j * kBlock + t]);
+      const double u = w.r2[j * kBlock + t];
+      r = r < u ? u : r;
+    }
+    w.cnt[t] = n;
+    w.mx[t] = m;
+    w.r2[t] = r;
+  }
+  __syncthreads();
+  // block_reduce's levels over the 256 lane sums
+  if (t < 128) {
+    for (int q = 0; q < nq; ++q) w.q[q][t] = w.q[q][t] + w.q[q][t + 128];
+    w.cnt[t] += w.cnt[t + 128];
+    w.mx[t] = ::fmax(w.mx[t], w.mx[t + 128]);
+    const double u = w.r2[t + 128];
+    w.r2[t] = w.r2[t] < u ? u : w.r2[t];
+  }
+  __syncthreads();
+  if (t < 64) {
+    double v[kScQ];
+    for (int q = 0; q < kScQ; ++q) v[q] = q < nq ? w.q[q][t] + w.q[q][t + 64] : 0.0;
+    uint32_t n = w.cnt[t] + w.cnt[t + 64];
+    double m = ::fmax(w.mx[t], w.mx[t + 64]);
+    double r = w.r2[t];
+    {
+      const double u = w.r2[t + 64];
+      r = r < u ? u : r;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+      for (int q = 0; q < kScQ; ++q) v[q] = v[q] + __shfl_down(v[q], o, 64);
+      n += __shfl_down(n, o, 64);
+      m = ::fmax(m, __shfl_down(m, o, 64));
+      const double u = __shfl_down(r, o, 64);
+      r = r < u ? u : r;
+    }
+    if (t == 0) {
+      if (a.stats == STAT_SC) {
+#pragma unroll
+        for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = v[q];
+        pr.cnt[c] = n;
+        pr.pmax[c] = m;
+        if (a.rmax) pr.rmax2[c] = r;
+      } else {
+        pr.sums[c] = v[0];
+        pr.sums[pr.mpad + c] = v[1];
+        pr.cnt[c] = n;
+      }
+    }
+  }
+}
+
+template <int kClass>
+__global__ __launch_bounds__(kWideBlock) void k_segment_wide(SegArgs a)
+{
+  extern __shared__ double wide_lds[];
+  segment_body_wide<kClass>(a, blockIdx.x, *reinterpret_cast<WideSmem*>(wide_lds));
+}
+
 // Cross-chunk step of the k_segment statistics in one workgroup, launched
 // after k_segment so that kernel order, not a device-wide fence per
 // workgroup, makes the partials visible: the aperture loss count and
@@ -1363,6 +1506,16 @@ inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 hipError_t launch_segment(const SegArgs& a, hipStream_t s)
 {
   const uint32_t nch = (a.b.n + kChunk - 1) / kChunk;
+  if (nch <= kWideMaxChunks) {
+    const size_t lds = sizeof(WideSmem);
+    if (a.op_class >= OPC_GAP)
+      hipLaunchKernelGGL((k_segment_wide<OPC_GAP>), dim3(nch), dim3(kWideBlock), lds, s, a);
+    else if (a.op_class == OPC_QUAD)
+      hipLaunchKernelGGL((k_segment_wide<OPC_QUAD>), dim3(nch), dim3(kWideBlock), lds, s, a);
+    else
+      hipLaunchKernelGGL((k_segment_wide<OPC_LIGHT>), dim3(nch), dim3(kWideBlock), lds, s, a);
+    return hipGetLastError();
+  }
   if (a.op_class >= OPC_GAP)
     hipLaunchKernelGGL((k_segment<OPC_GAP, kSegGroupGap>), dim3(nch), dim3(kBlock), 0, s, a);
   else if (a.op_class == OPC_QUAD)
