@@ -845,6 +845,60 @@ __device__ __forceinline__ void phase_body(const PhaseArgs& a, uint32_t c)
 
 __global__ __launch_bounds__(kBlock) void k_phase(PhaseArgs a) { phase_body(a, blockIdx.x); }
 
+// The same for beams of at most kWideMaxChunks chunks, one particle per
+// thread (as k_segment_wide): contributions in LDS, lane l (< 256) adds its
+// four in the canonical order, then block_reduce's halving tree.
+__global__ __launch_bounds__(kWideBlock) void k_phase_wide(PhaseArgs a)
+{
+  __shared__ double w[3][kWideBlock];
+  const int t = threadIdx.x;
+  const BeamArrays& b = a.b;
+  const Partials& pr = a.pr;
+  const uint32_t c = blockIdx.x;
+  const double ref = a.st->phi_avg_good;
+  const uint32_t i = c * kChunk + (uint32_t)t;
+  double pr_ = 0.0;
+  uint32_t loss = 1, lloss = 0;
+  if (i < b.n) { loss = b.loss[i]; lloss = b.lloss[i]; pr_ = relative_phi(b.phi[i], ref); }
+  const bool tg = (i < b.n) && loss == 0;
+  const bool g = (i < b.n) && is_good(loss, lloss);
+  const double vg = g ? pr_ : 0.0;
+  w[0][t] = tg ? pr_ : 0.0;
+  w[1][t] = vg;
+  w[2][t] = vg * vg;
+  __syncthreads();
+  if (t < (int)kBlock) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < (int)kPerLane; ++j) acc = acc + w[q][j * kBlock + t];
+      w[q][t] = acc;
+    }
+  }
+  __syncthreads();
+  if (t < 128) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) w[q][t] = w[q][t] + w[q][t + 128];
+  }
+  __syncthreads();
+  if (t < 64) {
+    double v[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[q] = w[q][t] + w[q][t + 64];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) v[q] = v[q] + __shfl_down(v[q], o, 64);
+    }
+    if (t == 0) {
+      pr.sums[c] = v[0];
+      pr.sums[pr.mpad + c] = v[1];
+      pr.sums[2 * pr.mpad + c] = v[2];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_phase_batch(const PhaseArgs* recs, const uint32_t* map)
 {
   const uint32_t e = map[blockIdx.x];
@@ -1555,7 +1609,10 @@ hipError_t launch_seg_final_batch(const SegArgs* recs, uint32_t nmem, hipStream_
 hipError_t launch_phase(const PhaseArgs& a, hipStream_t s)
 {
   const uint32_t nch = (a.b.n + kChunk - 1) / kChunk;
-  hipLaunchKernelGGL(k_phase, dim3(nch), dim3(kBlock), 0, s, a);
+  if (nch <= kWideMaxChunks)
+    hipLaunchKernelGGL(k_phase_wide, dim3(nch), dim3(kWideBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_phase, dim3(nch), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
