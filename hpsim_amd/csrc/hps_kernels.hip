@@ -423,6 +423,15 @@ __device__ __forceinline__ Particle load_particle(const BeamArrays& b, uint32_t 
   return p;
 }
 
+// Writes back a pushed particle that was alive on entry (the map kernels write
+// only those); a new loss code is written with it.
+__device__ __forceinline__ void store_particle(const BeamArrays& b, uint32_t i, const Particle& q)
+{
+  gst(b.x, i, q.x); gst(b.xp, i, q.xp); gst(b.y, i, q.y); gst(b.yp, i, q.yp);
+  gst(b.phi, i, q.phi); gst(b.w, i, q.w);
+  if (q.loss != 0) gst(b.loss, i, q.loss);
+}
+
 // One workgroup per 1024-particle chunk; each lane carries its four
 // particles (i = chunk*1024 + j*256 + lane) through the kick, the maps and
 // the store in groups of kGroup, and accumulates the chunk statistics in the
@@ -487,11 +496,7 @@ __device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSm
       const uint32_t i = c * kChunk + (j0 + g) * kBlock + t;
       const bool in = i < b.n;
       const Particle& q = p[g];
-      if (alive_in[g]) {
-        gst(b.x, i, q.x); gst(b.xp, i, q.xp); gst(b.y, i, q.y); gst(b.yp, i, q.yp);
-        gst(b.phi, i, q.phi); gst(b.w, i, q.w);
-        if (q.loss != 0) gst(b.loss, i, q.loss);
-      }
+      if (alive_in[g]) store_particle(b, i, q);
       if (a.stats == STAT_SC) {
         if (in && q.lloss == 0) sm.mred[t] = ::fmax(sm.mred[t], q.phi);
         const bool gd = in && is_good(q.loss, q.lloss);
@@ -600,11 +605,7 @@ __device__ __forceinline__ void segment_body_wide(const SegArgs& a, uint32_t c, 
     if (p.loss == 0) apply_op<kClass>(op, p, i, b.mass, b.charge, xa, ya);
   }
   const bool in = i < b.n;
-  if (alive_in) {
-    gst(b.x, i, p.x); gst(b.xp, i, p.xp); gst(b.y, i, p.y); gst(b.yp, i, p.yp);
-    gst(b.phi, i, p.phi); gst(b.w, i, p.w);
-    if (p.loss != 0) gst(b.loss, i, p.loss);
-  }
+  if (alive_in) store_particle(b, i, p);
   if (a.stats == STAT_NONE) return;
   const Partials& pr = a.pr;
   if (a.stats == STAT_SC) {
