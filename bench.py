@@ -315,9 +315,9 @@ def main():
                    "transmitted": b.num - lost},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(b.num),
-                     # beams of at most 256 chunks of 1024 particles run the
+                     # beams of at most 512 chunks of 1024 particles run the
                      # one-particle-per-thread form of the segment kernel
-                     "kernel": "k_segment_wide" if b.num <= 256 * 1024 else "k_segment",
+                     "kernel": "k_segment_wide" if b.num <= 512 * 1024 else "k_segment",
                      "avg_launch_us": avg_s * 1e6,
                      "launches": seg_n, "bytes_per_launch": SEG_BYTES_PER_PARTICLE * b.num,
                      "timing": "HIP events around every launch of one eager step after "

@@ -56,9 +56,9 @@ def test_tracking_no_space_charge_bit_exact(num, periods):
     assert err is None, err
 
 
-# beams up to 262,144 particles push one particle per thread (k_segment_wide),
+# beams up to 524,288 particles push one particle per thread (k_segment_wide),
 # larger ones four per thread (k_segment): both paths are covered
-@pytest.mark.parametrize("num,periods", [(3000, 2), (4096, 5), (300000, 2)])
+@pytest.mark.parametrize("num,periods", [(3000, 2), (4096, 5), (600000, 2)])
 def test_tracking_space_charge_bit_exact(num, periods):
     wl = H.workload(0, num=num, periods=periods, sc=True)
     ob, osc, _ = H.run_oracle(wl)
@@ -136,12 +136,12 @@ def test_graph_replay_same_simulator():
     assert sim.graph_replays() == 2
 
 
-@pytest.mark.parametrize("num", [5000, 150000, 300000])
+@pytest.mark.parametrize("num", [5000, 150000, 600000])
 def test_longitudinal_loss_rescan_bit_exact(num):
     """A 70-degree phase spread pushes particles out of the bucket, so the
     lloss flags and the chunk rescan path are exercised: 5 chunks (one per
     wavefront of the rescanning workgroup, one wave taking two), 147 chunks
-    (more than the 64 one rescan round lists) and 293 chunks (the
+    (more than the 64 one rescan round lists) and 586 chunks (the
     four-particles-per-thread segment kernel)."""
     import math
     wl = H.workload(0, num=num, periods=6, sc=True)
