@@ -27,6 +27,23 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// Loads and stores through global-address-space pointers.  Pointers that a
+// kernel reads from device memory (the per-simulation records of the
+// batched launches) are generic to the compiler, which then emits flat
+// accesses (64-bit VGPR addresses, waits on both memory counters); every
+// buffer of the engine is global memory.
+template <class T>
+__device__ __forceinline__ T gld(const T* p, size_t i)
+{
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+
+template <class T>
+__device__ __forceinline__ void gst(T* p, size_t i, T v)
+{
+  ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+
 // --- canonical reductions -------------------------------------------------
 
 // Canonical halving tree over the 256 lane values of Q sums (levels 128 and
@@ -88,44 +105,71 @@ __device__ __forceinline__ double block_max(double v, double* buf, int t)
   return r;
 }
 
+// Thread t's entries c = t + 256 k, k < K, of the cross-chunk trees below
+// (mpad = 256 K, or K = 1 and mpad <= 256): the Q partial sums folded by the
+// upper levels of the halving tree, the integer counts C[c] of c < nch (C
+// may be null), and, when M is given, the largest M[c] of c < nch (-inf if
+// none; a NaN entry is never taken).  Every load is issued before the first
+// addition, so the K * (Q + 2) loads share one round trip.
+template <int Q, int K>
+__device__ __forceinline__ void fold_entries(const double* P, const uint32_t* C, const double* M,
+                                             uint32_t mpad, uint32_t nch, int t, double (&v)[Q],
+                                             uint32_t& cnt, double& mx)
+{
+  const bool in = K > 1 || (uint32_t)t < mpad;
+  double r[Q][K];
+  uint32_t n[K];
+  double m[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t c = (uint32_t)t + kBlock * k;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) r[q][k] = in ? gld(P, (size_t)q * mpad + c) : 0.0;
+    n[k] = (C && in) ? gld(C, c) : 0u;
+    m[k] = (M && in) ? gld(M, c) : -__builtin_huge_val();
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+#pragma unroll
+    for (int h = K / 2; h >= 1; h /= 2) {
+#pragma unroll
+      for (int k = 0; k < h; ++k) r[q][k] = r[q][k] + r[q][k + h];
+    }
+    v[q] = r[q][0];
+  }
+  cnt = 0;
+  mx = -__builtin_huge_val();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool live = (uint32_t)t + kBlock * k < nch;
+    cnt += live ? n[k] : 0u;
+    if (live && m[k] > mx) mx = m[k];
+  }
+}
+
 // Cross-chunk trees of Q quantities stored P[q * mpad + c] (zero padded to
 // mpad, a power of two) and the sum of the integer counts C[0, nch).  Thread
 // t first folds its entries t + 256k in registers (the upper levels of the
 // halving tree), then one block tree finishes.  Every thread gets the
-// results.  mpad <= 4096 (16 entries per thread).
+// results; with M given, *own_max receives the largest M[c] over the
+// thread's own entries c < nch.  mpad <= 4096 (16 entries per thread).
 template <int Q>
 __device__ void final_reduce(const double* P, const uint32_t* C, uint32_t mpad, uint32_t nch,
                              double* red, uint32_t* ured, double* mred, int t, double (&out)[Q],
-                             uint32_t& cnt_out)
+                             uint32_t& cnt_out, const double* M = nullptr,
+                             double* own_max = nullptr)
 {
-  const uint32_t K = mpad > (uint32_t)kBlock ? mpad / kBlock : 1;
   double v[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    double r[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const uint32_t c = t + kBlock * k;
-      r[k] = ((uint32_t)k < K && c < mpad) ? P[q * mpad + c] : 0.0;
-    }
-    if (K >= 16) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) r[k] = r[k] + r[k + 8];
-    }
-    if (K >= 8) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) r[k] = r[k] + r[k + 4];
-    }
-    if (K >= 4) {
-      r[0] = r[0] + r[2];
-      r[1] = r[1] + r[3];
-    }
-    if (K >= 2) r[0] = r[0] + r[1];
-    v[q] = r[0];
+  uint32_t cnt;
+  double own;
+  switch (mpad > (uint32_t)kBlock ? mpad / kBlock : 1u) {
+    case 1: fold_entries<Q, 1>(P, C, M, mpad, nch, t, v, cnt, own); break;
+    case 2: fold_entries<Q, 2>(P, C, M, mpad, nch, t, v, cnt, own); break;
+    case 4: fold_entries<Q, 4>(P, C, M, mpad, nch, t, v, cnt, own); break;
+    case 8: fold_entries<Q, 8>(P, C, M, mpad, nch, t, v, cnt, own); break;
+    default: fold_entries<Q, 16>(P, C, M, mpad, nch, t, v, cnt, own); break;
   }
-  uint32_t cnt = 0;
-  if (C)
-    for (uint32_t c = t; c < nch; c += kBlock) cnt += C[c];
+  if (own_max) *own_max = own;
   double mx = 0.0;
   block_reduce<Q>(v, cnt, mx, red, ured, mred, t);
   __shared__ double bcast[kMaxQ];
@@ -145,23 +189,6 @@ __device__ void final_reduce(const double* P, const uint32_t* C, uint32_t mpad, 
 __device__ __forceinline__ bool is_good(uint32_t loss, uint32_t lloss)
 {
   return (uint32_t)(loss + lloss) == 0u;
-}
-
-// Loads and stores through global-address-space pointers.  Pointers that a
-// kernel reads from device memory (the per-simulation records of the
-// batched launches) are generic to the compiler, which then emits flat
-// accesses (64-bit VGPR addresses, waits on both memory counters); every
-// buffer of the engine is global memory.
-template <class T>
-__device__ __forceinline__ T gld(const T* p, size_t i)
-{
-  return ((const __attribute__((address_space(1))) T*)p)[i];
-}
-
-template <class T>
-__device__ __forceinline__ void gst(T* p, size_t i, T v)
-{
-  ((__attribute__((address_space(1))) T*)p)[i] = v;
 }
 
 // --- SCHEFF kick (ApplySpchKickKernel, space_charge_kernel.cu:572-706) -----
@@ -742,26 +769,18 @@ __device__ __forceinline__ void seg_final_body(const SegArgs& a)
   hps_beam_stats* st = a.st;
   const uint32_t num_loss = st->num_loss;
   // the chunk maxima are loaded together with the partials (one round trip)
-  const uint32_t K = pr.mpad > (uint32_t)kBlock ? pr.mpad / kBlock : 1;
-  double pm[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const uint32_t cc = t + kBlock * k;
-    pm[k] = ((uint32_t)k < K && cc < pr.nch) ? pr.pmax[cc] : -__builtin_huge_val();
-  }
   double s[kScQ];
   uint32_t g1;
-  final_reduce<kScQ>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, g1);
+  double pm;
+  final_reduce<kScQ>(pr.sums, pr.cnt, pr.mpad, pr.nch, sm.red, sm.ured, sm.mred, t, s, g1,
+                     pr.pmax, &pm);
   const double avg1 = g1 > 0 ? s[0] / (double)g1 : -99999;
   const double thr = avg1 + TWOPI + PI;
   // Chunks where a not-yet-flagged particle is beyond the threshold get their
   // lloss flags set and their partials recomputed (rare: once per particle).
   if (t == 0) rescan_n = 0;
   __syncthreads();
-  bool hit = false;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) hit = hit || pm[k] > thr;
-  if (hit) atomicAdd(&rescan_n, 1u);
+  if (pm > thr) atomicAdd(&rescan_n, 1u);
   __syncthreads();
   bool any = false;
   for (uint32_t it = 0; rescan_n != 0 && it < pr.nch / 64 + 2; ++it) {
