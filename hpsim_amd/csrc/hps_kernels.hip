@@ -1310,11 +1310,29 @@ __device__ __forceinline__ void field_win_body(const FieldArgs& f, uint32_t nr, 
   if (kLds) {
     double2* lg = reinterpret_cast<double2*>(lds);   // [nz][nc] (E_r, E_z)
     double* lq = lds + 2 * (size_t)nz * nc;           // [nz][nc] charge
-    for (uint32_t i = threadIdx.x; i < nz * nc; i += blockDim.x) {
-      const uint32_t z = i / nc, c = i % nc;
-      const uint32_t src = z * nr + set + c * S;
-      lg[i] = gsrc[src];
-      lq[i] = (double)f.bins[src] * 0x1p-36;
+    // kStage entries per thread are loaded before any is stored: one memory
+    // round trip per kStage entries instead of one per entry.  Indexes past
+    // the slice are clamped to its last entry, so every load and store is
+    // unconditional (the clamped stores write that entry's own values).
+    constexpr uint32_t kStage = 8;
+    const uint32_t total = nz * nc;
+    for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kStage * blockDim.x) {
+      double2 gv[kStage];
+      int64_t qv[kStage];
+#pragma unroll
+      for (uint32_t u = 0; u < kStage; ++u) {
+        const uint32_t i = ::min(i0 + u * blockDim.x, total - 1);
+        const uint32_t src = (i / nc) * nr + set + (i % nc) * S;
+        const double* e = reinterpret_cast<const double*>(gsrc) + 2 * (size_t)src;
+        gv[u] = make_double2(gld(e, 0), gld(e, 1));
+        qv[u] = gld(f.bins, src);
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kStage; ++u) {
+        const uint32_t i = ::min(i0 + u * blockDim.x, total - 1);
+        lg[i] = gv[u];
+        lq[i] = (double)qv[u] * 0x1p-36;
+      }
     }
     __syncthreads();
     g = lg;
