@@ -89,7 +89,9 @@ struct BuildArgs {
   const hps_beam_stats* st;
   int64_t* bins;
   uint32_t greens_blocks;   // leading workgroups that fill the Green's table
-  uint32_t dep_blocks;      // deposit workgroups after them (0: empty deposit)
+  uint32_t dep_blocks;      // deposit workgroups: after them, or the first dep_blocks
+                            // when a lone beam has dep_blocks >= greens_blocks
+                            // (0: empty deposit)
 };
 
 struct FieldArgs {
