@@ -154,6 +154,24 @@ def test_longitudinal_loss_rescan_bit_exact(num):
     assert err is None, err
 
 
+@pytest.mark.parametrize("num", [400_000, 1_500_000, 2_500_000])
+def test_cross_chunk_tree_widths_bit_exact(num):
+    """Beams of 391, 1,465 and 2,442 chunks: the cross-chunk trees run with
+    2, 8 and 16 entries per thread (chunk partials padded to 512, 2,048 and
+    4,096), each a specialised load-then-fold path, with the 70-degree phase
+    spread so the chunk maxima of the longitudinal-loss test are live."""
+    import math
+    wl = H.workload(0, num=num, periods=2, sc=True)
+    wl.beam.sig_phi = math.radians(70.0)
+    ob, osc, _ = H.run_oracle(wl)
+    gb, gsc, _ = H.run_gpu(wl)
+    err = H.compare_arrays(gb.arrays(), ob.arrays())
+    assert err is None, err
+    gs, os_ = gsc.state(), osc.state()
+    for k in ("remesh_count", "kick_count", "num_on_mesh"):
+        assert gs[k] == os_[k], (k, gs[k], os_[k])
+
+
 def test_aperture_elements_and_rotation_steerer():
     from hpsim_amd import configs as cfg
     wl = H.workload(0, num=4096, periods=3, sc=True)
