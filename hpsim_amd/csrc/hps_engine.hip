@@ -1791,7 +1791,10 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
       a.sc = m->sc->dev; a.fld1 = m->sc->fld1; a.b = m->b->d; a.st = m->b->st; a.bins = m->sc->bins;
       a.greens_blocks = gb;
       a.dep_blocks = m->b->blck_size / 4 != 0 ? deposit_blocks(m->b->n, 4) : 0;
-      for (uint32_t c = 0; c < gb + a.dep_blocks; ++c)
+      // dep_blocks workgroups when the deposit covers the table (they fill it
+      // first), else the two roles side by side
+      const uint32_t nb = a.dep_blocks >= gb ? a.dep_blocks : gb + a.dep_blocks;
+      for (uint32_t c = 0; c < nb; ++c)
         build_map.push_back(((uint32_t)k << kBatchLocalBits) | c);
       FieldArgs& f = field[k];
       f.sc = m->sc->dev; f.bins = m->sc->bins; f.fld1 = m->sc->fld1; f.fld2 = m->sc->fld2;

@@ -1228,17 +1228,32 @@ __device__ __forceinline__ void deposit_block(const BeamArrays& b, const hps_bea
   if (t == 0 && ured[0]) atomicAdd(&sc->st.num_on_mesh, ured[0]);
 }
 
-// The SCHEFF rebuild after a remesh decision, one launch: workgroups
-// [0, greens_blocks) fill the Green's table (one entry per thread), the
-// next dep_blocks workgroups deposit the charge (the batched form, and a
-// lone beam whose deposit takes fewer workgroups than the table).  Both
-// read only the control step's results, so they run side by side; all exit
+// The SCHEFF rebuild after a remesh decision, one launch of
+// sc_build_blocks(a) workgroups per simulation.  When the deposit takes at
+// least as many workgroups as the Green's table, there are dep_blocks of
+// them: the first greens_blocks fill a Green's entry per thread and all
+// then deposit (fewer, fuller workgroups than side by side: the
+// 1,048,576-particle step measured 65.1 -> 63.7 ms).  Otherwise workgroups
+// [0, greens_blocks) fill the table and the next dep_blocks deposit
+// (65,536 particles: sc_build 1.55 ms per step side by side, 1.82
+// combined).  Both roles read only the control step's results; all exit
 // when there is no remesh.
+__host__ __device__ __forceinline__ uint32_t sc_build_blocks(const BuildArgs& a)
+{
+  return a.dep_blocks >= a.greens_blocks ? a.dep_blocks : a.greens_blocks + a.dep_blocks;
+}
+
 __device__ __forceinline__ void sc_build_body(const BuildArgs& a, uint32_t nr, uint32_t nz,
                                               uint32_t blk, unsigned long long* hist,
                                               uint32_t* ured)
 {
   if (!a.sc->remesh) return;
+  if (a.dep_blocks >= a.greens_blocks) {
+    if (blk < a.greens_blocks)
+      greens_entry(a.sc, a.fld1, nr, nz, a.b.n, a.b.mass, blk * kDepositBlock + threadIdx.x);
+    deposit_block(a.b, a.st, a.sc, a.bins, blk, a.dep_blocks, hist, ured);
+    return;
+  }
   if (blk < a.greens_blocks) {
     greens_entry(a.sc, a.fld1, nr, nz, a.b.n, a.b.mass, blk * kDepositBlock + threadIdx.x);
     return;
@@ -1246,25 +1261,11 @@ __device__ __forceinline__ void sc_build_body(const BuildArgs& a, uint32_t nr, u
   deposit_block(a.b, a.st, a.sc, a.bins, blk - a.greens_blocks, a.dep_blocks, hist, ured);
 }
 
-// A lone beam's rebuild.  When the deposit takes at least as many
-// workgroups as the Green's table, the grid is dep_blocks workgroups: the
-// first greens_blocks fill a Green's entry per thread and all then deposit
-// (fewer, fuller workgroups than side by side: the 1,048,576-particle step
-// measured 65.1 -> 63.7 ms).  Smaller beams keep the two side by side
-// (65,536 particles: sc_build 1.55 ms per step side by side, 1.82 combined).
 __global__ __launch_bounds__(kDepositBlock) void k_sc_build(BuildArgs a, uint32_t nr, uint32_t nz)
 {
   extern __shared__ unsigned long long hist[];
   __shared__ uint32_t ured[kDepositBlock];
-  if (a.dep_blocks < a.greens_blocks) {
-    sc_build_body(a, nr, nz, blockIdx.x, hist, ured);
-    return;
-  }
-  if (!a.sc->remesh) return;
-  if (blockIdx.x < a.greens_blocks)
-    greens_entry(a.sc, a.fld1, nr, nz, a.b.n, a.b.mass, blockIdx.x * kDepositBlock + threadIdx.x);
-  if (blockIdx.x < a.dep_blocks)
-    deposit_block(a.b, a.st, a.sc, a.bins, blockIdx.x, a.dep_blocks, hist, ured);
+  sc_build_body(a, nr, nz, blockIdx.x, hist, ured);
 }
 
 __global__ __launch_bounds__(kDepositBlock) void k_sc_build_batch(const BuildArgs* recs,
@@ -1714,9 +1715,8 @@ hipError_t launch_sc_build(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, Be
   // 262,144 particles on the 256-workgroup cap holds
   a.dep_blocks = deposit ? deposit_blocks(b.n, 1) : 0;
   const size_t lds = (size_t)nr * nz * sizeof(unsigned long long);
-  const uint32_t grid = a.dep_blocks < a.greens_blocks ? a.greens_blocks + a.dep_blocks
-                                                       : a.dep_blocks;
-  hipLaunchKernelGGL(k_sc_build, dim3(grid), dim3(kDepositBlock), lds, s, a, nr, nz);
+  hipLaunchKernelGGL(k_sc_build, dim3(sc_build_blocks(a)), dim3(kDepositBlock), lds, s, a, nr,
+                     nz);
   return hipGetLastError();
 }
 
