@@ -28,6 +28,56 @@ def _lib():
     return N.lib()
 
 
+_NUM_RE = re.compile(r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?")
+_ONLY_NUM_RE = re.compile(r"^[\s\d.eE+-]+$")
+
+
+def write_beam_file(path, msg, charge, mass, current, freq, arrays):
+    """The text form of Beam::PrintToFile (beam.cpp:670-707): an info line,
+    the charge, mass, current and frequency lines, then one row per particle
+    of x, xp, y, yp, phi, w, loss, lloss (15 significant digits)."""
+    with open(path, "w") as f:
+        f.write(f"Info {msg}\nCharge  {charge:g}\nMass {mass:g}\n"
+                f"Current {current:g}\nFrequency {freq:g}\n")
+        n = len(arrays["x"])
+        for i in range(n):
+            f.write("  ".join(f"{arrays[k][i]:.15g}" for k in _COORD)
+                    + f"  {arrays['loss'][i]}  {arrays['lloss'][i]}\n")
+
+
+def read_beam_file(path):
+    """Parse a beam file as Beam::InitBeamFromFile / UpdateBeamFromFile do
+    (beam.cpp:179-298): a line naming charge, mass, current or frequency
+    (case-insensitive) and holding a number sets that value (its first
+    number); every line of numbers only is a particle (x, xp, y, yp, phi, w
+    [, loss [, lloss]]).  Particle 0 is the reference particle and its
+    transverse coordinates are set to 0 (beam.cpp:289).  Returns (values,
+    six coordinate arrays, loss or None, lloss or None)."""
+    vals = {}
+    rows = []
+    with open(path) as f:
+        for line in f:
+            low = line.lower()
+            m = _NUM_RE.search(line)
+            for key in ("charge", "mass", "current", "frequency"):
+                if key in low and m:
+                    vals[key] = float(m.group(0))
+            if line.strip() and _ONLY_NUM_RE.match(line):
+                rows.append(line.split())
+    for key in ("charge", "mass", "current", "frequency"):
+        if key not in vals:
+            raise ValueError(f"Beam input file error: must define {key} value!")
+    if not rows:
+        raise ValueError("Beam input file error: no particle rows")
+    ncol = len(rows[-1])
+    cols = [np.array([float(r[k]) for r in rows]) for k in range(6)]
+    for k in range(4):
+        cols[k][0] = 0.0
+    loss = np.array([int(float(r[6])) for r in rows], np.uint32) if ncol > 6 else None
+    lloss = np.array([int(float(r[7])) for r in rows], np.uint32) if ncol > 7 else None
+    return vals, cols, loss, lloss
+
+
 class Beam:
     """Beam(mass=, charge=, current=, num=) or Beam(file=), wrap_beam.cpp:22-50."""
 
@@ -55,26 +105,8 @@ class Beam:
     # -- initialisation -------------------------------------------------
     def _init_from_file(self, path):
         """Beam::InitBeamFromFile, beam.cpp:179-298."""
-        num_re = re.compile(r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?")
-        only_num = re.compile(r"^[\s\d.eE+-]+$")
-        vals = {}
-        rows = []
-        with open(path) as f:
-            for line in f:
-                low = line.lower()
-                for key in ("charge", "mass", "current", "frequency"):
-                    if key in low and num_re.search(line):
-                        vals[key] = float(num_re.search(line).group(0))
-                if line.strip() and only_num.match(line):
-                    rows.append(line.split())
-        for key in ("charge", "mass", "current", "frequency"):
-            if key not in vals:
-                raise ValueError(f"Beam input file error: must define {key} value!")
-        self._create(len(rows), vals["mass"], vals["charge"], vals["current"])
-        ncol = len(rows[-1]) if rows else 6
-        cols = [np.array([float(r[k]) for r in rows]) for k in range(6)]
-        loss = np.array([int(float(r[6])) for r in rows], np.uint32) if ncol > 6 else None
-        lloss = np.array([int(float(r[7])) for r in rows], np.uint32) if ncol > 7 else None
+        vals, cols, loss, lloss = read_beam_file(path)
+        self._create(len(cols[0]), vals["mass"], vals["charge"], vals["current"])
         self.set_distribution(*cols, loss, lloss)
         check(_lib().hps_beam_set_ref_w(self._h, float(cols[5][0])))
         self.set_frequency(vals["frequency"])
@@ -113,13 +145,8 @@ class Beam:
     def print_to(self, output_file, msg=""):
         """Beam::PrintToFile, beam.cpp:670-707."""
         self.get_loss_num()
-        a = self._download()
-        with open(output_file, "w") as f:
-            f.write(f"Info {msg}\nCharge  {self.get_charge():g}\nMass {self.get_mass():g}\n"
-                    f"Current {self._scalar(5):g}\nFrequency {self.get_frequency():g}\n")
-            for i in range(self._n):
-                f.write("  ".join(f"{a[k][i]:.15g}" for k in _COORD)
-                        + f"  {a['loss'][i]}  {a['lloss'][i]}\n")
+        write_beam_file(output_file, msg, self.get_charge(), self.get_mass(), self._scalar(5),
+                        self.get_frequency(), self._download())
 
     def print_simple(self):
         a = self._download()

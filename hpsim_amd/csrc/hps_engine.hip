@@ -1438,8 +1438,9 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
             HPS_CHECK(hipStreamSynchronize(s));
             (void)hipFree(m->d_ops);
             m->d_ops = nullptr;
+            m->ops_cap = 0;
+            HPS_CHECK(hipMalloc(&m->d_ops, (2 * (size_t)st.op_end + 64) * sizeof(Op)));
             m->ops_cap = 2 * (size_t)st.op_end + 64;
-            HPS_CHECK(hipMalloc(&m->d_ops, m->ops_cap * sizeof(Op)));
           }
           HPS_CHECK(hipMemcpyAsync(m->d_ops + st.op_begin, c.ops.data() + st.op_begin,
                                    (size_t)(st.op_end - st.op_begin) * sizeof(Op),
@@ -1470,8 +1471,10 @@ extern "C" int hps_sim_simulate(hps_sim* m, int32_t start, int32_t end)
 
     if (c.ops.size() > m->ops_cap) {
       (void)hipFree(m->d_ops);
+      m->d_ops = nullptr;
+      m->ops_cap = 0;
+      HPS_CHECK(hipMalloc(&m->d_ops, (c.ops.size() + 64) * sizeof(Op)));
       m->ops_cap = c.ops.size() + 64;
-      HPS_CHECK(hipMalloc(&m->d_ops, m->ops_cap * sizeof(Op)));
     }
     if (!c.ops.empty())
       HPS_CHECK(hipMemcpyAsync(m->d_ops, c.ops.data(), c.ops.size() * sizeof(Op),
@@ -1763,8 +1766,9 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
       HPS_CHECK(hipStreamSynchronize(s));
       (void)hipFree(m->d_ops);
       m->d_ops = nullptr;
+      m->ops_cap = 0;
+      HPS_CHECK(hipMalloc(&m->d_ops, (c.ops.size() + 64) * sizeof(Op)));
       m->ops_cap = c.ops.size() + 64;
-      HPS_CHECK(hipMalloc(&m->d_ops, m->ops_cap * sizeof(Op)));
     }
     if (!c.ops.empty())
       HPS_CHECK(hipMemcpyAsync(m->d_ops, c.ops.data(), c.ops.size() * sizeof(Op),
@@ -1791,9 +1795,7 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
       a.sc = m->sc->dev; a.fld1 = m->sc->fld1; a.b = m->b->d; a.st = m->b->st; a.bins = m->sc->bins;
       a.greens_blocks = gb;
       a.dep_blocks = m->b->blck_size / 4 != 0 ? deposit_blocks(m->b->n, 4) : 0;
-      // dep_blocks workgroups when the deposit covers the table (they fill it
-      // first), else the two roles side by side
-      const uint32_t nb = a.dep_blocks >= gb ? a.dep_blocks : gb + a.dep_blocks;
+      const uint32_t nb = sc_build_blocks(a);
       for (uint32_t c = 0; c < nb; ++c)
         build_map.push_back(((uint32_t)k << kBatchLocalBits) | c);
       FieldArgs& f = field[k];
@@ -1819,8 +1821,9 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
     HPS_CHECK(hipStreamSynchronize(s));
     (void)hipFree(bt->d_rec);
     bt->d_rec = nullptr;
+    bt->rec_cap = 0;
+    HPS_CHECK(hipMalloc(&bt->d_rec, off + off / 4));
     bt->rec_cap = off + off / 4;
-    HPS_CHECK(hipMalloc(&bt->d_rec, bt->rec_cap));
   }
   std::vector<uint8_t>& H = bt->host_rec;
   H.assign(off, 0);

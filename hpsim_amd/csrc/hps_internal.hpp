@@ -94,6 +94,15 @@ struct BuildArgs {
                             // (0: empty deposit)
 };
 
+// Workgroups of one simulation's rebuild grid (k_sc_build): dep_blocks when
+// the deposit covers the Green's table (its first workgroups fill the table),
+// else the table workgroups followed by the deposit workgroups.  The host
+// builds the batched block map from it and the kernel splits the roles by it.
+__host__ __device__ __forceinline__ uint32_t sc_build_blocks(const BuildArgs& a)
+{
+  return a.dep_blocks >= a.greens_blocks ? a.dep_blocks : a.greens_blocks + a.dep_blocks;
+}
+
 struct FieldArgs {
   const ScDev* sc;
   const int64_t* bins;

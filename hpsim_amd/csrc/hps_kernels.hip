@@ -1238,10 +1238,6 @@ __device__ __forceinline__ void deposit_block(const BeamArrays& b, const hps_bea
 // (65,536 particles: sc_build 1.55 ms per step side by side, 1.82
 // combined).  Both roles read only the control step's results; all exit
 // when there is no remesh.
-__host__ __device__ __forceinline__ uint32_t sc_build_blocks(const BuildArgs& a)
-{
-  return a.dep_blocks >= a.greens_blocks ? a.dep_blocks : a.greens_blocks + a.dep_blocks;
-}
 
 __device__ __forceinline__ void sc_build_body(const BuildArgs& a, uint32_t nr, uint32_t nz,
                                               uint32_t blk, unsigned long long* hist,

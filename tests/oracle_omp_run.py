@@ -3,9 +3,16 @@ process of its own (the serial and OpenMP oracle libraries cannot share a
 process) and saves the final particle arrays and SCHEFF counters:
 
     python tests/oracle_omp_run.py IDX NUM PERIODS OUT.npz [SIG_PHI_DEG]
+    python tests/oracle_omp_run.py SPEC.json OUT.npz
+
+A SPEC.json names a member of the batched-scan configuration or a
+single-beam configuration with its own mesh:
+    {"config": IDX, "num": N, "periods": P, "nr": NR, "nz": NZ}
+    {"scan_member": K, "nbeams": B, "periods": P, "seed": S, "nr": NR, "nz": NZ}
 
 Used by the tests as the checker; it touches no GPU.
 """
+import json
 import math
 import os
 import sys
@@ -18,22 +25,41 @@ from hpsim_amd import configs as cfg  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 
+def _spec(path):
+    """(BeamSpec, element list, nr, nz, interval) of a JSON spec."""
+    with open(path) as f:
+        sp = json.load(f)
+    if "scan_member" in sp:
+        bw = cfg.batched_scan(nbeams=sp["nbeams"], periods=sp["periods"], seed=sp.get("seed", 7))
+        k = sp["scan_member"]
+        return bw.beams[k], bw.lattices[k], sp.get("nr", bw.nr), sp.get("nz", bw.nz), bw.interval
+    wl = cfg.config(sp["config"], num=sp.get("num"), periods=sp.get("periods"))
+    return (wl.beam, cfg.build_lattice(wl.lattice), sp.get("nr", wl.nr), sp.get("nz", wl.nz),
+            wl.interval)
+
+
 def main():
-    idx, num, periods, out = int(sys.argv[1]), int(sys.argv[2]), int(sys.argv[3]), sys.argv[4]
+    if sys.argv[1].endswith(".json"):
+        b, elems, nr, nz, interval = _spec(sys.argv[1])
+        out = sys.argv[2]
+    else:
+        idx, num, periods, out = int(sys.argv[1]), int(sys.argv[2]), int(sys.argv[3]), sys.argv[4]
+        wl = cfg.config(idx, num=num, periods=periods)
+        if len(sys.argv) > 5:
+            wl.beam.sig_phi = math.radians(float(sys.argv[5]))
+        b, elems, nr, nz, interval = (wl.beam, cfg.build_lattice(wl.lattice), wl.nr, wl.nz,
+                                      wl.interval)
     O.use_openmp()
-    wl = cfg.config(idx, num=num, periods=periods)
-    if len(sys.argv) > 5:
-        wl.beam.sig_phi = math.radians(float(sys.argv[5]))
-    b = wl.beam
     beam = O.Beam(b.num, b.mass, b.charge, b.current)
     beam.set_distribution(*cfg.make_distribution(b))
     beam.set_frequency(b.freq)
     beam.set_ref_w(b.w0)
-    sc = O.SpaceCharge(wl.nr, wl.nz, wl.interval)
-    O.Simulator(beam, cfg.build_lattice(wl.lattice), sc).simulate()
+    sc = O.SpaceCharge(nr, nz, interval)
+    O.Simulator(beam, elems, sc).simulate()
     st = sc.state()
     np.savez(out, threads=O.lib().orc_threads(), remesh_count=st["remesh_count"],
-             kick_count=st["kick_count"], **beam.arrays())
+             kick_count=st["kick_count"], num_on_mesh=st["num_on_mesh"], dr=st["dr"],
+             dz=st["dz"], **beam.arrays())
 
 
 if __name__ == "__main__":
