@@ -1523,6 +1523,27 @@ static void kick(orc_spch* s, orc_beam* b, double r_length, double r_ratio_r,
   }
 }
 
+/* Test hook: the SCHEFF kick alone (ApplySpchKickKernel,
+ * space_charge_kernel.cu:572-706) with a caller's field table fld2
+ * ((nz+1)*(nr+1) double2, [zp][rp]) on a mesh of spacing dr, dz, in the
+ * frame of velocity beta at frequency freq, all remesh ratios 1: the beam's
+ * SCHEFF statistics and relative phases are recomputed first
+ * (UpdateStatForSpaceCharge), then every particle is kicked. */
+void orc_t_kick(orc_spch* s, orc_beam* b, const double* fld2, double dr, double dz, double beta,
+                double freq, double length, uint32_t num_on_mesh)
+{
+  orc_beam_update_stat_for_space_charge(b);
+  s->s.dr = dr;
+  s->s.dz = dz;
+  s->beta = beta;
+  s->freq = freq;
+  s->frac = 1.0;
+  s->adj_eff = 0;
+  s->s.num_on_mesh = num_on_mesh;
+  memcpy(s->fld2, fld2, (size_t)(s->s.nz + 1) * (s->s.nx + 1) * 2 * sizeof(double));
+  kick(s, b, length, 1.0, 1.0, 1.0, 1.0);
+}
+
 /* Largest (x - x_avg)^2 + (y - y_avg)^2 over particles with loss == 0
  * (MaxR2PhiReduceKernel, beam_kernel.cu:793-930): the comparison a < v
  * keeps a when v is NaN, as fmax does; a maximum is order-free. */

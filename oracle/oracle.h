@@ -170,6 +170,8 @@ void orc_t_offmesh_field(double z, double r, double pl, uint32_t adj_bunch, doub
                          double* zod3);
 void orc_t_field(uint32_t nr, uint32_t nz, const int64_t* bins, const double* fld1, double* fld2);
 uint32_t orc_t_nextpow2(uint32_t v);
+void orc_t_kick(orc_spch* s, orc_beam* b, const double* fld2, double dr, double dz, double beta,
+                double freq, double length, uint32_t num_on_mesh);
 
 #ifdef __cplusplus
 }

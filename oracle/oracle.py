@@ -164,6 +164,8 @@ def lib():
                                         C.c_uint32, dp]
         L.orc_t_offmesh_field.argtypes = [C.c_double, C.c_double, C.c_double, C.c_uint32, dp, dp]
         L.orc_t_field.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_int64), dp, dp]
+        L.orc_t_kick.argtypes = [P, P, dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, C.c_uint32]
         L.orc_t_nextpow2.restype = C.c_uint32
         L.orc_t_nextpow2.argtypes = [C.c_uint32]
         _lib = L
