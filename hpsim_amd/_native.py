@@ -83,6 +83,7 @@ EXPORTS = (
     "hps_nn_create", "hps_nn_destroy", "hps_nn_num_params", "hps_nn_get_params",
     "hps_nn_set_params", "hps_nn_set_data", "hps_nn_train", "hps_nn_predict",
     "hps_nn_loss_grad", "hps_nn_layers", "hps_nn_set_output_scale", "hps_nn_get_output_scale",
+    "hps_emittance_noalloc",
 )
 
 _lib = None
@@ -166,6 +167,7 @@ def _declare(L):
         "hps_nn_set_output_scale": (C.c_int, [P, C.c_float]),
         "hps_nn_get_output_scale": (C.c_int, [P, fp]),
         "hps_batch_get_tuning": (C.c_int, [P, up, C.POINTER(C.c_float), u32]),
+        "hps_emittance_noalloc": (C.c_int, [dp] * 6 + [up, up, u32, d, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

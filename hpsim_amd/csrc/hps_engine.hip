@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1998,5 +1999,125 @@ extern "C" int hps_probe_math(int fn, const double* x, const double* y, double* 
   HPS_CHECK(hipStreamSynchronize(s));
   HPS_CHECK(hipMemcpy(out, dout, nb, hipMemcpyDeviceToHost));
   (void)hipFree(dx); (void)hipFree(dy); (void)hipFree(dout);
+  return 0;
+}
+
+// ---- Emittances over caller-owned device arrays, allocating nothing ----
+
+namespace {
+
+// The per-chunk partials of the canonical sums live in this array instead of
+// a Beam's allocation: 8 quantities and 3 counts for every chunk of the
+// largest beam, then the final sums.  Every device holds its own copy for the
+// life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (the array's name is a host
+// stand-in there).
+constexpr uint32_t kNoAllocMaxParticles = 4194304;
+constexpr uint32_t kNoAllocChunks = kNoAllocMaxParticles / kChunk;             // 4096
+constexpr size_t kNoAllocSumsOff = 0;
+constexpr size_t kNoAllocCntOff = kNoAllocSumsOff + (size_t)kMaxQ * kNoAllocChunks * 8;
+constexpr size_t kNoAllocOutOff = kNoAllocCntOff + 3 * (size_t)kNoAllocChunks * 4;
+constexpr size_t kNoAllocOutCntOff = kNoAllocOutOff + kMaxQ * 8;
+constexpr size_t kNoAllocBytes = kNoAllocOutCntOff + 256;                       // 311,616 B
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[16] = {};
+
+// The device address of `scratch` on device dev (looked up once per device,
+// with that device current).
+int scratch_for(int dev, unsigned char** out)
+{
+  if (dev < 0 || dev >= 16) return fail("device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HPS_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail("hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int hps_emittance_noalloc(const double* x, const double* xp, const double* y,
+                                     const double* yp, const double* phi, const double* w,
+                                     const uint32_t* loss, const uint32_t* lloss, uint32_t n,
+                                     double mass, double* out)
+{
+  if (!x || !xp || !y || !yp || !phi || !w || !loss || !lloss || !out)
+    return fail("null argument");
+  if (n > kNoAllocMaxParticles)
+    return fail("hps_emittance_noalloc: " + std::to_string(n) + " particles exceed its limit of " +
+                std::to_string(kNoAllocMaxParticles));
+  if (n <= 2) {   // Beam::UpdateEmittance computes nothing for such beams (beam.cpp:985-994)
+    out[0] = out[1] = out[2] = 0.0;
+    return 0;
+  }
+  int dev = 0;
+  HPS_CHECK(hipGetDevice(&dev));
+  unsigned char* base = nullptr;
+  if (scratch_for(dev, &base)) return -1;
+  hipStream_t s = nullptr;   // the device's null stream: nothing is created
+  Partials pr{};
+  pr.sums = (double*)(base + kNoAllocSumsOff);
+  pr.cnt = (uint32_t*)(base + kNoAllocCntOff);
+  pr.out = (double*)(base + kNoAllocOutOff);
+  pr.out_cnt = (uint32_t*)(base + kNoAllocOutCntOff);
+  pr.nch = (n + kChunk - 1) / kChunk;
+  pr.mpad = next_pow2(pr.nch);
+  // the trees read the zero padding past the last chunk (left over by an
+  // earlier, larger call otherwise)
+  if (pr.mpad > pr.nch) {
+    for (int q = 0; q < kMaxQ; ++q)
+      HPS_CHECK(hipMemsetAsync(pr.sums + (size_t)q * pr.mpad + pr.nch, 0,
+                               (size_t)(pr.mpad - pr.nch) * 8, s));
+    for (int q = 0; q < 3; ++q)
+      HPS_CHECK(hipMemsetAsync(pr.cnt + (size_t)q * pr.mpad + pr.nch, 0,
+                               (size_t)(pr.mpad - pr.nch) * 4, s));
+  }
+  BeamArrays a{};
+  a.x = const_cast<double*>(x); a.xp = const_cast<double*>(xp);
+  a.y = const_cast<double*>(y); a.yp = const_cast<double*>(yp);
+  a.phi = const_cast<double*>(phi); a.w = const_cast<double*>(w);
+  a.loss = const_cast<uint32_t*>(loss); a.lloss = const_cast<uint32_t*>(lloss);
+  a.n = n;
+  a.mass = mass;
+  double w0 = 0;
+  HPS_CHECK(hipMemcpyAsync(&w0, w, sizeof(double), hipMemcpyDeviceToHost, s));
+  // the same sums and expressions as hps_beam_get_emittance (UpdateLoss, then
+  // Beam::UpdateEmittance, beam.cpp:985-994); num_loss is the loss count of
+  // the same pass
+  auto sums = [&](const double* pa, const double* pb, const double* rel, double* r5,
+                  uint32_t* nloss) -> int {
+    HPS_CHECK(launch_sums(a, pa, pb, 0, pr, s, rel));
+    uint32_t c[3];
+    HPS_CHECK(hipMemcpyAsync(r5, pr.out, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HPS_CHECK(hipMemcpyAsync(c, pr.out_cnt, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HPS_CHECK(hipStreamSynchronize(s));
+    *nloss = c[0];
+    return 0;
+  };
+  for (int axis = 0; axis < 2; ++axis) {
+    double r[5];
+    uint32_t nloss = 0;
+    if (sums(axis == 0 ? x : y, axis == 0 ? xp : yp, nullptr, r, &nloss)) return -1;
+    const double sv = (double)(uint32_t)(n - nloss);
+    const double m1 = r[0] / sv, m2 = r[1] / sv, p1 = r[2] / sv, p2 = r[3] / sv, mp = r[4] / sv;
+    const double gamma1 = w0 / mass;
+    const double betagamma = std::sqrt(gamma1 * (gamma1 + 2.0));
+    out[axis] = std::sqrt((m2 - m1 * m1) * (p2 - p1 * p1) - (mp - m1 * p1) * (mp - m1 * p1)) *
+                betagamma;
+  }
+  double r[5];
+  uint32_t nloss = 0;
+  if (sums(phi, w, phi, r, &nloss)) return -1;
+  const double sv = (double)(uint32_t)(n - nloss);
+  const double p1 = r[0] / sv, p2 = r[1] / sv, w1 = r[2] / sv, w2 = r[3] / sv, pw = r[4] / sv;
+  const double cov = pw - p1 * w1;
+  out[2] = std::sqrt((p2 - p1 * p1) * (w2 - w1 * w1) - cov * cov);
   return 0;
 }

@@ -163,9 +163,10 @@ hipError_t launch_field_batch(const FieldArgs* recs, uint32_t nmem, uint32_t nr,
 // Generic canonical sums for the getters: over the filtered particles
 // (filter 0: loss == 0, 1: loss + lloss == 0, 2: all) of a, a*a, b, b*b, a*b,
 // and the count of loss != 0, of (loss+lloss) == 0 and of lloss != 0.
-// Results land in pr.out[0..4] and pr.out_cnt[0..2].
+// Results land in pr.out[0..4] and pr.out_cnt[0..2].  With rel_ref, a is
+// replaced by its relative phase against rel_ref[0] (k_relative_phi's value).
 hipError_t launch_sums(BeamArrays b, const double* a, const double* bb, int filter,
-                       Partials pr, hipStream_t s);
+                       Partials pr, hipStream_t s, const double* rel_ref = nullptr);
 hipError_t launch_relative_phi(BeamArrays b, const double* ref, hipStream_t s);
 hipError_t launch_lloss_flag(BeamArrays b, const double* avg, hipStream_t s);
 hipError_t launch_shift(double* a, uint32_t n, double v, hipStream_t s);

@@ -1485,8 +1485,10 @@ __global__ __launch_bounds__(kBlock) void k_field_combine(FieldArgs f, uint32_t 
 
 // --- getter reductions ---------------------------------------------------------
 
+// With rel_ref given, the first variable is the relative phase of pa
+// against rel_ref[0] (what k_relative_phi stores), computed in registers.
 __global__ __launch_bounds__(kBlock) void k_sums(BeamArrays b, const double* pa, const double* pb,
-                                                 int filter, Partials pr)
+                                                 int filter, Partials pr, const double* rel_ref)
 {
   __shared__ RedSmem sm;
   const int t = threadIdx.x;
@@ -1502,7 +1504,8 @@ __global__ __launch_bounds__(kBlock) void k_sums(BeamArrays b, const double* pa,
     nlloss += lloss != 0;
     ngood += is_good(loss, lloss);
     const bool inc = filter == 0 ? loss == 0 : (filter == 1 ? is_good(loss, lloss) : true);
-    const double va = (inc && pa) ? pa[i] : 0.0;
+    double va = (inc && pa) ? pa[i] : 0.0;
+    if (inc && pa && rel_ref) va = relative_phi(va, rel_ref[0]);
     const double vb = (inc && pb) ? pb[i] : 0.0;
     acc[0] = acc[0] + va;
     acc[1] = acc[1] + va * va;
@@ -1846,10 +1849,10 @@ hipError_t launch_field_batch(const FieldArgs* recs, uint32_t nmem, uint32_t nr,
 }
 
 hipError_t launch_sums(BeamArrays b, const double* a, const double* bb, int filter, Partials pr,
-                       hipStream_t s)
+                       hipStream_t s, const double* rel_ref)
 {
   const uint32_t nch = (b.n + kChunk - 1) / kChunk;
-  hipLaunchKernelGGL(k_sums, dim3(nch), dim3(kBlock), 0, s, b, a, bb, filter, pr);
+  hipLaunchKernelGGL(k_sums, dim3(nch), dim3(kBlock), 0, s, b, a, bb, filter, pr, rel_ref);
   hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(kBlock), 0, s, pr);
   return hipGetLastError();
 }

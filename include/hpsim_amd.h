@@ -281,6 +281,22 @@ int hps_nn_get_output_scale(const hps_nn* net, float* v);
 /* layer table rows {Cin, Cout, taps, dilation, relu, residual, weight offset, bias offset} */
 int hps_nn_layers(const hps_nn* net, int32_t* table, uint32_t max_layers, uint32_t* nlayers);
 
+/* ---- Emittances without allocation ---- */
+/* The three emittances of hps_beam_get_emittance (axis 0, 1, 2 -> out[0..2],
+ * the same sums, order and expressions, so the same bits) over particle
+ * arrays the caller holds in device memory on the current device: x, xp, y,
+ * yp, phi, w (n doubles each) and loss, lloss (n words each).  No
+ * counterpart in the reference: this entry point is for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc);
+ * it calls no allocator.  Its reduction partials live in a static device
+ * array of 311,616 bytes that the library holds on every device it runs on
+ * until the process ends, so n is at most 4,194,304 (larger n is an error).
+ * Work runs on the device's null stream and the call returns when out is
+ * written.  Calls that overlap on one device race on that array. */
+int hps_emittance_noalloc(const double* x, const double* xp, const double* y, const double* yp,
+                          const double* phi, const double* w, const uint32_t* loss,
+                          const uint32_t* lloss, uint32_t n, double mass, double* out);
+
 /* ---- elementary-function probes (host-callable, run on the device) ---- */
 /* Evaluates fn over n inputs on the GPU: 0 exp, 1 log, 2 sin, 3 cos, 4 sinh,
  * 5 cosh, 6 sqrt, 7 fmod(x, 2pi), 8 x/y, 9 Bessel I0, 10 Bessel I1,
