@@ -1,23 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark: particle-element pushes/s including SCHEFF space charge.
 
-One step = one simulate() of the whole lattice for the 1,048,576-particle
-DTL-like workload (200 Drift/Quad/Drift/DTL-gap periods, gap voltage ramped
-0.3 -> 0.6 MV, quad gradients scaled with beta*gamma, SCHEFF 32x128 after
-every element), started from the saved initial beam.  value = particles x
-elements x steps / wall time of the timed steps.
+Default workload: the combined stress configuration -- 64 beams of
+4,194,304 particles (drawn on the device), the 400-period Drift/Quad/Drift/
+DTL-gap lattice, SCHEFF on a 64x256 mesh after every element, all beams in
+one batch whose fields come from the learned surrogate (audited against the
+exact grouped SCHEFF product every 20 kick points, fallback above 2 %
+relative field error; grouped product on the FP32 matrix cores when it
+runs).  One step = one simulate() over the next 40 lattice periods (160
+elements) of every beam; the beams go on from where the previous step ended
+and start again at the lattice entrance after the last period.  value =
+particles x elements x steps / wall time of the timed steps.
 
 The JSON line also carries
-  roofline:      HBM roofline of the fused element-map kernel (k_segment):
-                 algorithmic bytes per launch / average launch time, the
-                 average measured with HIP events (on the engine's stream)
-                 around every launch of one extra eager step run after the
-                 timed steps, which replay a captured HIP graph;
-  cpu_baseline:  the FP64 oracle built with OpenMP (oracle/liboracle_omp.so,
-                 bit-identical to the serial checker) on all host cores
-                 (OMP_NUM_THREADS) over the whole workload, timed on this
-                 host; "serial" inside it is the one-core oracle over the
-                 first 10 periods.
+  roofline:      HBM roofline of the batched element-map kernel
+                 (k_segment_batch), algorithmic bytes per launch / average
+                 launch time from HIP events around every launch of one extra
+                 step; roofline_gemm: the grouped field product's FP32 matrix-
+                 core rate (2 M K flops per simulation in a product);
+  accuracy:      rms emittance, transmission and mean output energy of a few
+                 beams of the workload tracked through the whole lattice with
+                 the benchmarked field mode, against the same beams with the
+                 FP64 per-beam SCHEFF solve (bit-identical to the FP64 oracle);
+  plain_fp64:    the plain FP64 single-beam paths of the same box (1,048,576
+                 particles x 800 elements and 65,536 x 160) and the ratio;
+  cpu_baseline:  the FP64 oracle built with OpenMP (bit-identical to the
+                 serial checker) on whole beams of the workload over their
+                 first elements, timed on this host's cores.
 """
 from __future__ import annotations
 
@@ -31,6 +40,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
+FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
+GEMM_ASPECT_STEP = 0.05        # aspect grid of the grouped field solve
+SURROGATE_AUDIT_EVERY = 20     # kick points between audits of the surrogate
+SURROGATE_MAX_ERR = 0.02       # relative L2 field error that triggers the fallback
 SEG_BYTES_PER_PARTICLE = 104   # k_segment: reads 6 FP64 + 2 u32, writes 6 FP64
 # HBM bytes per k_segment launch at 1,048,576 particles, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE counter runs (calibrated on a plain device copy)
@@ -57,20 +70,29 @@ def parse():
     p.add_argument("--periods", type=int, default=None, help="override lattice periods")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--eager", action="store_true",
-                   help="launch every kernel from the host instead of replaying a HIP graph "
+                   help="launch every kernel from the host instead of replaying HIP graphs "
                         "(for profilers that do not follow graph launches)")
     p.add_argument("--cpu-periods", type=int, default=200,
-                   help="lattice periods of the all-core CPU-baseline sample")
+                   help="lattice periods of the all-core CPU-baseline sample (single beam)")
     p.add_argument("--cpu-serial-periods", type=int, default=10,
-                   help="lattice periods of the one-core CPU-baseline sample")
-    p.add_argument("--workload", default="plain",
-                   choices=("plain", "small", "batched-scan", "stress"),
-                   help="plain: the 1,048,576-particle single beam (default); small: the "
-                        "65,536-particle single beam on 40 periods; batched-scan: 256 beams "
-                        "of 16K-1M particles with per-beam lattice settings in one batch; "
-                        "stress: 64 beams of 4,194,304 particles, 400 periods, 64x256 mesh, "
-                        "in one batch")
-    p.add_argument("--beams", type=int, default=256, help="beams of the batched scan")
+                   help="lattice periods of the one-core CPU-baseline sample (single beam)")
+    p.add_argument("--workload", default="stress",
+                   choices=("stress", "batched-scan", "plain", "small"),
+                   help="stress: 64 beams of 4,194,304 particles, 400 periods, 64x256 mesh, in "
+                        "one batch (default); batched-scan: 256 beams of 16K-1M particles with "
+                        "per-beam lattice settings in one batch; plain: the 1,048,576-particle "
+                        "single beam; small: the 65,536-particle single beam on 40 periods")
+    p.add_argument("--field", default=None, choices=("fp64", "gemm", "surrogate"),
+                   help="batched field solve: fp64 (per-beam FP64 tables, bit-exact), gemm "
+                        "(grouped FP32 matrix-core product on aspect-grid meshes) or surrogate "
+                        "(the trained network, audited by gemm); default surrogate")
+    p.add_argument("--beams", type=int, default=None, help="beams of a batched workload")
+    p.add_argument("--slice-periods", type=int, default=None,
+                   help="lattice periods per step of a batched workload (stress: 40, scan: all)")
+    p.add_argument("--validate-beams", type=int, default=4,
+                   help="beams tracked through the whole lattice for the accuracy figures")
+    p.add_argument("--no-plain", action="store_true",
+                   help="skip the plain FP64 single-beam reference runs")
     return p.parse_args()
 
 
@@ -163,39 +185,123 @@ def batched_cpu_baseline(w, budget_s=20.0, max_elements=None):
                       f"OpenMP oracle, one beam after the other, {busy:.1f} s"}
 
 
-def main_batched(args):
-    import multiprocessing as mp
-
-    from hpsim_amd import HPSim
+def _batch_workload(args):
     from hpsim_amd import configs as cfg
-
     if args.workload == "stress":
-        w = cfg.combined_stress(nbeams=args.beams if args.beams != 256 else 64,
-                                num=args.num or 4194304, periods=args.periods or 400)
-    else:
-        w = cfg.batched_scan(nbeams=args.beams, periods=args.periods or 40)
-    total = sum(b.num for b in w.beams)
-    nel = len(w.lattices[0])
-    _log(f"{len(w.beams)} beams, {total} particles, {nel} elements: sampling distributions")
-    with mp.get_context("spawn").Pool(min(16, os.cpu_count() or 1)) as pool:
-        dists = pool.map(_member_dist, w.beams, chunksize=1)
-    _log("uploading beams")
+        return cfg.combined_stress(nbeams=args.beams or 64, num=args.num or 4194304,
+                                   periods=args.periods or 400)
+    return cfg.batched_scan(nbeams=args.beams or 256, periods=args.periods or 40)
+
+
+def _make_batch(w, members, mode, net=None):
+    """Batch of the workload's members with beams drawn on the device; the
+    field mode and, for 'surrogate', the network attached."""
+    from hpsim_amd import HPSim
+    from hpsim_amd import surrogate as S
     sims = []
-    for spec, els, d in zip(w.beams, w.lattices, dists):
+    for k in members:
+        spec = w.beams[k]
         beam = HPSim.Beam(mass=spec.mass, charge=spec.charge, current=spec.current, num=spec.num)
-        beam.set_distribution(*d)
+        beam.sample_gaussian(spec)
         beam.set_frequency(spec.freq)
         beam.set_ref_w(spec.w0)
         beam.save_initial_beam()
         sc = HPSim.SpaceCharge(nr=w.nr, nz=w.nz, interval=w.interval)
-        sims.append(HPSim.Simulator(beam, HPSim.BeamLine(elements=els), sc))
-    del dists
+        sims.append(HPSim.Simulator(beam, HPSim.BeamLine(elements=w.lattices[k]), sc))
     batch = HPSim.Batch(sims)
+    if mode in ("gemm", "surrogate"):
+        batch.set_field_mode("gemm", GEMM_ASPECT_STEP)
+    if mode == "surrogate":
+        S.attach(batch, net, audit_every=SURROGATE_AUDIT_EVERY, max_err=SURROGATE_MAX_ERR)
+    return batch, sims
+
+
+def _accuracy(w, mode, net, nval):
+    """Beams 0 .. nval-1 of the workload through the whole lattice with the
+    field mode and with the FP64 per-beam solve; relative errors of the rms
+    emittances, transmission and mean energy (max over the beams)."""
+    from hpsim_amd.metrics import beam_metrics, metric_errors
+    members = list(range(min(nval, len(w.beams))))
+    out = {}
+    runs = {}
+    for m in (mode, "fp64"):
+        batch, sims = _make_batch(w, members, m, net)
+        batch.simulate()
+        runs[m] = [beam_metrics(s.beam.arrays(), s.beam.get_mass()) for s in sims]
+        if m == "surrogate":
+            from hpsim_amd import surrogate as S
+            out["surrogate_counters"] = S.info(batch)
+        del batch, sims
+    errs = [metric_errors(g, r) for g, r in zip(runs[mode], runs["fp64"])]
+    out.update({k: max(e[k] for e in errs) for k in errs[0]})
+    out["beams"] = len(members)
+    out["reference"] = ("the same beams with the FP64 per-beam SCHEFF solve, whole lattice "
+                        "(bit-identical to the FP64 oracle)")
+    return out
+
+
+def _plain_rates():
+    """pushes/s of the plain FP64 single-beam path on this box: the
+    1,048,576-particle 800-element and the 65,536-particle 160-element
+    workloads, graph-launched, from the saved initial beam."""
+    from hpsim_amd import HPSim
+    from hpsim_amd import configs as cfg
+    out = {}
+    for idx, steps in ((1, 4), (0, 20)):
+        wl = cfg.config(idx)
+        b = wl.beam
+        elems = cfg.build_lattice(wl.lattice)
+        beam = HPSim.Beam(mass=b.mass, charge=b.charge, current=b.current, num=b.num)
+        beam.sample_gaussian(b)
+        beam.set_frequency(b.freq)
+        beam.set_ref_w(b.w0)
+        beam.save_initial_beam()
+        sim = HPSim.Simulator(beam, HPSim.BeamLine(elements=elems),
+                              HPSim.SpaceCharge(nr=wl.nr, nz=wl.nz, interval=wl.interval))
+        for k in range(steps + 1):
+            if k == 1:
+                t0 = time.perf_counter()
+            beam.restore_initial_beam()
+            sim.simulate()
+        dt = time.perf_counter() - t0
+        out[wl.name] = b.num * len(elems) * steps / dt
+    return out
+
+
+def main_batched(args):
+    from hpsim_amd import surrogate as S
+
+    w = _batch_workload(args)
+    mode = args.field or "surrogate"
+    net = None
+    if mode == "surrogate":
+        path = S.weights_path(w.nr, w.nz)
+        if not os.path.exists(path):
+            sys.exit(f"bench.py: no trained surrogate for the {w.nr}x{w.nz} mesh ({path}); "
+                     f"train it with python -m hpsim_amd.surrogate --nr {w.nr} --nz {w.nz}, or "
+                     "pass --field gemm")
+        net = S.load(path, max_batch=64)
+    total = sum(b.num for b in w.beams)
+    nel = len(w.lattices[0])
+    slice_periods = args.slice_periods or (40 if args.workload == "stress" else nel // 4)
+    sel = 4 * slice_periods
+    if nel % sel:
+        sys.exit(f"bench.py: {nel} elements are not a whole number of {slice_periods}-period "
+                 "slices")
+    _log(f"{len(w.beams)} beams, {total} particles, {nel} elements, field {mode}: "
+         "drawing beams on the device")
+    batch, sims = _make_batch(w, range(len(w.beams)), mode, net)
+    batch.set_graph(not args.eager)
+    nslice = nel // sel
+    pos = [0]
 
     def step():
-        for m in sims:
-            m.beam.restore_initial_beam()
-        batch.simulate()
+        # the next slice of the lattice; back to the entrance after the last
+        if pos[0] == 0:
+            for m in sims:
+                m.beam.restore_initial_beam()
+        batch.simulate(pos[0] * sel, pos[0] * sel + sel - 1)
+        pos[0] = (pos[0] + 1) % nslice
 
     for k in range(args.warmup):
         step()
@@ -205,17 +311,20 @@ def main_batched(args):
         step()
         _log(f"timed step {k + 1} done")
     dt = time.perf_counter() - t0
+    replays, per_period = batch.graph_info()
+    gi0 = batch.gemm_info() if mode != "fp64" else None
     batch.set_profiling(True)
     step()
     prof = batch.profile()
     batch.set_profiling(False)
+    gi1 = batch.gemm_info() if mode != "fp64" else None
     seg_ms, seg_n = prof["segment"]
     avg_s = seg_ms / 1e3 / max(seg_n, 1)
     achieved = SEG_BYTES_PER_PARTICLE * total / avg_s / 1e9
     transmitted = sum(m.beam.get_size() - m.beam.get_loss_num("t") for m in sims)
     out = {
         "metric": "particle-element pushes/s incl. space charge",
-        "value": total * nel * args.steps / dt,
+        "value": total * sel * args.steps / dt,
         "unit": "particle-element pushes/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -225,25 +334,53 @@ def main_batched(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": ("synthetic (Philox-seeded truncated Gaussian beams, generated lattice"
-                 + (")" if args.workload == "stress" else " with seeded per-beam perturbations)")),
+        "data": ("synthetic (Philox-seeded truncated Gaussian beams drawn on the device, "
+                 "generated lattice" +
+                 (")" if args.workload == "stress" else " with seeded per-beam perturbations)")),
         "config": {"workload": w.name, "beams": len(w.beams), "particles": total,
                    "particles_min": min(b.num for b in w.beams),
                    "particles_max": max(b.num for b in w.beams), "elements": nel,
-                   "mesh": f"{w.nr}x{w.nz}", "transmitted": transmitted},
+                   "elements_per_step": sel, "mesh": f"{w.nr}x{w.nz}", "field": mode,
+                   "transmitted_after_profiled_step": transmitted},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_segment_batch", "avg_launch_us": avg_s * 1e6,
                      "launches": seg_n, "bytes_per_launch": SEG_BYTES_PER_PARTICLE * total,
-                     "timing": "HIP events around every launch of one extra step"},
+                     "timing": "HIP events around every launch of one extra (eager) step"},
         "kernel_ms_per_step": {k: v[0] for k, v in prof.items()},
-        "launch": "eager batched",
-        "field_shape_autotuned": dict(zip(("shape", "candidate_ms"), batch.tuning())),
+        "launch": "eager" if args.eager else "lattice-period HIP graphs",
+        "period_graph_replays": replays,
+        "launches_per_replayed_period": per_period,
     }
+    if mode != "fp64":
+        prod_ms, prod_n = prof["field_product"]
+        cols = gi1["product_columns"] - gi0["product_columns"]
+        flops = 2.0 * (2 * (w.nr + 1) * (w.nz + 1)) * (w.nr * w.nz) * cols
+        out["roofline_gemm"] = {
+            "bound": "mfma", "kernel": "k_gemm_field (FP32 MFMA 16x16x4)",
+            "achieved": flops / (prod_ms / 1e3) / 1e12 if prod_ms > 0 else None,
+            "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": (flops / (prod_ms / 1e3) / 1e12 / FP32_MFMA_PEAK_TFLOPS
+                     if prod_ms > 0 else None),
+            "launches": prod_n, "product_columns": cols,
+            "shape": gi1["shape"], "shape_candidate_ms": gi1["times_ms"],
+            "class_tables_built": gi1["tables_built"]}
+    if args.validate_beams > 0:
+        _log("accuracy run")
+        out["accuracy"] = _accuracy(w, mode, net, args.validate_beams)
+    if not args.no_plain:
+        _log("plain FP64 single-beam runs")
+        plain = _plain_rates()
+        out["plain_fp64"] = {k: v for k, v in plain.items()}
+        out["plain_fp64"]["ratio_to_" + "plain-fp64-1048576x200"] = (
+            out["value"] / plain["plain-fp64-1048576x200"])
+        out["plain_fp64"]["ratio_to_" + "plain-fp64-65536x40"] = (
+            out["value"] / plain["plain-fp64-65536x40"])
     if not args.no_cpu_baseline:
         _log("cpu baseline")
         out["cpu_baseline"] = batched_cpu_baseline(
             w, max_elements=40 if args.workload == "stress" else None)
+        out["cpu_baseline"]["build"] = "gcc -O2 -march=x86-64-v3 -ffp-contract=off -fopenmp"
     print(json.dumps(out), flush=True)
 
 
@@ -255,6 +392,8 @@ def main():
         sys.exit(2)
     if args.workload in ("batched-scan", "stress"):
         return main_batched(args)
+    if args.field:
+        sys.exit("bench.py: --field applies to the batched workloads")
     from hpsim_amd import HPSim
     from hpsim_amd import configs as cfg
 

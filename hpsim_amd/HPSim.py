@@ -137,6 +137,17 @@ class Beam:
             ax, bx, ex, ay, by, ey, dphi, sync_phi, sync_w)], int(seed)))
 
     # -- save / restore / output ---------------------------------------------
+    def sample_gaussian(self, spec):
+        """Draw the truncated-Gaussian beam of a configs.BeamSpec on the device
+        (the distribution configs.make_distribution gives, without a host
+        upload)."""
+        import numpy as np
+        center = np.array([0.0, 0.0, 0.0, 0.0, spec.phi0, spec.w0])
+        sigma = np.array([spec.sig_x, spec.sig_xp, spec.sig_y, spec.sig_yp, spec.sig_phi,
+                          spec.sig_w])
+        check(_lib().hps_beam_sample_gaussian(self._h, spec.seed & 0xFFFFFFFF, dptr(center),
+                                              dptr(sigma)))
+
     def save_initial_beam(self): check(_lib().hps_beam_save(self._h, 0))
     def save_intermediate_beam(self): check(_lib().hps_beam_save(self._h, 1))
     def restore_initial_beam(self): check(_lib().hps_beam_restore(self._h, 0))
@@ -491,7 +502,8 @@ class Simulator:
 
     def profile(self):
         """{family: (total ms, launches)} since profiling was enabled."""
-        names = ("segment", "phase", "sc_build", "field", "segment_final", "phase_final")
+        names = ("segment", "phase", "sc_build", "field", "segment_final", "phase_final",
+                 "field_product")
         ms = (C.c_double * len(names))()
         n = (C.c_uint32 * len(names))()
         check(_lib().hps_sim_get_profile(self._h, ms, n))
@@ -511,7 +523,8 @@ class Batch:
     must share one layout (element values may differ) and one SCHEFF mesh
     size."""
 
-    _FAMILIES = ("segment", "phase", "sc_build", "field", "segment_final", "phase_final")
+    _FAMILIES = ("segment", "phase", "sc_build", "field", "segment_final", "phase_final",
+                 "field_product")
 
     def __init__(self, simulators=()):
         h = C.c_void_p()
@@ -549,6 +562,42 @@ class Batch:
 
     def set_profiling(self, on=True):
         check(_lib().hps_batch_set_profiling(self._h, 1 if on else 0))
+
+    def set_graph(self, on=True):
+        """Replay repeated lattice periods as one HIP graph (default on)."""
+        check(_lib().hps_batch_set_graph(self._h, 1 if on else 0))
+
+    def graph_info(self):
+        """(period graph replays so far, launches per replayed period)."""
+        r = C.c_uint64()
+        p = C.c_uint32()
+        check(_lib().hps_batch_graph_info(self._h, C.byref(r), C.byref(p)))
+        return r.value, p.value
+
+    def set_field_mode(self, mode, aspect_step=0.05):
+        """'fp64' (default: each beam's own Green's table and FP64 field sum,
+        results identical to separate simulate calls) or 'gemm' (the grouped
+        FP32 matrix-core field solve: shared Green's tables on the aspect grid
+        (1 + aspect_step)^c, each beam's field interpolated quadratically
+        between its three nearest classes; approximate)."""
+        modes = {"fp64": 0, "gemm": 1}
+        if mode not in modes:
+            raise ValueError("field mode is 'fp64' or 'gemm'")
+        check(_lib().hps_batch_set_field_mode(self._h, modes[mode], float(aspect_step)))
+
+    def set_gemm_shape(self, shape=None):
+        """Fix the grouped product's launch shape (None: autotune)."""
+        check(_lib().hps_batch_set_gemm_shape(self._h, 0xFFFFFFFF if shape is None else shape))
+
+    def gemm_info(self):
+        """{shape, tables_built, conflicts, columns_last, product_columns, times_ms}:
+        product columns are three per remeshing simulation."""
+        info = (C.c_uint64 * 5)()
+        ms = (C.c_float * 16)()
+        check(_lib().hps_batch_get_gemm_info(self._h, info, ms, 16))
+        return {"shape": None if info[0] == 0xFFFFFFFF else info[0], "tables_built": info[1],
+                "conflicts": info[2], "columns_last": info[3], "product_columns": info[4],
+                "times_ms": [t for t in ms if t > 0]}
 
     def set_autotune(self, on=True):
         """Time the field-sum launch shapes at the first rebuild (default on)."""

@@ -65,7 +65,7 @@ class SpchState(C.Structure):
 EXPORTS = (
     "hps_last_error", "hps_set_device",
     "hps_beam_create", "hps_beam_destroy", "hps_beam_set_distribution", "hps_beam_download",
-    "hps_beam_set_waterbag", "hps_beam_set_dc",
+    "hps_beam_set_waterbag", "hps_beam_set_dc", "hps_beam_sample_gaussian",
     "hps_beam_relative_phi", "hps_beam_size", "hps_beam_set_ref_w", "hps_beam_set_ref_phi",
     "hps_beam_set_frequency", "hps_beam_get_scalar", "hps_beam_get_loss_num",
     "hps_beam_get_avg", "hps_beam_get_sig", "hps_beam_get_emittance", "hps_beam_get_stats",
@@ -81,8 +81,13 @@ EXPORTS = (
     "hps_batch_simulate", "hps_batch_last_time_ms", "hps_batch_set_profiling",
     "hps_batch_get_profile", "hps_batch_set_autotune", "hps_batch_get_tuning",
     "hps_nn_create", "hps_nn_destroy", "hps_nn_num_params", "hps_nn_get_params",
-    "hps_nn_set_params", "hps_nn_set_data", "hps_nn_train", "hps_nn_predict",
-    "hps_nn_loss_grad", "hps_nn_layers", "hps_nn_set_output_scale", "hps_nn_get_output_scale",
+    "hps_nn_set_params", "hps_nn_layers", "hps_nn_dims", "hps_nn_set_norm", "hps_nn_get_norm",
+    "hps_nn_data_reserve", "hps_nn_data_append", "hps_nn_data_size", "hps_nn_data_get",
+    "hps_nn_data_shuffle",
+    "hps_nn_train", "hps_nn_eval", "hps_nn_predict", "hps_nn_loss_grad",
+    "hps_batch_set_capture", "hps_batch_set_surrogate", "hps_batch_get_surrogate_info",
+    "hps_batch_set_field_mode", "hps_batch_set_gemm_shape", "hps_batch_get_gemm_info",
+    "hps_batch_set_graph", "hps_batch_graph_info",
     "hps_emittance_noalloc",
 )
 
@@ -107,6 +112,7 @@ def _declare(L):
         "hps_beam_download": (C.c_int, [P, dp, dp, dp, dp, dp, dp, up, up]),
         "hps_beam_set_waterbag": (C.c_int, [P] + [d] * 12 + [u32]),
         "hps_beam_set_dc": (C.c_int, [P] + [d] * 9 + [u32]),
+        "hps_beam_sample_gaussian": (C.c_int, [P, u32, dp, dp]),
         "hps_beam_relative_phi": (C.c_int, [P, C.c_int, dp]),
         "hps_beam_size": (u32, [P]),
         "hps_beam_set_ref_w": (C.c_int, [P, d]),
@@ -154,19 +160,34 @@ def _declare(L):
         "hps_batch_set_profiling": (C.c_int, [P, C.c_int]),
         "hps_batch_get_profile": (C.c_int, [P, dp, up]),
         "hps_batch_set_autotune": (C.c_int, [P, C.c_int]),
-        "hps_nn_create": (C.c_int, [PP, u32, u32, u32, u32, u32]),
+        "hps_nn_create": (C.c_int, [PP, u32, u32, up, u32, u32, u32]),
         "hps_nn_destroy": (C.c_int, [P]),
         "hps_nn_num_params": (u32, [P]),
         "hps_nn_get_params": (C.c_int, [P, fp]),
         "hps_nn_set_params": (C.c_int, [P, fp]),
-        "hps_nn_set_data": (C.c_int, [P, fp, fp, u32]),
-        "hps_nn_train": (C.c_int, [P, u32, u32, C.c_float, u32, fp]),
-        "hps_nn_predict": (C.c_int, [P, fp, u32, fp]),
-        "hps_nn_loss_grad": (C.c_int, [P, fp, fp, u32, fp, fp]),
         "hps_nn_layers": (C.c_int, [P, C.POINTER(C.c_int32), u32, up]),
-        "hps_nn_set_output_scale": (C.c_int, [P, C.c_float]),
-        "hps_nn_get_output_scale": (C.c_int, [P, fp]),
+        "hps_nn_dims": (C.c_int, [P, up]),
+        "hps_nn_set_norm": (C.c_int, [P, C.c_float, C.c_float, C.c_float]),
+        "hps_nn_get_norm": (C.c_int, [P, fp]),
+        "hps_nn_data_reserve": (C.c_int, [P, u32]),
+        "hps_nn_data_append": (C.c_int, [P, fp, fp, fp, u32]),
+        "hps_nn_data_size": (C.c_int, [P, up]),
+        "hps_nn_data_get": (C.c_int, [P, u32, u32, fp, fp, fp]),
+        "hps_nn_data_shuffle": (C.c_int, [P, u32]),
+        "hps_nn_train": (C.c_int, [P, u32, u32, u32, u32, C.c_float, C.c_float, u32, fp]),
+        "hps_nn_eval": (C.c_int, [P, u32, u32, fp]),
+        "hps_nn_predict": (C.c_int, [P, fp, fp, u32, fp]),
+        "hps_nn_loss_grad": (C.c_int, [P, fp, fp, fp, u32, fp, fp]),
+        "hps_batch_set_capture": (C.c_int, [P, P]),
+        "hps_batch_set_surrogate": (C.c_int, [P, P, u32, d]),
+        "hps_batch_get_surrogate_info": (C.c_int, [P, dp]),
         "hps_batch_get_tuning": (C.c_int, [P, up, C.POINTER(C.c_float), u32]),
+        "hps_batch_set_field_mode": (C.c_int, [P, C.c_int, d]),
+        "hps_batch_set_graph": (C.c_int, [P, C.c_int]),
+        "hps_batch_graph_info": (C.c_int, [P, C.POINTER(C.c_uint64), up]),
+        "hps_batch_set_gemm_shape": (C.c_int, [P, u32]),
+        "hps_batch_get_gemm_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_float),
+                                             u32]),
         "hps_emittance_noalloc": (C.c_int, [dp] * 6 + [up, up, u32, d, dp]),
     }
     for name, (res, args) in sig.items():

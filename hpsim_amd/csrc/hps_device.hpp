@@ -674,6 +674,54 @@ __device__ __forceinline__ void apply_op(const OpT& op, Particle& p, uint32_t in
   }
 }
 
+// Radial and longitudinal field at radius rp and axial distance d of a ring
+// of radius rs, before the c1 charge factor: the complete-elliptic-integral
+// fit of UpdateFldTbl1Kernel (space_charge_kernel.cu:52-71, repeated for the
+// image rings at :74-94); er is 0 on the axis.
+HPS_HD void ring_green(double rs, double rp, double d, double& er, double& ez)
+{
+  const double d2 = d * d;
+  const double c = (rp - rs) * (rp - rs);
+  const double b = (rp + rs) * (rp + rs);
+  const double rsrp = rs * rp;
+  double a = 4.0 * rsrp / (b + d2);
+  const double g = 1.0 - a;
+  const double h = hps::log(g);
+  const double ee = 1.0 + g * (0.4630106 - 0.2452740 * h + g * (0.1077857 - 0.04125321 * h));
+  const double ek = 1.38629436 - 0.5 * h + g * (0.1119697 - 0.1213486 * h + g *
+                    (0.07253230 - 0.02887472 * h));
+  a = ::sqrt(b + d2);
+  const double rs2mrp2 = (rs * rs - rp * rp);
+  er = 0.0;
+  if (rp != 0.0) er = (ek - (rs2mrp2 + d2) * ee / (c + d2)) / (2.0 * rp * a);
+  ez = d * ee / (a * (c + d2));
+}
+
+// log of the bunch-frame cell aspect dz*gm/dr of a SCHEFF mesh (0 for
+// spacings that are not positive and finite): the one parameter of the
+// Green's table normalised to dr = 1, which the grouped field solve of a
+// batch interpolates in.
+HPS_HD double log_aspect(double dr, double dz, double gm)
+{
+  const double v = dz * gm / dr;
+  if (!(v > 0.0) || !(v < 1.0e300)) return 0.0;
+  return hps::log(v);
+}
+
+// Nearest class c = round(lk / lnr) of a log aspect lk on the grid of step
+// lnr and the Lagrange weights of classes c - 1, c, c + 1 for quadratic
+// interpolation in lk (t = lk / lnr - c in [-1/2, 1/2)).
+HPS_HD int32_t aspect_lagrange(double lk, double lnr, double (&w)[3])
+{
+  const double x = lk / lnr;
+  const double c = ::floor(x + 0.5);
+  const double t = x - c;
+  w[0] = 0.5 * t * (t - 1.0);
+  w[1] = (1.0 - t) * (1.0 + t);
+  w[2] = 0.5 * t * (t + 1.0);
+  return (int32_t)c;
+}
+
 // UpdateRelativePhiKernel body, beam_kernel.cu:1514-1520
 __device__ __forceinline__ double relative_phi(double phi, double ref)
 {

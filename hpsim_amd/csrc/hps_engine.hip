@@ -21,6 +21,7 @@
 
 #include "hpsim_amd.h"
 #include "hps_internal.hpp"
+#include "hps_unet.hpp"
 
 using namespace hps;
 
@@ -503,6 +504,16 @@ int hps_beam_set_dc(hps_beam* b, double ax, double bx, double ex, double ay, dou
     v[5 * n + i] = sync_w;
   }
   return upload_sampled(b, v);
+}
+
+int hps_beam_sample_gaussian(hps_beam* b, uint32_t seed, const double* center, const double* sigma)
+{
+  if (!b || !center || !sigma) return fail("null argument");
+  GaussSpec g{};
+  for (int c = 0; c < 6; ++c) { g.center[c] = center[c]; g.sigma[c] = sigma[c]; }
+  HPS_CHECK(launch_sample_gaussian(b->d, seed, g, b->stream));
+  HPS_CHECK(hipStreamSynchronize(b->stream));
+  return init_phi_avg_good(b);
 }
 
 int hps_beam_download(hps_beam* b, double* x, double* xp, double* y, double* yp, double* phi,
@@ -1562,7 +1573,155 @@ struct hps_batch {
   std::vector<hipEvent_t> evpool;
   double prof_ms[HPS_PROFILE_FAMILIES] = {};
   uint32_t prof_n[HPS_PROFILE_FAMILIES] = {};
+  // grouped field solve (field_mode 1): class tables, charges, partial sums
+  int field_mode = 0;
+  double aspect_step = 0.05, aspect_lnr = 0.0;
+  uint32_t gemm_key[3] = {0, 0, 0};    // nr, nz, members the buffers were sized for
+  double gemm_key_lnr = 0.0;           // aspect grid the class tables hold
+  GemmMember* d_mem = nullptr;
+  GemmState* d_gs = nullptr;
+  uint32_t* d_perm = nullptr;
+  float *d_tables = nullptr, *d_q = nullptr, *d_w = nullptr;
+  MemberMap mm{};
+  // surrogate field network (grouped mode): inputs, device control
+  hps_nn* sur = nullptr;
+  hps_nn* capture = nullptr;
+  SurCtl sur_cfg{};
+  SurCtl* d_sur = nullptr;
+  float *d_xinf = nullptr, *d_uinf = nullptr;
+  uint32_t gemm_shape = UINT32_MAX;    // chosen product shape (UINT32_MAX: tune)
+  uint32_t gemm_shape_key[3] = {0, 0, 0};
+  std::vector<float> gemm_times;
+  uint32_t cus = 0;
+  // lattice-period graph: [0] record offset of the replayed period, [1] zero
+  uint64_t* d_ctl = nullptr;
+  bool use_graph = true;
+  hipGraphExec_t pgraph_exec = nullptr;
+  std::vector<uint8_t> pgraph_sig;
+  uint64_t period_replays = 0;
+  uint32_t period_launches = 0;
 };
+
+// One launch of a batch program for the period search: its kind and map
+// class, whether it reads per-step records, and their offset.
+struct PeriodKey {
+  int kind, op_class;
+  bool carries;
+  size_t rec;
+};
+// A run of n >= 2 repetitions of the p launches starting at a, whose record
+// offsets advance by stride bytes per repetition.
+struct PeriodRun {
+  size_t a, p;
+  uint32_t n;
+  uint64_t stride;
+};
+
+// The longest periodic run (most launches covered, then the shortest
+// period) with a period of at most 128 launches starting within the first
+// two periods.
+static PeriodRun find_period_run(const std::vector<PeriodKey>& k)
+{
+  PeriodRun best{0, 0, 0, 0};
+  const size_t L = k.size();
+  for (size_t p = 1; p <= 128 && 2 * p <= L; ++p) {
+    for (size_t a = 0; a < 2 * p && a + 2 * p <= L; ++a) {
+      size_t j0 = a;
+      while (j0 < a + p && !k[j0].carries) ++j0;
+      if (j0 == a + p) continue;
+      const uint64_t stride = (uint64_t)(k[j0 + p].rec - k[j0].rec);
+      uint32_t n = 1;
+      for (;;) {
+        const size_t b = a + (size_t)n * p;
+        if (b + p > L) break;
+        bool same = true;
+        for (size_t j = b; j < b + p && same; ++j) {
+          const PeriodKey &x = k[j], &y = k[j - p];
+          same = x.kind == y.kind && x.op_class == y.op_class && x.carries == y.carries &&
+                 (!x.carries || x.rec - y.rec == stride) && (x.carries || x.rec == y.rec);
+        }
+        if (!same) break;
+        ++n;
+      }
+      if (n >= 2 && (size_t)n * p > (size_t)best.n * best.p) best = {a, p, n, stride};
+    }
+  }
+  return best;
+}
+
+static void free_gemm(hps_batch* bt)
+{
+  (void)hipFree(bt->d_mem); (void)hipFree(bt->d_gs); (void)hipFree(bt->d_perm);
+  (void)hipFree(bt->d_tables); (void)hipFree(bt->d_q); (void)hipFree(bt->d_w);
+  (void)hipFree(bt->mm.pos_exact); (void)hipFree(bt->mm.pos_nn); (void)hipFree(bt->mm.fallback);
+  (void)hipFree(bt->mm.qtot); (void)hipFree(bt->d_xinf); (void)hipFree(bt->d_uinf);
+  bt->mm = MemberMap{};
+  bt->d_xinf = nullptr; bt->d_uinf = nullptr;
+  bt->d_mem = nullptr; bt->d_gs = nullptr; bt->d_perm = nullptr;
+  bt->d_tables = nullptr; bt->d_q = nullptr; bt->d_w = nullptr;
+  bt->gemm_key[0] = bt->gemm_key[1] = bt->gemm_key[2] = 0;
+}
+
+// Buffers of the grouped field solve for B members on an nr x nz mesh; the
+// class-table cache starts empty (a new mesh size or aspect grid discards it).
+static int alloc_gemm(hps_batch* bt, uint32_t nr, uint32_t nz, uint32_t B, hipStream_t s)
+{
+  if (bt->d_gs && bt->gemm_key[0] == nr && bt->gemm_key[1] == nz && bt->gemm_key[2] >= B &&
+      bt->gemm_key_lnr == bt->aspect_lnr)
+    return 0;
+  HPS_CHECK(hipStreamSynchronize(s));
+  free_gemm(bt);
+  const size_t slot_sz = (size_t)(nr + 1) * nr * 4 * nz;
+  const size_t M = 2 * (size_t)(nr + 1) * (nz + 1), K = (size_t)nr * nz;
+  HPS_CHECK(hipMalloc(&bt->d_mem, B * sizeof(GemmMember)));
+  HPS_CHECK(hipMalloc(&bt->d_gs, sizeof(GemmState)));
+  // kGemmCols product columns per member (its aspect classes)
+  HPS_CHECK(hipMalloc(&bt->d_perm, kGemmCols * B * sizeof(uint32_t)));
+  HPS_CHECK(hipMalloc(&bt->d_tables, kGemmSlots * slot_sz * sizeof(float)));
+  HPS_CHECK(hipMalloc(&bt->d_q, kGemmCols * B * K * sizeof(float)));
+  HPS_CHECK(hipMalloc(&bt->d_w, kGemmMaxSplits * kGemmCols * B * M * sizeof(float)));
+  HPS_CHECK(hipMalloc(&bt->mm.pos_exact, kGemmCols * B * sizeof(int32_t)));
+  HPS_CHECK(hipMalloc(&bt->mm.pos_nn, B * sizeof(int32_t)));
+  HPS_CHECK(hipMalloc(&bt->mm.fallback, B * sizeof(uint32_t)));
+  HPS_CHECK(hipMemset(bt->mm.fallback, 0, B * sizeof(uint32_t)));
+  HPS_CHECK(hipMalloc(&bt->mm.qtot, B * sizeof(float)));
+  HPS_CHECK(hipMalloc(&bt->d_xinf, B * (size_t)(nz + kZPad) * nr * sizeof(float)));
+  HPS_CHECK(hipMemset(bt->d_xinf, 0, B * (size_t)(nz + kZPad) * nr * sizeof(float)));
+  HPS_CHECK(hipMalloc(&bt->d_uinf, B * sizeof(float)));
+  GemmState g{};
+  for (uint32_t k = 0; k < kGemmSlots; ++k) g.slot_class[k] = INT32_MIN;
+  HPS_CHECK(hipMemcpy(bt->d_gs, &g, sizeof g, hipMemcpyHostToDevice));
+  bt->gemm_key[0] = nr; bt->gemm_key[1] = nz; bt->gemm_key[2] = B;
+  bt->gemm_key_lnr = bt->aspect_lnr;
+  return 0;
+}
+
+// Times every product shape once on the live operands of a kick point whose
+// schedule, class tables and charges are in place, and keeps the fastest;
+// the caller then runs the chosen shape for the results.
+static int tune_gemm(hps_batch* bt, uint32_t nr, uint32_t nz, uint32_t B, hipStream_t s)
+{
+  const uint32_t n = gemm_shapes();
+  std::vector<hipEvent_t> ev(2 * n);
+  for (auto& e : ev) HPS_CHECK(hipEventCreate(&e));
+  HPS_CHECK(launch_gemm_product(bt->d_gs, bt->d_tables, bt->d_q, bt->d_w, nr, nz, 0, bt->cus, s));
+  for (uint32_t k = 0; k < n; ++k) {
+    HPS_CHECK(hipEventRecord(ev[2 * k], s));
+    HPS_CHECK(launch_gemm_product(bt->d_gs, bt->d_tables, bt->d_q, bt->d_w, nr, nz, k, bt->cus, s));
+    HPS_CHECK(hipEventRecord(ev[2 * k + 1], s));
+  }
+  HPS_CHECK(hipEventSynchronize(ev[2 * n - 1]));
+  bt->gemm_times.assign(n, 0.0f);
+  uint32_t best = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    HPS_CHECK(hipEventElapsedTime(&bt->gemm_times[k], ev[2 * k], ev[2 * k + 1]));
+    if (bt->gemm_times[k] < bt->gemm_times[best]) best = k;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  bt->gemm_shape = best;
+  bt->gemm_shape_key[0] = nr; bt->gemm_shape_key[1] = nz; bt->gemm_shape_key[2] = B;
+  return 0;
+}
 
 // Times every field-sum launch shape once on the batch's live tables (the
 // rebuild of a kick point the control step marked for remesh, so the kernels
@@ -1611,6 +1770,10 @@ extern "C" int hps_batch_destroy(hps_batch* bt)
   if (!bt) return 0;
   if (!bt->sims.empty()) (void)hipStreamSynchronize(bt->sims[0]->b->stream);
   for (hipEvent_t e : bt->evpool) (void)hipEventDestroy(e);
+  free_gemm(bt);
+  if (bt->pgraph_exec) (void)hipGraphExecDestroy(bt->pgraph_exec);
+  (void)hipFree(bt->d_ctl);
+  (void)hipFree(bt->d_sur);
   (void)hipFree(bt->d_rec);
   (void)hipEventDestroy(bt->ev0);
   (void)hipEventDestroy(bt->ev1);
@@ -1669,6 +1832,106 @@ extern "C" int hps_batch_get_tuning(hps_batch* bt, uint32_t* shape, float* ms, u
   return 0;
 }
 
+extern "C" int hps_batch_set_capture(hps_batch* bt, hps_nn* net)
+{
+  if (!bt) return fail("null batch");
+  bt->capture = net;
+  return 0;
+}
+
+extern "C" int hps_batch_set_surrogate(hps_batch* bt, hps_nn* net, uint32_t audit_every,
+                                       double max_err)
+{
+  if (!bt) return fail("null batch");
+  if (net && !(max_err > 0.0)) return fail("the surrogate needs a positive error bound");
+  bt->sur = net;
+  if (!net) return 0;
+  if (!bt->d_sur) HPS_CHECK(hipMalloc(&bt->d_sur, sizeof(SurCtl)));
+  SurCtl c{};
+  c.audit_every = audit_every;
+  c.err_bound = (float)max_err;
+  bt->sur_cfg = c;
+  HPS_CHECK(hipMemcpy(bt->d_sur, &c, sizeof c, hipMemcpyHostToDevice));
+  if (bt->mm.fallback && !bt->sims.empty())
+    HPS_CHECK(hipMemset(bt->mm.fallback, 0, bt->gemm_key[2] * sizeof(uint32_t)));
+  return 0;
+}
+
+extern "C" int hps_batch_get_surrogate_info(hps_batch* bt, double* info)
+{
+  if (!bt || !info) return fail("null argument");
+  SurCtl c{};
+  if (bt->d_sur) {
+    if (!bt->sims.empty()) HPS_CHECK(hipStreamSynchronize(bt->sims[0]->b->stream));
+    HPS_CHECK(hipMemcpy(&c, bt->d_sur, sizeof c, hipMemcpyDeviceToHost));
+  }
+  info[0] = c.kick;
+  info[1] = c.audited;
+  info[2] = c.audit_fail;
+  info[3] = c.max_err;
+  info[4] = c.nn_member_kicks;
+  info[5] = c.exact_member_kicks;
+  return 0;
+}
+
+extern "C" int hps_batch_set_graph(hps_batch* bt, int on)
+{
+  if (!bt) return fail("null batch");
+  bt->use_graph = on != 0;
+  return 0;
+}
+
+extern "C" int hps_batch_graph_info(hps_batch* bt, uint64_t* replays, uint32_t* period_launches)
+{
+  if (!bt || !replays || !period_launches) return fail("null argument");
+  *replays = bt->period_replays;
+  *period_launches = bt->period_launches;
+  return 0;
+}
+
+extern "C" int hps_batch_set_field_mode(hps_batch* bt, int mode, double aspect_step)
+{
+  if (!bt) return fail("null batch");
+  if (mode != 0 && mode != 1) return fail("field mode must be 0 (FP64 per beam) or 1 (grouped FP32 GEMM)");
+  if (mode == 1 && !(aspect_step > 0.0 && aspect_step <= 0.5))
+    return fail("the aspect grid step must be in (0, 0.5]");
+  bt->field_mode = mode;
+  if (mode == 1) {
+    bt->aspect_step = aspect_step;
+    bt->aspect_lnr = hps::log(1.0 + aspect_step);
+  } else {
+    bt->aspect_lnr = 0.0;
+  }
+  return 0;
+}
+
+extern "C" int hps_batch_set_gemm_shape(hps_batch* bt, uint32_t shape)
+{
+  if (!bt) return fail("null batch");
+  if (shape != UINT32_MAX && shape >= gemm_shapes()) return fail("no such GEMM shape");
+  bt->gemm_shape = shape;
+  bt->gemm_shape_key[0] = bt->gemm_shape_key[1] = bt->gemm_shape_key[2] = 0;
+  if (shape != UINT32_MAX) bt->gemm_shape_key[2] = UINT32_MAX;   // fixed for every batch shape
+  return 0;
+}
+
+extern "C" int hps_batch_get_gemm_info(hps_batch* bt, uint64_t* info, float* ms, uint32_t n)
+{
+  if (!bt || !info) return fail("null argument");
+  GemmState g{};
+  if (bt->d_gs && !bt->sims.empty()) {
+    HPS_CHECK(hipStreamSynchronize(bt->sims[0]->b->stream));
+    HPS_CHECK(hipMemcpy(&g, bt->d_gs, sizeof g, hipMemcpyDeviceToHost));
+  }
+  info[0] = bt->gemm_shape;
+  info[1] = g.builds_total;
+  info[2] = g.conflicts;
+  info[3] = g.nrem;
+  info[4] = g.exact_total;
+  for (uint32_t k = 0; ms && k < n && k < bt->gemm_times.size(); ++k) ms[k] = bt->gemm_times[k];
+  return 0;
+}
+
 extern "C" int hps_batch_last_time_ms(hps_batch* bt, double* out)
 {
   if (!bt || !out) return fail("null argument");
@@ -1707,6 +1970,24 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
         return fail("the SCHEFF meshes of a batch must have the same size");
       if ((size_t)cf.nx * cf.nz * 8 + kBlockLdsReserve > m->sc->max_lds)
         return fail("the SCHEFF mesh has more bins than one workgroup's LDS histogram holds");
+      if (bt->field_mode == 1) {
+        if (cf.adj_bunch != 0)
+          return fail("the grouped field solve needs adj_bunch = 0 (no image rings)");
+        if (cf.nz % 64 != 0)
+          return fail("the grouped field solve needs nz to be a multiple of 64");
+      }
+    }
+  }
+  if (sc_on && bt->field_mode == 1 && B > kGemmMaxMembers)
+    return fail("the grouped field solve takes at most 4096 simulations");
+  if (sc_on && (bt->sur || bt->capture)) {
+    if (bt->field_mode != 1)
+      return fail("the surrogate and the training capture need the grouped field solve");
+    for (hps_nn* h : {bt->sur, bt->capture}) {
+      if (!h) continue;
+      const NetView v = net_view(net_of(h));
+      if ((uint32_t)v.nr != m0->sc->cfg.nx || (uint32_t)v.nz != m0->sc->cfg.nz)
+        return fail("the surrogate network was made for another mesh size");
     }
   }
   // compile every simulation (restoring the simulators on a structure error)
@@ -1794,7 +2075,8 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
       hps_sim* m = bt->sims[k];
       BuildArgs& a = build[k];
       a.sc = m->sc->dev; a.fld1 = m->sc->fld1; a.b = m->b->d; a.st = m->b->st; a.bins = m->sc->bins;
-      a.greens_blocks = gb;
+      // the grouped field solve has no per-beam Green's table
+      a.greens_blocks = bt->field_mode == 1 ? 0 : gb;
       a.dep_blocks = m->b->blck_size / 4 != 0 ? deposit_blocks(m->b->n, 4) : 0;
       const uint32_t nb = sc_build_blocks(a);
       for (uint32_t c = 0; c < nb; ++c)
@@ -1804,6 +2086,40 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
       f.part = m->sc->part;
     }
   }
+  const bool gemm = sc_on && bt->field_mode == 1;
+  if (gemm) {
+    if (alloc_gemm(bt, nr, nz, (uint32_t)B, s)) { restore(); return -1; }
+    if (!bt->cus) {
+      int dev = 0, n = 0;
+      HPS_CHECK(hipGetDevice(&dev));
+      HPS_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+      bt->cus = (uint32_t)n;
+    }
+    std::vector<GemmMember> mem(B);
+    for (size_t k = 0; k < B; ++k) {
+      hps_sim* m = bt->sims[k];
+      mem[k].sc = m->sc->dev; mem[k].bins = m->sc->bins; mem[k].fld2 = m->sc->fld2;
+      mem[k].n = m->b->n; mem[k].mass = m->b->mass;
+    }
+    HPS_CHECK(hipMemcpyAsync(bt->d_mem, mem.data(), B * sizeof(GemmMember),
+                             hipMemcpyHostToDevice, s));
+    if (bt->gemm_shape_key[2] != UINT32_MAX &&
+        (bt->gemm_shape_key[0] != nr || bt->gemm_shape_key[1] != nz ||
+         bt->gemm_shape_key[2] != (uint32_t)B))
+      bt->gemm_shape = UINT32_MAX;
+    if (bt->sur) {
+      NetImpl* net = net_of(bt->sur);
+      if (net_reserve_infer(net, (int)B)) { restore(); return -1; }
+      const NetView v = net_view(net);
+      SurCtl& c = bt->sur_cfg;
+      c.out_scale = v.out_scale; c.u_mu = v.u_mu; c.inv_u_sigma = 1.0f / v.u_sigma;
+      c.lnr = bt->aspect_lnr;
+      HPS_CHECK(hipMemcpyAsync(bt->d_sur, &c, kSurCtlConfigBytes, hipMemcpyHostToDevice, s));
+    }
+  }
+  NetData cap_data{};
+  if (gemm && bt->capture && net_data(net_of(bt->capture), &cap_data)) { restore(); return -1; }
+  if (gemm && bt->capture && !cap_data.x) { restore(); return fail("the capture dataset has no capacity (hps_nn_data_reserve)"); }
   auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
   size_t off = 0;
   const size_t off_chunk = off; off = align(off + chunk_map.size() * 4);
@@ -1850,59 +2166,119 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
   size_t uploaded = 0;         // bytes of H already on the device
   std::vector<std::pair<int, size_t>> marks;
   size_t nev = 0;
-  auto flush = [&]() -> int {
+  if (!bt->d_ctl) {
+    HPS_CHECK(hipMalloc(&bt->d_ctl, 2 * sizeof(uint64_t)));
+    HPS_CHECK(hipMemset(bt->d_ctl, 0, 2 * sizeof(uint64_t)));
+  }
+  uint64_t* const d_roff = bt->d_ctl;           // record offset of a replayed period
+  const uint64_t* const d_zero = bt->d_ctl + 1;  // eager launches
+  // one launch of the program (roff: where the batched kernels add the
+  // replayed period's record offset)
+  auto issue = [&](const BLaunch& l, const uint64_t* roff) -> int {
+    const void* rec = bt->d_rec + l.rec;
+    switch (l.kind) {
+      case L_SEG:
+        HPS_CHECK(launch_segment_batch((const SegArgs*)rec, d_chunk_map, total_chunks,
+                                       l.op_class, roff, s));
+        break;
+      case L_SEG_FINAL:
+        HPS_CHECK(launch_seg_final_batch((const SegArgs*)rec, (uint32_t)B, roff, s));
+        break;
+      case L_PHASE:
+        HPS_CHECK(launch_phase_batch((const PhaseArgs*)rec, d_chunk_map, total_chunks, roff, s));
+        break;
+      case L_PHASE_FINAL:
+        HPS_CHECK(launch_phase_final_batch((const PhaseArgs*)rec, (uint32_t)B, roff, s));
+        break;
+      case L_BUILD:
+        HPS_CHECK(launch_sc_build_batch(d_build, d_build_map, build_blocks, nr, nz, s));
+        break;
+      default: {
+        if (gemm) {
+          SurCtl* ctl = bt->sur ? bt->d_sur : nullptr;
+          HPS_CHECK(launch_gemm_prepare(bt->d_mem, (uint32_t)B, bt->d_gs, bt->d_perm, bt->mm, ctl,
+                                        bt->d_tables, bt->d_q, nr, nz, bt->aspect_lnr, bt->cus,
+                                        s));
+          if (bt->gemm_shape == UINT32_MAX && tune_gemm(bt, nr, nz, (uint32_t)B, s)) return -1;
+          if (bt->profile) {   // the product alone (family 6), inside the field family
+            while (bt->evpool.size() < nev + 2) {
+              hipEvent_t e;
+              HPS_CHECK(hipEventCreate(&e));
+              bt->evpool.push_back(e);
+            }
+            marks.push_back({6, nev});
+            HPS_CHECK(hipEventRecord(bt->evpool[nev], s));
+          }
+          HPS_CHECK(launch_gemm_product(bt->d_gs, bt->d_tables, bt->d_q, bt->d_w, nr, nz,
+                                        bt->gemm_shape, bt->cus, s));
+          if (bt->profile) {
+            HPS_CHECK(hipEventRecord(bt->evpool[nev + 1], s));
+            nev += 2;
+          }
+          float* y = nullptr;
+          if (ctl) {
+            // the network serves every remeshing member; the audit compares it
+            // with the exact product where both ran
+            HPS_CHECK(launch_sur_input(bt->d_mem, (uint32_t)B, bt->mm, ctl, bt->d_xinf,
+                                       bt->d_uinf, nr, nz, s));
+            if (net_infer(net_of(bt->sur), bt->d_xinf, bt->d_uinf, &ctl->nn_n, (int)B, &y, s))
+              return -1;
+            HPS_CHECK(launch_sur_audit(bt->d_mem, (uint32_t)B, bt->d_gs, bt->mm, bt->d_w, y, ctl,
+                                       nr, nz, bt->gemm_shape, s));
+          }
+          if (bt->capture)
+            HPS_CHECK(launch_capture(bt->d_mem, (uint32_t)B, bt->d_gs, bt->mm, bt->d_w, cap_data,
+                                     bt->aspect_lnr, nr, nz, bt->gemm_shape, s));
+          HPS_CHECK(launch_gemm_store(bt->d_mem, (uint32_t)B, bt->d_gs, bt->mm, bt->d_w, y, ctl,
+                                      nr, nz, bt->aspect_lnr, bt->gemm_shape, s));
+          break;
+        }
+        if (bt->autotune && bt->field_shape == UINT32_MAX) {
+          // first rebuild of this batch shape: time every field launch shape
+          // on the live tables (each gives the same bits) and keep the fastest
+          if (tune_field(bt, d_field, (uint32_t)B, nr, nz, m0->sc->max_lds, s)) return -1;
+        }
+        HPS_CHECK(launch_field_batch(d_field, (uint32_t)B, nr, nz, m0->sc->max_lds,
+                                     bt->field_shape, s));
+        break;
+      }
+    }
+    return 0;
+  };
+  auto upload = [&]() -> int {
     if (cursor > uploaded) {
       HPS_CHECK(hipMemcpyAsync(bt->d_rec + uploaded, H.data() + uploaded, cursor - uploaded,
                                hipMemcpyHostToDevice, s));
       uploaded = cursor;
     }
-    for (const BLaunch& l : launches) {
+    return 0;
+  };
+  auto eager = [&](size_t from, size_t to) -> int {
+    for (size_t j = from; j < to; ++j) {
+      const BLaunch& l = launches[j];
+      size_t ev = 0;   // this launch's event pair (events inside it come after)
       if (bt->profile) {
         while (bt->evpool.size() < nev + 2) {
           hipEvent_t e;
           HPS_CHECK(hipEventCreate(&e));
           bt->evpool.push_back(e);
         }
-        marks.push_back({l.kind, nev});
-        HPS_CHECK(hipEventRecord(bt->evpool[nev], s));
-      }
-      const void* rec = bt->d_rec + l.rec;
-      switch (l.kind) {
-        case L_SEG:
-          HPS_CHECK(launch_segment_batch((const SegArgs*)rec, d_chunk_map, total_chunks,
-                                         l.op_class, s));
-          break;
-        case L_SEG_FINAL:
-          HPS_CHECK(launch_seg_final_batch((const SegArgs*)rec, (uint32_t)B, s));
-          break;
-        case L_PHASE:
-          HPS_CHECK(launch_phase_batch((const PhaseArgs*)rec, d_chunk_map, total_chunks, s));
-          break;
-        case L_PHASE_FINAL:
-          HPS_CHECK(launch_phase_final_batch((const PhaseArgs*)rec, (uint32_t)B, s));
-          break;
-        case L_BUILD:
-          HPS_CHECK(launch_sc_build_batch(d_build, d_build_map, build_blocks, nr, nz, s));
-          break;
-        default: {
-          if (bt->autotune && bt->field_shape == UINT32_MAX) {
-            // first rebuild of this batch shape: time every field launch shape
-            // on the live tables (each gives the same bits) and keep the fastest
-            if (tune_field(bt, d_field, (uint32_t)B, nr, nz, m0->sc->max_lds, s)) return -1;
-          }
-          HPS_CHECK(launch_field_batch(d_field, (uint32_t)B, nr, nz, m0->sc->max_lds,
-                                       bt->field_shape, s));
-          break;
-        }
-      }
-      if (bt->profile) {
-        HPS_CHECK(hipEventRecord(bt->evpool[nev + 1], s));
+        ev = nev;
         nev += 2;
+        marks.push_back({l.kind, ev});
+        HPS_CHECK(hipEventRecord(bt->evpool[ev], s));
       }
+      if (issue(l, d_zero)) return -1;
+      if (bt->profile) HPS_CHECK(hipEventRecord(bt->evpool[ev + 1], s));
     }
+    return 0;
+  };
+  auto flush = [&]() -> int {
+    if (upload() || eager(0, launches.size())) return -1;
     launches.clear();
     return 0;
   };
+  bool host_steps = false;
 
   HPS_CHECK(hipEventRecord(bt->ev0, s));
   // the records of the host beam state (current, frequency) are built after
@@ -1911,6 +2287,7 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
     const Step& st0 = c0.steps[i];
     if (st0.kind == 1) {
       if (flush()) return -1;
+      host_steps = true;
       for (size_t k = 0; k < B; ++k)
         if (change_frequency(bt->sims[k]->b, cs[k].steps[i].new_freq)) return -1;
       continue;
@@ -1962,7 +2339,69 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
       cursor += B * sizeof(PhaseArgs);
     }
   }
-  if (flush()) return -1;
+  // Lattice periods as one replayed HIP graph: a run of n repetitions of the
+  // same p launches whose records advance by a fixed stride is captured once
+  // (the batched kernels read their records at the base plus a device offset
+  // that the graph's last node advances); the launches before and after the
+  // run go eagerly.  Needs every launch shape chosen (tuning is eager) and no
+  // host step (beam frequency change) inside the program.
+  const bool tuned = !sc_on || (gemm ? bt->gemm_shape != UINT32_MAX
+                                     : (!bt->autotune || bt->field_shape != UINT32_MAX));
+  PeriodRun run{};
+  if (bt->use_graph && !bt->profile && !host_steps && tuned) {
+    std::vector<PeriodKey> keys(launches.size());
+    for (size_t j = 0; j < launches.size(); ++j) {
+      const BLaunch& l = launches[j];
+      const bool carries = l.kind != L_BUILD && l.kind != L_FIELD;
+      keys[j] = {l.kind, l.op_class, carries, l.rec};
+    }
+    run = find_period_run(keys);
+  }
+  if (run.n >= 2) {
+    if (upload() || eager(0, run.a)) return -1;
+    std::vector<uint8_t> sig;
+    const auto put = [&](const void* p, size_t nb) {
+      const uint8_t* q = (const uint8_t*)p;
+      sig.insert(sig.end(), q, q + nb);
+    };
+    put(&run, sizeof run);
+    for (size_t j = run.a; j < run.a + run.p; ++j) put(&launches[j], sizeof(BLaunch));
+    const void* ptrs[] = {bt->d_rec, d_chunk_map, d_build, d_build_map, d_field, bt->d_mem,
+                          bt->d_gs, bt->d_perm, bt->d_tables, bt->d_q, bt->d_w, bt->d_ctl,
+                          bt->sur, bt->capture, bt->d_sur, bt->d_xinf, cap_data.x,
+                          bt->mm.pos_exact};
+    put(ptrs, sizeof ptrs);
+    const uint32_t dims[] = {(uint32_t)B, nr, nz, total_chunks, build_blocks, bt->field_shape,
+                             bt->gemm_shape, bt->cus, gemm ? 1u : 0u};
+    put(dims, sizeof dims);
+    put(&bt->aspect_lnr, sizeof bt->aspect_lnr);
+    const size_t max_lds = sc_on ? m0->sc->max_lds : 0;
+    put(&max_lds, sizeof max_lds);
+    if (!bt->pgraph_exec || sig != bt->pgraph_sig) {
+      if (bt->pgraph_exec) { (void)hipGraphExecDestroy(bt->pgraph_exec); bt->pgraph_exec = nullptr; }
+      HPS_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
+      int rc = 0;
+      for (size_t j = run.a; j < run.a + run.p && rc == 0; ++j) rc = issue(launches[j], d_roff);
+      hipError_t e1 = rc == 0 ? launch_roff(d_roff, run.stride, true, s) : hipSuccess;
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(s, &g);
+      if (rc != 0) { if (g) (void)hipGraphDestroy(g); return -1; }
+      if (e1 != hipSuccess) { if (g) (void)hipGraphDestroy(g); HPS_CHECK(e1); }
+      HPS_CHECK(e2);
+      hipError_t e3 = hipGraphInstantiate(&bt->pgraph_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HPS_CHECK(e3);
+      bt->pgraph_sig.swap(sig);
+    }
+    HPS_CHECK(launch_roff(d_roff, 0, false, s));
+    for (uint32_t r = 0; r < run.n; ++r) HPS_CHECK(hipGraphLaunch(bt->pgraph_exec, s));
+    bt->period_replays += run.n;
+    bt->period_launches = (uint32_t)run.p;
+    if (eager(run.a + (size_t)run.n * run.p, launches.size())) return -1;
+    launches.clear();
+  } else if (flush()) {
+    return -1;
+  }
   HPS_CHECK(hipEventRecord(bt->ev1, s));
   HPS_CHECK(hipEventSynchronize(bt->ev1));
   float ms = 0;

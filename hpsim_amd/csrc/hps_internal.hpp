@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "hpsim_amd.h"
@@ -34,6 +35,7 @@ struct ScDev {
   double k_pl;          // bunch spacing beta*gamma*lambda (adjacent-bunch images)
   uint32_t adj_eff;     // adjacent bunches each side at this kick point
   uint32_t pad_;
+  double log_aspect;    // log(dz * gamma / dr) of the last remesh (grouped field solve)
 };
 
 // Scratch for the canonical reductions: per-chunk partial sums, padded to a
@@ -144,12 +146,18 @@ uint32_t greens_blocks(uint32_t nr, uint32_t nz);
 
 // Batched forms: recs holds one argument record per simulation (the segment
 // and phase kernels also a block map over their chunks).
+// The kernels read their records at recs + roff[0] bytes (a device word:
+// zero for eager launches, the lattice period's offset in a period graph).
 hipError_t launch_segment_batch(const SegArgs* recs, const uint32_t* map, uint32_t nblocks,
-                                int op_class, hipStream_t s);
-hipError_t launch_seg_final_batch(const SegArgs* recs, uint32_t nmem, hipStream_t s);
+                                int op_class, const uint64_t* roff, hipStream_t s);
+hipError_t launch_seg_final_batch(const SegArgs* recs, uint32_t nmem, const uint64_t* roff,
+                                  hipStream_t s);
 hipError_t launch_phase_batch(const PhaseArgs* recs, const uint32_t* map, uint32_t nblocks,
-                              hipStream_t s);
-hipError_t launch_phase_final_batch(const PhaseArgs* recs, uint32_t nmem, hipStream_t s);
+                              const uint64_t* roff, hipStream_t s);
+hipError_t launch_phase_final_batch(const PhaseArgs* recs, uint32_t nmem, const uint64_t* roff,
+                                    hipStream_t s);
+// roff[0] = v, or roff[0] += v when add
+hipError_t launch_roff(uint64_t* roff, uint64_t v, bool add, hipStream_t s);
 hipError_t launch_sc_build_batch(const BuildArgs* recs, const uint32_t* map, uint32_t nblocks,
                                  uint32_t nr, uint32_t nz, hipStream_t s);
 // Field sum of a batch in launch shape `shape` < field_batch_shapes() (any
@@ -159,6 +167,96 @@ uint32_t field_batch_shapes();
 uint32_t field_batch_default_shape(uint32_t nr, uint32_t nmem);
 hipError_t launch_field_batch(const FieldArgs* recs, uint32_t nmem, uint32_t nr, uint32_t nz,
                               size_t max_lds, uint32_t shape, hipStream_t s);
+
+// Grouped field solve of a batch (hps_field_gemm.hip): per-member record,
+// device state, launch shapes.
+struct GemmMember {
+  const ScDev* sc;
+  const int64_t* bins;     // fixed-point histogram [zs][rs]
+  double* fld2;            // field table [zp][rp] double2
+  uint32_t n;              // beam size (the c1 denominator)
+  uint32_t pad_;
+  double mass;
+};
+// class tables: direct-mapped cache, class c in slot c mod kGemmSlots
+constexpr uint32_t kGemmSlots = 64;
+constexpr uint32_t kGemmMaxMembers = 4096;
+constexpr uint32_t kGemmCols = 3;   // product columns per member: classes c - 1, c, c + 1
+struct GemmState {
+  int32_t slot_class[kGemmSlots];   // class whose table the slot holds (INT32_MIN: none)
+  uint32_t slot_start[kGemmSlots];  // first remeshing position of the slot's members
+  uint32_t slot_count[kGemmSlots];
+  uint32_t build[kGemmSlots];       // slots to build at this kick point
+  uint32_t nbuild;
+  uint32_t nrem;                    // product columns at this kick point (3 per member)
+  uint32_t conflicts;               // members that used another class's table (cumulative)
+  uint32_t builds_total;            // class tables built (cumulative)
+  uint64_t exact_total;             // product columns so far (cumulative)
+};
+struct GemmShape {
+  uint32_t nt;        // beams per tile (32 or 64)
+  uint32_t splits;    // K splits (source-ring ranges)
+  uint32_t per_cu;    // persistent workgroups per CU
+};
+constexpr uint32_t kGemmShapes = 8;
+uint32_t gemm_shapes();
+GemmShape gemm_shape(uint32_t i);
+constexpr uint32_t kGemmMaxSplits = 4;
+// zero rows after the nz bins in the surrogate network's input and output
+constexpr uint32_t kZPad = 8;
+// Per-member positions of one kick point: its product columns (classes
+// c - 1, c, c + 1: pos_exact[3 m + h]) and its place in the network's batch
+// (-1: none), the audit verdict, the total charge.
+struct MemberMap {
+  int32_t* pos_exact;
+  int32_t* pos_nn;
+  uint32_t* fallback;   // last audit failed: exact product until one passes
+  float* qtot;
+};
+// Device control of the surrogate inside a batch (null: no surrogate):
+// configuration (uploaded at every simulate), then running counters.
+struct SurCtl {
+  uint32_t audit_every;          // audit at every audit_every-th kick point (0: never)...
+  float err_bound;               // ... with this relative L2 field error bound
+  float out_scale, u_mu, inv_u_sigma;
+  uint32_t pad_;
+  double lnr;                    // log(1 + aspect step)
+  uint32_t kick;                 // kick points so far
+  uint32_t pending;              // an audit is due at the next kick point with remeshes
+  uint32_t audit_now;
+  uint32_t nn_n;                 // members the network serves at this kick point
+  uint32_t audited, audit_fail;  // member audits, failed member audits
+  uint32_t nn_member_kicks;      // member remeshes served by the network...
+  uint32_t exact_member_kicks;   // ... and by the exact product
+  float max_err;                 // largest audited relative L2 field error
+  uint32_t pad2_;
+};
+constexpr size_t kSurCtlConfigBytes = 32;   // audit_every .. lnr
+static_assert(offsetof(SurCtl, kick) == kSurCtlConfigBytes, "SurCtl configuration block");
+// one kick point: prepare (schedule, class tables to build, charges), the
+// product in launch shape `shape` (all shapes give the same field up to the
+// FP32 summation order of their K splits), store (field tables; y: the
+// network's output for members without an exact product)
+hipError_t launch_gemm_prepare(const GemmMember* mem, uint32_t nmem, GemmState* gs,
+                               uint32_t* perm, MemberMap mm, SurCtl* ctl, float* tables, float* q,
+                               uint32_t nr, uint32_t nz, double lnr, uint32_t cus, hipStream_t s);
+hipError_t launch_gemm_product(const GemmState* gs, const float* tables, const float* q, float* w,
+                               uint32_t nr, uint32_t nz, uint32_t shape, uint32_t cus,
+                               hipStream_t s);
+hipError_t launch_gemm_store(const GemmMember* mem, uint32_t nmem, const GemmState* gs,
+                             MemberMap mm, const float* w, const float* y, const SurCtl* ctl,
+                             uint32_t nr, uint32_t nz, double lnr, uint32_t shape, hipStream_t s);
+// surrogate: network inputs of this kick point, audit against the exact product
+hipError_t launch_sur_input(const GemmMember* mem, uint32_t nmem, MemberMap mm, const SurCtl* ctl,
+                            float* x, float* u, uint32_t nr, uint32_t nz, hipStream_t s);
+hipError_t launch_sur_audit(const GemmMember* mem, uint32_t nmem, const GemmState* gs,
+                            MemberMap mm, const float* w, const float* y, SurCtl* ctl,
+                            uint32_t nr, uint32_t nz, uint32_t shape, hipStream_t s);
+struct NetData;
+// training pairs (charge, exact field) of this kick point into a dataset
+hipError_t launch_capture(const GemmMember* mem, uint32_t nmem, const GemmState* gs, MemberMap mm,
+                          const float* w, const NetData& d, double lnr, uint32_t nr, uint32_t nz,
+                          uint32_t shape, hipStream_t s);
 
 // Generic canonical sums for the getters: over the filtered particles
 // (filter 0: loss == 0, 1: loss + lloss == 0, 2: all) of a, a*a, b, b*b, a*b,
@@ -173,6 +271,12 @@ hipError_t launch_shift(double* a, uint32_t n, double v, hipStream_t s);
 hipError_t launch_scale(double* a, uint32_t n, double v, hipStream_t s);
 hipError_t launch_cut(const double* a, uint32_t* loss, uint32_t n, double lo, double hi,
                       hipStream_t s);
+// Truncated-Gaussian synthetic beam (Philox4x32-10, Box-Muller, |z| <= 3)
+struct GaussSpec {
+  double center[6];   // x, xp, y, yp, phi, w
+  double sigma[6];
+};
+hipError_t launch_sample_gaussian(BeamArrays b, uint32_t seed, const GaussSpec& g, hipStream_t s);
 hipError_t launch_probe(int fn, const double* x, const double* y, double* out, uint32_t n,
                         hipStream_t s);
 

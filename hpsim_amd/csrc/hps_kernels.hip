@@ -587,13 +587,22 @@ __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
 
 // The same over the chunks of several simulations: map[block] packs the
 // simulation (high bits) and its chunk (low kBatchLocalBits bits).
+// A batched launch finds its records at recs + *roff bytes: 0 for an eager
+// launch, the replayed lattice period's offset inside a period graph.
+template <class T>
+__device__ __forceinline__ const T* period_recs(const T* recs, const uint64_t* roff)
+{
+  return (const T*)((const char*)recs + roff[0]);
+}
+
 template <int kClass, int kGroup>
 __global__ __launch_bounds__(kBlock) void k_segment_batch(const SegArgs* __restrict__ recs,
-                                                          const uint32_t* __restrict__ map)
+                                                          const uint32_t* __restrict__ map,
+                                                          const uint64_t* __restrict__ roff)
 {
   __shared__ RedSmem sm;
   const uint32_t e = map[blockIdx.x];
-  const SegArgs& a = recs[e >> kBatchLocalBits];
+  const SegArgs& a = period_recs(recs, roff)[e >> kBatchLocalBits];
   segment_body<kClass, kGroup>(a, e & kBatchLocalMask, sm);
 }
 
@@ -822,9 +831,10 @@ __device__ __forceinline__ void seg_final_body(const SegArgs& a)
 __global__ __launch_bounds__(kBlock) void k_seg_final(SegArgs a) { seg_final_body(a); }
 
 // one workgroup per simulation of a batch
-__global__ __launch_bounds__(kBlock) void k_seg_final_batch(const SegArgs* recs)
+__global__ __launch_bounds__(kBlock) void k_seg_final_batch(const SegArgs* recs,
+                                                            const uint64_t* roff)
 {
-  const SegArgs a = recs[blockIdx.x];
+  const SegArgs a = period_recs(recs, roff)[blockIdx.x];
   seg_final_body(a);
 }
 
@@ -920,10 +930,11 @@ __global__ __launch_bounds__(kWideBlock) void k_phase_wide(PhaseArgs a)
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_phase_batch(const PhaseArgs* recs, const uint32_t* map)
+__global__ __launch_bounds__(kBlock) void k_phase_batch(const PhaseArgs* recs, const uint32_t* map,
+                                                        const uint64_t* roff)
 {
   const uint32_t e = map[blockIdx.x];
-  const PhaseArgs a = recs[e >> kBatchLocalBits];
+  const PhaseArgs a = period_recs(recs, roff)[e >> kBatchLocalBits];
   phase_body(a, e & kBatchLocalMask);
 }
 
@@ -1031,6 +1042,9 @@ __device__ __forceinline__ void phase_final_body(const PhaseArgs& a)
         ss.dr = 3.0 * y_sig_good / ss.nx;
         ss.dz = 3.0 * bl * phi_sig_good / (ss.nz * PI);
       }
+      // the aspect the grouped field solve of a batch interpolates its
+      // shared Green's tables in (gm as the Green's table computes it)
+      sc->log_aspect = log_aspect(ss.dr, ss.dz, 1.0 / ::sqrt(1.0 - beta * beta));
       ss.num_on_mesh = 0;
       rm = 1;
     } else {
@@ -1065,9 +1079,10 @@ __device__ __forceinline__ void phase_final_body(const PhaseArgs& a)
 
 __global__ __launch_bounds__(kBlock) void k_phase_final(PhaseArgs a) { phase_final_body(a); }
 
-__global__ __launch_bounds__(kBlock) void k_phase_final_batch(const PhaseArgs* recs)
+__global__ __launch_bounds__(kBlock) void k_phase_final_batch(const PhaseArgs* recs,
+                                                              const uint64_t* roff)
 {
-  const PhaseArgs a = recs[blockIdx.x];
+  const PhaseArgs a = period_recs(recs, roff)[blockIdx.x];
   phase_final_body(a);
 }
 
@@ -1095,21 +1110,8 @@ __device__ __forceinline__ void greens_entry(const ScDev* sc, double* fld1, uint
   const double zp = (t + 1) * dzg;
   const double rp = rpi * dr;
   const double d = zp - zs;
-  double d2 = d * d;
-  const double c = (rp - rs) * (rp - rs);
-  const double b = (rp + rs) * (rp + rs);
-  const double rsrp = rs * rp;
-  double a = 4.0 * rsrp / (b + d2);
-  double g = 1.0 - a;
-  double h = hps::log(g);
-  double ee = 1.0 + g * (0.4630106 - 0.2452740 * h + g * (0.1077857 - 0.04125321 * h));
-  double ek = 1.38629436 - 0.5 * h + g * (0.1119697 - 0.1213486 * h + g *
-              (0.07253230 - 0.02887472 * h));
-  double er1 = 0.0;
-  a = ::sqrt(b + d2);
-  const double rs2mrp2 = (rs * rs - rp * rp);
-  if (rp != 0.0) er1 = (ek - (rs2mrp2 + d2) * ee / (c + d2)) / (2.0 * rp * a);
-  double ez1 = d * ee / (a * (c + d2));
+  double er1, ez1;
+  ring_green(rs, rp, d, er1, ez1);
   // image rings of the neighbouring bunches, one RF period apart in the
   // bunch frame (space_charge_kernel.cu:74-94)
   const uint32_t adj = sc->adj_eff;
@@ -1119,17 +1121,10 @@ __device__ __forceinline__ void greens_entry(const ScDev* sc, double* fld1, uint
     for (uint32_t i = 1; i <= adj; ++i) {
       double xi = (double)i;
       for (int j = 0; j < 2; ++j) {
-        const double d1 = d - xi * pl;
-        d2 = d1 * d1;
-        a = 4.0 * rsrp / (b + d2);
-        g = 1.0 - a;
-        h = hps::log(g);
-        ee = 1.0 + g * (0.4630106 - 0.2452740 * h + g * (0.1077857 - 0.04125321 * h));
-        ek = 1.38629436 - 0.5 * h + g * (0.1119697 - 0.1213486 * h + g *
-             (0.07253230 - 0.02887472 * h));
-        a = ::sqrt(b + d2);
-        if (rp != 0.0) er1 += (ek - (rs2mrp2 + d2) * ee / (c + d2)) / (2.0 * rp * a);
-        ez1 += d1 * ee / (a * (c + d2));
+        double er, ez;
+        ring_green(rs, rp, d - xi * pl, er, ez);
+        if (rp != 0.0) er1 += er;
+        ez1 += ez;
         xi = -xi;
       }
     }
@@ -1634,17 +1629,17 @@ hipError_t launch_segment(const SegArgs& a, hipStream_t s)
 }
 
 hipError_t launch_segment_batch(const SegArgs* recs, const uint32_t* map, uint32_t nblocks,
-                                int op_class, hipStream_t s)
+                                int op_class, const uint64_t* roff, hipStream_t s)
 {
   if (op_class >= OPC_GAP)
     hipLaunchKernelGGL((k_segment_batch<OPC_GAP, kSegGroupGap>), dim3(nblocks), dim3(kBlock), 0,
-                       s, recs, map);
+                       s, recs, map, roff);
   else if (op_class == OPC_QUAD)
     hipLaunchKernelGGL((k_segment_batch<OPC_QUAD, kSegGroupQuad>), dim3(nblocks), dim3(kBlock),
-                       0, s, recs, map);
+                       0, s, recs, map, roff);
   else
     hipLaunchKernelGGL((k_segment_batch<OPC_LIGHT, kSegGroupLight>), dim3(nblocks),
-                       dim3(kBlock), 0, s, recs, map);
+                       dim3(kBlock), 0, s, recs, map, roff);
   return hipGetLastError();
 }
 
@@ -1654,9 +1649,10 @@ hipError_t launch_seg_final(const SegArgs& a, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_seg_final_batch(const SegArgs* recs, uint32_t nmem, hipStream_t s)
+hipError_t launch_seg_final_batch(const SegArgs* recs, uint32_t nmem, const uint64_t* roff,
+                                  hipStream_t s)
 {
-  hipLaunchKernelGGL(k_seg_final_batch, dim3(nmem), dim3(kBlock), 0, s, recs);
+  hipLaunchKernelGGL(k_seg_final_batch, dim3(nmem), dim3(kBlock), 0, s, recs, roff);
   return hipGetLastError();
 }
 
@@ -1677,15 +1673,29 @@ hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s)
 }
 
 hipError_t launch_phase_batch(const PhaseArgs* recs, const uint32_t* map, uint32_t nblocks,
-                              hipStream_t s)
+                              const uint64_t* roff, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_phase_batch, dim3(nblocks), dim3(kBlock), 0, s, recs, map);
+  hipLaunchKernelGGL(k_phase_batch, dim3(nblocks), dim3(kBlock), 0, s, recs, map, roff);
   return hipGetLastError();
 }
 
-hipError_t launch_phase_final_batch(const PhaseArgs* recs, uint32_t nmem, hipStream_t s)
+hipError_t launch_phase_final_batch(const PhaseArgs* recs, uint32_t nmem, const uint64_t* roff,
+                                    hipStream_t s)
 {
-  hipLaunchKernelGGL(k_phase_final_batch, dim3(nmem), dim3(kBlock), 0, s, recs);
+  hipLaunchKernelGGL(k_phase_final_batch, dim3(nmem), dim3(kBlock), 0, s, recs, roff);
+  return hipGetLastError();
+}
+
+// roff[0] = v (set) or roff[0] += v (advance): the record offset of the
+// next replay of a period graph, as a kernel node (no memset node).
+__global__ void k_roff(uint64_t* roff, uint64_t v, int add)
+{
+  if (threadIdx.x == 0) roff[0] = add ? roff[0] + v : v;
+}
+
+hipError_t launch_roff(uint64_t* roff, uint64_t v, bool add, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_roff, dim3(1), dim3(64), 0, s, roff, v, add ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1885,6 +1895,64 @@ hipError_t launch_cut(const double* a, uint32_t* loss, uint32_t n, double lo, do
                       hipStream_t s)
 {
   hipLaunchKernelGGL(k_cut, dim3(blocks_for(n)), dim3(kBlock), 0, s, a, loss, n, lo, hi);
+  return hipGetLastError();
+}
+
+// --- synthetic beams ---------------------------------------------------------------
+
+// Philox4x32-10 block (Salmon et al., SC'11): counter c, key (k0, k1).
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b)
+{
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
+}
+
+// Coordinate c of particle i: center + sigma * z, z a standard normal
+// truncated to |z| <= 3 by rejection with the attempt number in the Philox
+// counter (i, c, attempt), Box-Muller from two 53-bit uniforms -- the
+// sampler of hpsim_amd/configs.py (truncated_normal) with the engine's own
+// log and cos.  Particle 0 is the reference particle (on axis, centred).
+__global__ void k_sample_gaussian(BeamArrays b, uint32_t seed, GaussSpec g)
+{
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= b.n) return;
+  double* out[6] = {b.x, b.xp, b.y, b.yp, b.phi, b.w};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double z = 0.0;
+    for (uint32_t attempt = 0; attempt <= 64; ++attempt) {
+      uint32_t r[4] = {i, 0u, (uint32_t)c, attempt};
+      philox(r, seed, 0x48505331u);
+      const double u1 = 1.0 - u53(r[0], r[1]);
+      const double u2 = u53(r[2], r[3]);
+      double sn, cs;
+      hps::sincos(2.0 * PI * u2, &sn, &cs);
+      z = ::sqrt(-2.0 * hps::log(u1)) * cs;
+      if (__builtin_fabs(z) <= 3.0) break;
+    }
+    double v = g.center[c] + g.sigma[c] * z;
+    if (i == 0) v = c < 4 ? 0.0 : g.center[c];
+    out[c][i] = v;
+  }
+  b.loss[i] = 0;
+  b.lloss[i] = 0;
+}
+
+hipError_t launch_sample_gaussian(BeamArrays b, uint32_t seed, const GaussSpec& g, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sample_gaussian, dim3(blocks_for(b.n)), dim3(kBlock), 0, s, b, seed, g);
   return hipGetLastError();
 }
 

@@ -1,28 +1,29 @@
-"""The learned SCHEFF field surrogate: network wrapper, training data from the
-engine's own SCHEFF solve, and the normalisation that makes the field
-dimensionless.
+"""The learned SCHEFF field surrogate: the network wrapper (hps_nn_*), its
+normalisation, training-data capture from the engine's own SCHEFF solve in
+batched tracking, and training.
 
-The SCHEFF field table is linear in the charge histogram q and, for a fixed
-mesh shape, scales as c1 / dr^2 (the Green's table entries, space_charge_kernel.cu:38-100,
-are c1 times ring fields with lengths in metres).  So
+The grouped field solve of a batch (Batch.set_field_mode('gemm')) puts every
+beam's mesh on an aspect grid, so its field is
 
-    E(zp, rp) = (c1 Q / dr^2) * F(q / Q, dz gamma / dr)
+    E = (c1 / dr^2) * G(kappa) n
 
-with Q the total charge on the mesh and F dimensionless.  The network learns
-F / s (s: the RMS of F over its training set, kept with the network as its
-output scale) on the (nz+1) x (nr+1) field mesh from four input channels:
-nr nz q / Q (bins at rows zs < nz, columns rs < nr; zero elsewhere; mean 1
-over the mesh), log(dz gamma / dr), zp / nz - 0.5 and rp / nr.
+with n the particles per bin, kappa = dz gamma / dr the cell aspect and G the
+Green's operator normalised to dr = 1 (space_charge_kernel.cu:38-100).  The
+network learns G(kappa) n / Q (Q = sum of n) from x = n nr nz / Q, with the
+conditioning log(kappa); its output is divided by out_scale (the RMS of the
+training targets) and log(kappa) is standardised by (u_mu, u_sigma).
 """
 from __future__ import annotations
 
 import ctypes as C
-import math
 
 import numpy as np
 
 from . import _native as N
 from ._native import check
+
+DEFAULT_WIDTHS = (48, 64, 96, 96)
+DEFAULT_NBOT = 4
 
 
 def _lib():
@@ -33,14 +34,24 @@ def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
-class Network:
-    """The surrogate network (hps_nn_*): H x W field mesh, `width` channels."""
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
 
-    def __init__(self, H, W, width=32, max_batch=32, seed=1):
+
+class Network:
+    """The U-Net surrogate for an nr x nz SCHEFF mesh."""
+
+    def __init__(self, nr=32, nz=128, widths=DEFAULT_WIDTHS, nbot=DEFAULT_NBOT, max_batch=64,
+                 seed=1):
         h = C.c_void_p()
-        check(_lib().hps_nn_create(C.byref(h), int(H), int(W), int(width), int(max_batch), int(seed)))
+        w = (C.c_uint32 * 4)(*[int(v) for v in widths])
+        check(_lib().hps_nn_create(C.byref(h), int(nr), int(nz), w, int(nbot), int(max_batch),
+                                   int(seed)))
         self._h = h
-        self.H, self.W, self.width, self.max_batch = int(H), int(W), int(width), int(max_batch)
+        self.max_batch = int(max_batch)
+        d = (C.c_uint32 * 7)()
+        check(_lib().hps_nn_dims(h, d))
+        self.Z0, self.cin, self.cout, self.cpad, self.zn, self.nr, self.nz = (int(v) for v in d)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -58,141 +69,235 @@ class Network:
         return p
 
     def set_params(self, p):
-        p = np.ascontiguousarray(p, np.float32)
+        p = _f32(p)
         assert p.size == self.num_params
         check(_lib().hps_nn_set_params(self._h, _fp(p)))
 
     def layers(self):
         n = C.c_uint32()
         check(_lib().hps_nn_layers(self._h, None, 0, C.byref(n)))
-        t = np.zeros((n.value, 8), np.int32)
-        check(_lib().hps_nn_layers(self._h, t.ctypes.data_as(C.POINTER(C.c_int32)), n.value,
-                                   C.byref(n)))
-        return t
+        t = (C.c_int32 * (8 * n.value))()
+        check(_lib().hps_nn_layers(self._h, t, n.value, C.byref(n)))
+        return [tuple(t[8 * k:8 * k + 8]) for k in range(n.value)]
 
-    @property
-    def out_scale(self):
-        v = C.c_float()
-        check(_lib().hps_nn_get_output_scale(self._h, C.byref(v)))
-        return v.value
+    def set_norm(self, out_scale, u_mu, u_sigma):
+        check(_lib().hps_nn_set_norm(self._h, float(out_scale), float(u_mu), float(u_sigma)))
 
-    @out_scale.setter
-    def out_scale(self, v):
-        check(_lib().hps_nn_set_output_scale(self._h, float(v)))
+    def norm(self):
+        v = np.zeros(3, np.float32)
+        check(_lib().hps_nn_get_norm(self._h, _fp(v)))
+        return tuple(float(a) for a in v)
 
-    def set_data(self, x, y):
-        """Training set: inputs x and dimensionless fields y; the output scale
-        becomes the RMS of y and the network is trained on y / scale."""
-        x = np.ascontiguousarray(x, np.float32)
-        y = np.asarray(y, np.float32)
-        assert x.shape[1:] == (self.H, self.W, 4) and y.shape[1:] == (self.H, self.W, 2)
-        s = float(np.sqrt(np.mean(np.square(y, dtype=np.float64))))
-        self.out_scale = s
-        ys = np.ascontiguousarray(y / s, np.float32)
-        check(_lib().hps_nn_set_data(self._h, _fp(x), _fp(ys), x.shape[0]))
+    # dataset ------------------------------------------------------------------
+    def reserve(self, capacity):
+        check(_lib().hps_nn_data_reserve(self._h, int(capacity)))
 
-    def train(self, steps, batch=16, lr=1e-3, seed=1):
-        losses = np.empty(int(steps), np.float32)
-        check(_lib().hps_nn_train(self._h, int(steps), int(batch), float(lr), int(seed), _fp(losses)))
+    def size(self):
+        n = C.c_uint32()
+        check(_lib().hps_nn_data_size(self._h, C.byref(n)))
+        return n.value
+
+    def shuffle(self, seed=0):
+        """Random order of the dataset samples."""
+        check(_lib().hps_nn_data_shuffle(self._h, int(seed)))
+
+    def append(self, x, y, log_aspect):
+        x, y, u = _f32(x), _f32(y), _f32(log_aspect)
+        check(_lib().hps_nn_data_append(self._h, _fp(x), _fp(y), _fp(u), len(u)))
+
+    def get(self, first, count, x=True, y=True):
+        """Dataset samples [first, first + count): (x, y, log aspect); x or y
+        None when not asked for."""
+        xa = np.empty((count, self.nz, self.cin), np.float32) if x else None
+        ya = np.empty((count, self.zn, self.cout), np.float32) if y else None
+        u = np.empty(count, np.float32)
+        check(_lib().hps_nn_data_get(self._h, int(first), int(count),
+                                     None if xa is None else _fp(xa),
+                                     None if ya is None else _fp(ya), _fp(u)))
+        return xa, ya, u
+
+    def fit_norm(self, first=0, count=None, sample=2048, seed=0):
+        """out_scale = RMS of the targets over a random sample of the dataset
+        range; (u_mu, u_sigma) = mean and spread of its log aspects."""
+        n = self.size() - first if count is None else count
+        rng = np.random.default_rng(seed)
+        ms = []
+        for i in np.sort(rng.choice(n, min(n, sample), replace=False)) + first:
+            _, y, _ = self.get(int(i), 1, x=False)
+            ms.append(float((y.astype(np.float64) ** 2).mean()))
+        _, _, u = self.get(first, n, x=False, y=False)
+        sig = float(np.std(u))
+        self.set_norm(float(np.sqrt(np.mean(ms))), float(np.mean(u)), sig if sig > 1e-3 else 1.0)
+
+    # training and evaluation ---------------------------------------------------
+    def train(self, steps, batch=64, lr=1e-3, lr_end=1e-5, seed=0, first=0, count=None):
+        """Adam steps on random batches of dataset samples; per-step mean
+        relative squared error of the batch."""
+        n = self.size() - first if count is None else count
+        losses = np.zeros(steps, np.float32)
+        check(_lib().hps_nn_train(self._h, int(first), int(n), int(steps), int(batch),
+                                  float(lr), float(lr_end), int(seed), _fp(losses)))
         return losses
 
-    def predict(self, x):
-        """The dimensionless field F (the network output times its scale)."""
-        x = np.ascontiguousarray(x, np.float32)
-        out = np.empty(x.shape[:3] + (2,), np.float32)
-        check(_lib().hps_nn_predict(self._h, _fp(x), x.shape[0], _fp(out)))
-        return out
+    def eval(self, first=0, count=None):
+        """Relative L2 field error of every sample of a dataset range."""
+        n = self.size() - first if count is None else count
+        loss = np.zeros(n, np.float32)
+        check(_lib().hps_nn_eval(self._h, int(first), int(n), _fp(loss)))
+        return np.sqrt(np.maximum(loss, 0.0))
 
-    def loss_grad(self, x, y):
-        x = np.ascontiguousarray(x, np.float32)
-        y = np.ascontiguousarray(y, np.float32)
-        loss = np.empty(x.shape[0], np.float32)
-        g = np.empty(self.num_params, np.float32)
-        check(_lib().hps_nn_loss_grad(self._h, _fp(x), _fp(y), x.shape[0], _fp(loss), _fp(g)))
+    def predict(self, x, log_aspect):
+        x, u = _f32(x), _f32(log_aspect)
+        y = np.empty((len(u), self.zn, self.cout), np.float32)
+        check(_lib().hps_nn_predict(self._h, _fp(x), _fp(u), len(u), _fp(y)))
+        return y
+
+    def loss_grad(self, x, log_aspect, y):
+        x, u, y = _f32(x), _f32(log_aspect), _f32(y)
+        loss = np.zeros(len(u), np.float32)
+        g = np.zeros(self.num_params, np.float32)
+        check(_lib().hps_nn_loss_grad(self._h, _fp(x), _fp(u), _fp(y), len(u), _fp(loss), _fp(g)))
         return loss, g
 
 
-def field_scale(state, beam_current, beam_freq, mass, num, frac=1.0):
-    """c1 / dr^2 of the SCHEFF tables of one remesh: the factor between the
-    dimensionless field F and the field table (per unit of total charge)."""
-    c1 = 5721.67 * (beam_current * frac / mass) / (beam_freq * 1e6 * num)
-    return c1 / (state["dr"] * state["dr"])
+def attach_capture(batch, net):
+    """Append every exact-product (charge, field) pair of the batch's kick
+    points to net's dataset (None: stop)."""
+    check(_lib().hps_batch_set_capture(batch._h, None if net is None else net._h))
+    batch._capture = net
 
 
-def make_input(bins, nr, nz, aspect):
-    """Input channels [nz+1][nr+1][4] of one histogram (int64 bins [nz][nr] in
-    units of 2^-36); returns (x, Q) with Q the total charge in particles."""
-    q = np.asarray(bins, np.float64).reshape(nz, nr) * 2.0 ** -36
-    Q = float(q.sum())
-    x = np.zeros((nz + 1, nr + 1, 4), np.float32)
-    if Q > 0:
-        x[:nz, :nr, 0] = q * (nr * nz / Q)
-    x[..., 1] = math.log(aspect)
-    x[..., 2] = (np.arange(nz + 1, dtype=np.float32) / nz - 0.5)[:, None]
-    x[..., 3] = (np.arange(nr + 1, dtype=np.float32) / nr)[None, :]
-    return x, Q
+def attach(batch, net, audit_every=20, max_err=0.02):
+    """Let the network serve the batch's fields, audited against the exact
+    product every audit_every kick points with fallback above max_err
+    relative L2 error (None: off)."""
+    check(_lib().hps_batch_set_surrogate(batch._h, None if net is None else net._h,
+                                         int(audit_every), float(max_err)))
+    batch._surrogate = net
 
 
-def aspect_of(state, gamma):
-    return state["dz"] * gamma / state["dr"]
+def info(batch):
+    """Counters of the surrogate in a batch."""
+    v = np.zeros(6)
+    check(_lib().hps_batch_get_surrogate_info(batch._h, N.dptr(v)))
+    return {"kick_points": int(v[0]), "audits": int(v[1]), "audit_failures": int(v[2]),
+            "max_audit_error": float(v[3]), "network_remeshes": int(v[4]),
+            "exact_remeshes": int(v[5])}
 
 
-def rel_l2(a, b):
-    """Relative L2 error of a against b."""
-    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+# ---------------------------------------------------------------------------
+# training data from tracking, training, weights on disk
+# ---------------------------------------------------------------------------
 
-
-def collect_pairs(beam_spec, elements, nr=32, nz=128, interval=0.05, max_pairs=None):
-    """Track one beam element by element on the GPU and keep (input, F,
-    scale) at every SCHEFF remesh: input [nz+1][nr+1][4], the dimensionless
-    field F = field table / (c1 Q / dr^2) [nz+1][nr+1][2] and that factor."""
+def make_batch(bw, members=None, mode="gemm", aspect_step=0.05, sc=True):
+    """A Batch of the BatchWorkload's members (all by default) in the given
+    field mode; returns (batch, [(beam, spacecharge, simulator)])."""
     from . import HPSim
     from . import configs as cfg
-    b = HPSim.Beam(mass=beam_spec.mass, charge=beam_spec.charge, current=beam_spec.current,
-                   num=beam_spec.num)
-    b.set_distribution(*cfg.make_distribution(beam_spec))
-    b.set_frequency(beam_spec.freq)
-    b.set_ref_w(beam_spec.w0)
-    sc = HPSim.SpaceCharge(nr=nr, nz=nz, interval=interval)
-    sim = HPSim.Simulator(b, HPSim.BeamLine(elements=elements), sc)
-    sim.set_graph(False)
-    xs, fs, scales = [], [], []
-    last = 0
-    for i in range(len(elements)):
-        sim.simulate(i, i)
-        st = sc.state()
-        if st["remesh_count"] == last:
-            continue
-        last = st["remesh_count"]
-        _, fld2, bins = sc.tables()
-        gamma = b.stats()["w_avg_good"] / beam_spec.mass + 1.0
-        x, Q = make_input(bins, nr, nz, aspect_of(st, gamma))
-        if Q <= 0:
-            continue
-        scale = field_scale(st, b.get_current(), b.get_frequency(), beam_spec.mass, beam_spec.num) * Q
-        xs.append(x)
-        fs.append((fld2.reshape(nz + 1, nr + 1, 2) / scale).astype(np.float32))
-        scales.append(scale)
-        if max_pairs and len(xs) >= max_pairs:
-            break
-    return np.array(xs), np.array(fs), np.array(scales)
+    objs = []
+    for k in (range(len(bw.beams)) if members is None else members):
+        spec = bw.beams[k]
+        b = HPSim.Beam(mass=spec.mass, charge=spec.charge, current=spec.current, num=spec.num)
+        b.set_distribution(*cfg.make_distribution(spec))
+        b.set_frequency(spec.freq)
+        b.set_ref_w(spec.w0)
+        s = HPSim.SpaceCharge(nr=bw.nr, nz=bw.nz, interval=bw.interval) if sc else None
+        objs.append((b, s, HPSim.Simulator(b, HPSim.BeamLine(elements=bw.lattices[k]), s)))
+    batch = HPSim.Batch([m for _, _, m in objs])
+    if sc and mode == "gemm":
+        batch.set_field_mode("gemm", aspect_step)
+    return batch, objs
 
 
-def training_set(runs, num=65536, periods=40, seed=1, nr=32, nz=128):
-    """(x, F) pairs from `runs` tracking runs of the 40-period lattice with
-    varied current, transverse and longitudinal beam sizes (seeded)."""
+def training_workloads(nr=32, nz=128, seeds=(101, 102, 103), nbeams=64, ramp_beams=16):
+    """Tracking runs whose SCHEFF solves make the training data: batched
+    scans of the 40-period lattice (seeded perturbations, 16,384 .. 262,144
+    particles) and beams on the 200-period accelerating lattice (0.3 -> 0.6
+    MV, its own perturbations); none uses the seeds of the configured
+    workloads."""
     from . import configs as cfg
-    rng = np.random.default_rng(seed)
-    lattice = cfg.build_lattice(cfg.config(0, periods=periods).lattice)
-    X, F = [], []
-    for k in range(runs):
-        spec = cfg.BeamSpec(num=num, seed=seed * 1000 + k,
-                            current=float(rng.uniform(0.002, 0.03)),
-                            sig_x=1e-3 * float(rng.uniform(0.6, 1.6)),
-                            sig_y=1e-3 * float(rng.uniform(0.6, 1.6)),
-                            sig_phi=math.radians(float(rng.uniform(5.0, 20.0))),
-                            sig_w=2e-3 * float(rng.uniform(0.5, 2.0)))
-        x, f, _ = collect_pairs(spec, lattice, nr=nr, nz=nz)
-        X.append(x)
-        F.append(f)
-    return np.concatenate(X), np.concatenate(F)
+    out = []
+    for s in seeds:
+        bw = cfg.batched_scan(nbeams=nbeams, nmin=16384, nmax=262144, periods=40, seed=s)
+        bw.nr, bw.nz = nr, nz
+        out.append(bw)
+    base = cfg.config(1, periods=200)
+    lat = cfg.build_lattice(base.lattice)
+    beams = [cfg.BeamSpec(num=65536, seed=7000 + k) for k in range(ramp_beams)]
+    lats = [cfg.perturb_lattice(lat, 9000 + k) for k in range(ramp_beams)]
+    out.append(cfg.BatchWorkload(f"ramp-{ramp_beams}x65536x200", beams, lats, nr=nr, nz=nz))
+    return out
+
+
+def collect(net, workloads, aspect_step=0.05, log=None):
+    """Tracks every workload as one grouped batch with the capture on: each
+    remesh appends (charge, exact field) to net's dataset."""
+    for bw in workloads:
+        batch, objs = make_batch(bw, aspect_step=aspect_step)
+        attach_capture(batch, net)
+        batch.simulate()
+        attach_capture(batch, None)
+        if log:
+            log(f"{bw.name}: dataset {net.size()} samples")
+        del batch, objs
+
+
+def save(net, path, widths, nbot):
+    np.savez(path, params=net.params(), norm=np.array(net.norm(), np.float32),
+             dims=np.array([net.nr, net.nz, nbot] + list(widths), np.int64))
+
+
+def load(path, max_batch=64):
+    """A Network with the weights and normalisation saved by save()."""
+    d = np.load(path, allow_pickle=False)
+    nr, nz, nbot = (int(v) for v in d["dims"][:3])
+    widths = tuple(int(v) for v in d["dims"][3:7])
+    net = Network(nr=nr, nz=nz, widths=widths, nbot=nbot, max_batch=max_batch)
+    net.set_params(d["params"])
+    net.set_norm(*(float(v) for v in d["norm"]))
+    return net
+
+
+def weights_path(nr, nz):
+    import os
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
+                        f"unet_{nr}x{nz}.npz")
+
+
+def train_network(nr=32, nz=128, widths=DEFAULT_WIDTHS, nbot=DEFAULT_NBOT, steps=20000,
+                  batch=64, lr=5e-4, lr_end=5e-6, holdout=0.05, seed=1, workloads=None,
+                  capacity=250000, log=print):
+    """Data from the engine's own SCHEFF solves, then Adam; returns (net,
+    held-out relative L2 errors (a random 5 % of the samples), per-step
+    losses)."""
+    net = Network(nr=nr, nz=nz, widths=widths, nbot=nbot, max_batch=batch, seed=seed)
+    net.reserve(capacity)
+    collect(net, workloads if workloads is not None else training_workloads(nr, nz), log=log)
+    net.shuffle(seed)
+    n = net.size()
+    n_hold = max(1, int(n * holdout))
+    net.fit_norm(0, n - n_hold)
+    losses = net.train(steps, batch=batch, lr=lr, lr_end=lr_end, seed=seed, first=0,
+                       count=n - n_hold)
+    err = net.eval(n - n_hold, n_hold)
+    if log:
+        log(f"trained {net.num_params} parameters on {n - n_hold} samples, {steps} steps: "
+            f"held-out relative L2 error median {np.median(err):.4f}, "
+            f"95th percentile {np.percentile(err, 95):.4f}, max {err.max():.4f}")
+    return net, err, losses
+
+
+if __name__ == "__main__":
+    import argparse
+    import os
+    ap = argparse.ArgumentParser(description="train the SCHEFF field surrogate on the GPU")
+    ap.add_argument("--nr", type=int, default=32)
+    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    net, err, _ = train_network(a.nr, a.nz, steps=a.steps)
+    out = a.out or weights_path(a.nr, a.nz)
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    save(net, out, DEFAULT_WIDTHS, DEFAULT_NBOT)
+    print(f"saved {out}")

@@ -122,6 +122,16 @@ int hps_beam_set_waterbag(hps_beam* b, double ax, double bx, double ex, double a
  * waterbag, phase a linear ramp over [-dphi, dphi], w = sync_w. */
 int hps_beam_set_dc(hps_beam* b, double ax, double bx, double ex, double ay, double by,
     double ey, double dphi, double sync_phi, double sync_w, uint32_t seed);
+/* A synthetic beam drawn on the device: coordinate c (x, xp, y, yp, phi, w)
+ * of particle i = center[c] + sigma[c] * z, z a standard normal truncated to
+ * |z| <= 3 by rejection, from Philox4x32-10 with counter (i, c, attempt) and
+ * key (seed, 0x48505331), Box-Muller on two 53-bit uniforms; particle 0 is
+ * the reference particle (on axis, at the centre); no losses.  The sampler
+ * of hpsim_amd/configs.py, computed in place of a host upload (the
+ * distribution set_distribution would receive from it, up to the last bit
+ * of log and cos).  No reference counterpart (the reference samples on the
+ * host, beam.cpp:391-537). */
+int hps_beam_sample_gaussian(hps_beam* b, uint32_t seed, const double* center, const double* sigma);
 /* Copy coordinates to host; any pointer may be NULL.  Replaces the array
  * part of get_x/get_xp/.../get_losses, wrap_beam.cpp:440-912. */
 int hps_beam_download(hps_beam* b, double* x, double* xp, double* y, double* yp,
@@ -201,11 +211,12 @@ int hps_sim_set_space_charge(hps_sim* m, int on);
  * later simulate() calls (families: 0 fused element maps + kick + per-chunk
  * moments, 1 per-chunk relative-phase moments, 2 Green's table + charge
  * deposit, 3 field sum, 4 cross-chunk moments and longitudinal loss, 5
- * relative-phase moments + SCHEFF control).  ms and launches hold
- * HPS_PROFILE_FAMILIES entries.  Enabling resets the counters.  No reference
+ * relative-phase moments + SCHEFF control, 6 the matrix-core product of a
+ * batch's grouped field solve, also counted in family 3).  ms and launches
+ * hold HPS_PROFILE_FAMILIES entries.  Enabling resets the counters.  No reference
  * counterpart: the reference only times whole simulate() calls
  * (timer.cpp:4-19). */
-#define HPS_PROFILE_FAMILIES 6
+#define HPS_PROFILE_FAMILIES 7
 int hps_sim_set_profiling(hps_sim* m, int on);
 int hps_sim_get_profile(hps_sim* m, double* ms, uint32_t* launches);
 /* Launch the kernel sequence of simulate() as a HIP graph (default on),
@@ -245,41 +256,113 @@ int hps_batch_last_time_ms(hps_batch* bt, double* out);
  * n candidate times in ms.  No reference counterpart. */
 int hps_batch_set_autotune(hps_batch* bt, int on);
 int hps_batch_get_tuning(hps_batch* bt, uint32_t* shape, float* ms, uint32_t n);
+/* Lattice periods as one HIP graph (default on): when the launch sequence of
+ * a simulate() holds n >= 2 repetitions of one lattice period's launches
+ * (same kernels, records advancing by a fixed stride), that period is
+ * captured once and replayed n times, each replay reading the next period's
+ * records through a device offset; the graph is kept while the program's
+ * structure and buffers are unchanged.  Launches outside the repeated run,
+ * profiling runs, launch-shape tuning and programs with a beam frequency
+ * change go eagerly.  No reference counterpart (the reference launches every
+ * kernel from its element loop, simulation_engine.cu:125-373). */
+int hps_batch_set_graph(hps_batch* bt, int on);
+/* period graph replays so far and launches per replayed period (0: none) */
+int hps_batch_graph_info(hps_batch* bt, uint64_t* replays, uint32_t* period_launches);
+/* Field solve of the batch.  mode 0 (default): each simulation's own FP64
+ * Green's table and field sum, results identical to hps_sim_simulate.  mode
+ * 1: the grouped solve -- the Green's table normalised to dr = 1 depends on
+ * the bunch-frame cell aspect kappa = dz*gamma/dr alone; it is kept for the
+ * grid kappa_c = (1 + aspect_step)^c, each simulation's field is
+ * interpolated quadratically in log(kappa) between its three nearest classes
+ * (on its own, unchanged mesh), and the fields of all simulations that remesh at a
+ * kick point are one grouped matrix product on the FP32 matrix cores (FP32
+ * operands and accumulation), scaled per simulation by its c1 / dr^2.
+ * Replaces UpdateFldTbl1Kernel + UpdateFldTbl2Kernel
+ * (space_charge_kernel.cu:38-342) per beam; needs adj_bunch = 0 and nz a
+ * multiple of 64.  Results are approximate: the interpolation error is
+ * third order in the step (below 5e-6 of the field at 0.05) plus FP32
+ * rounding.  No reference counterpart. */
+int hps_batch_set_field_mode(hps_batch* bt, int mode, double aspect_step);
+/* Launch shape of the grouped product (index < 8, or UINT32_MAX: autotune
+ * at the first rebuild, the default). */
+int hps_batch_set_gemm_shape(hps_batch* bt, uint32_t shape);
+/* info[0] chosen shape, [1] class tables built, [2] product columns that
+ * used another class's table (cache conflicts), [3] product columns of the
+ * last kick point (three per simulation), [4] product columns so far;
+ * ms: the autotuner's time per shape. */
+int hps_batch_get_gemm_info(hps_batch* bt, uint64_t* info, float* ms, uint32_t n);
 /* per-kernel-family device time of the batched launches (families as
  * hps_sim_set_profiling) */
 int hps_batch_set_profiling(hps_batch* bt, int on);
 int hps_batch_get_profile(hps_batch* bt, double* ms, uint32_t* launches);
 
 /* ---- SCHEFF field surrogate network ---- */
-/* A dilated convolutional network on the (nz+1) x (nr+1) field mesh (input
- * [H][W][4]: charge / total charge, log(dz*gamma/dr), z/nz - 0.5, r/nr;
- * output [H][W][2]: E_r, E_z in units of c1 * Q / dr^2), BF16 weights, FP32
- * accumulation on MFMA, trained with Adam by the library's own forward and
- * backward kernels.  No reference counterpart: the reference always solves
- * SCHEFF (space_charge.cu:140-210). */
+/* A one-dimensional convolutional U-Net along the bunch axis z with the
+ * source rings as channels, for an nr x nz SCHEFF mesh (nr a multiple of
+ * 16, nz of 64): input x[z][rs] = charge of bin (z, rs) * nr * nz / Q for
+ * the nz bins (Q the total charge), conditioning the log cell aspect
+ * log(dz*gamma/dr); output y[zp][2 rp + comp] for the (nz+1) x (nr+1) field
+ * nodes = the SCHEFF field before the c1 / dr^2 factor, divided by Q.  Four
+ * levels of widths widths[0..3] (z, z/2, z/4, z/8), nbot dilated layers at
+ * the coarsest, FiLM scales from the conditioning on every layer, no biases
+ * or activations (the field is linear in the charge).  BF16 weights, FP32
+ * accumulation on the matrix cores, Adam training by the library's own
+ * forward and backward kernels.  No reference counterpart: the reference
+ * always solves SCHEFF (space_charge.cu:140-210). */
 typedef struct hps_nn hps_nn;
-int hps_nn_create(hps_nn** out, uint32_t H, uint32_t W, uint32_t width, uint32_t max_batch,
-                  uint32_t seed);
+int hps_nn_create(hps_nn** out, uint32_t nr, uint32_t nz, const uint32_t* widths, uint32_t nbot,
+                  uint32_t max_batch, uint32_t seed);
 int hps_nn_destroy(hps_nn* net);
 uint32_t hps_nn_num_params(const hps_nn* net);
 int hps_nn_get_params(hps_nn* net, float* host);
 int hps_nn_set_params(hps_nn* net, const float* host);
-/* training set: x [count][H][W][4], y [count][H][W][2] (host, copied to the device) */
-int hps_nn_set_data(hps_nn* net, const float* x, const float* y, uint32_t count);
-/* `steps` Adam steps on random batches; losses[k] (optional) the mean relative
- * L2 loss of step k's batch */
-int hps_nn_train(hps_nn* net, uint32_t steps, uint32_t batch, float lr, uint32_t seed,
-                 float* losses);
-int hps_nn_predict(hps_nn* net, const float* x, uint32_t count, float* out);
-/* per-sample losses and the gradient of their mean (for gradient checks) */
-int hps_nn_loss_grad(hps_nn* net, const float* x, const float* y, uint32_t count, float* loss,
-                     float* grad);
-/* The field is the network output times this scale (training targets are
- * divided by it); hps_nn_predict applies it. */
-int hps_nn_set_output_scale(hps_nn* net, float v);
-int hps_nn_get_output_scale(const hps_nn* net, float* v);
-/* layer table rows {Cin, Cout, taps, dilation, relu, residual, weight offset, bias offset} */
+/* layer rows {mode (0 same, 1 point, 2 down, 3 up), Cin, Cout, taps,
+ * dilation, skip layer (-1 none), weight offset, FiLM offset} */
 int hps_nn_layers(const hps_nn* net, int32_t* table, uint32_t max_layers, uint32_t* nlayers);
+/* {Z0 (positions incl. padding), Cin, Cout, Cout padded, field rows, nr, nz} */
+int hps_nn_dims(const hps_nn* net, uint32_t* out);
+/* normalisation: the network predicts y / out_scale from the conditioning
+ * (log aspect - u_mu) / u_sigma */
+int hps_nn_set_norm(hps_nn* net, float out_scale, float u_mu, float u_sigma);
+int hps_nn_get_norm(const hps_nn* net, float* out);
+/* device dataset of (x [nz][nr], y [nz+1][2(nr+1)], log aspect) samples */
+int hps_nn_data_reserve(hps_nn* net, uint32_t capacity);
+int hps_nn_data_append(hps_nn* net, const float* x, const float* y, const float* log_aspect,
+                       uint32_t count);
+int hps_nn_data_size(hps_nn* net, uint32_t* count);
+/* random order of the samples (so that any range is a random subset) */
+int hps_nn_data_shuffle(hps_nn* net, uint32_t seed);
+int hps_nn_data_get(hps_nn* net, uint32_t first, uint32_t count, float* x, float* y,
+                    float* log_aspect);
+/* Adam on random batches of dataset samples [first, first + count), the
+ * learning rate decaying from lr to lr_end (cosine); losses[k] (optional)
+ * the mean relative squared error of step k's batch */
+int hps_nn_train(hps_nn* net, uint32_t first, uint32_t count, uint32_t steps, uint32_t batch,
+                 float lr, float lr_end, uint32_t seed, float* losses);
+/* per-sample relative squared error of dataset samples */
+int hps_nn_eval(hps_nn* net, uint32_t first, uint32_t count, float* loss);
+/* fields of host samples: y [count][nz+1][2(nr+1)] (times out_scale) */
+int hps_nn_predict(hps_nn* net, const float* x, const float* log_aspect, uint32_t count, float* y);
+/* per-sample losses and the gradient of their mean (for gradient checks) */
+int hps_nn_loss_grad(hps_nn* net, const float* x, const float* log_aspect, const float* y,
+                     uint32_t count, float* loss, float* grad);
+
+/* Surrogate in a batch (grouped field solve only).  set_capture: every kick
+ * point appends the (charge, exact field) pairs of its exact products to
+ * the network's device dataset (while its capacity lasts): training data
+ * from the engine's own SCHEFF solve.  set_surrogate: the network serves the
+ * field of every remeshing simulation; at every audit_every-th kick point
+ * (or the next one with remeshes) the exact product runs too and a
+ * simulation whose relative L2 field error exceeds max_err falls back to the
+ * exact product until a later audit passes; audited kick points use the
+ * exact field; audit_every 0: no audits (every field from the network, for
+ * ablations).  net NULL switches either off.  The network must be made for
+ * the batch's mesh size.  No reference counterpart. */
+int hps_batch_set_capture(hps_batch* bt, hps_nn* net);
+int hps_batch_set_surrogate(hps_batch* bt, hps_nn* net, uint32_t audit_every, double max_err);
+/* info: kick points, member audits, failed member audits, largest audited
+ * error, member remeshes served by the network, by the exact product */
+int hps_batch_get_surrogate_info(hps_batch* bt, double* info);
 
 /* ---- Emittances without allocation ---- */
 /* The three emittances of hps_beam_get_emittance (axis 0, 1, 2 -> out[0..2],

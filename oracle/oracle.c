@@ -1184,6 +1184,16 @@ struct orc_spch {
   /* SetParameters state, space_charge.cu:77-86 */
   double beta, freq, frac;
   uint32_t adj_eff;
+  /* field table stored in single precision, as the reference's default
+   * build keeps it (DOUBLE_PRECISION off, make_root.mk:22-23,
+   * space_charge.h:147-161): the precision floor the engine's approximate
+   * field modes are compared with; 0 = FP64 (the checker) */
+  int fp32_table;
+  /* > 0: the field interpolated linearly in log(dz*gm/dr) between the
+   * aspect classes (1+step)^c and (1+step)^(c+1) of the engine's grouped
+   * batch solve, in FP64 (log(1+step)); 0 = the reference field */
+  double interp_lnr;
+  int interp_order;   /* 1 linear (two classes), 2 quadratic (three) */
 };
 
 static void spch_alloc(orc_spch* s)
@@ -1219,6 +1229,12 @@ void orc_spch_set_remesh_threshold(orc_spch* s, double v) { s->s.remesh_val = v;
 void orc_spch_set_adj_bunch(orc_spch* s, uint32_t v) { s->s.adj_bunch = v; }
 void orc_spch_set_adj_bunch_cutoff_w(orc_spch* s, double v) { s->s.adj_bunch_w = v; }
 void orc_spch_set_mesh_size_cutoff_w(orc_spch* s, double v) { s->s.mesh_w = v; }
+void orc_spch_set_fp32_table(orc_spch* s, int on) { s->fp32_table = on != 0; }
+void orc_spch_set_aspect_interp(orc_spch* s, double step, int order)
+{
+  s->interp_lnr = step > 0.0 ? orc_log(1.0 + step) : 0.0;
+  s->interp_order = order;
+}
 /* Scheff::SetMeshSize, space_charge.cpp:61-73 */
 void orc_spch_set_mesh_size(orc_spch* s, uint32_t nr, uint32_t nz)
 {
@@ -1629,6 +1645,42 @@ int orc_spch_start(orc_spch* s, orc_beam* b, double r_length)
                       b->current * s->frac / b->mass, adj_bunch, s->fld1);
     deposit(s, b);
     orc_t_field(s->s.nx, s->s.nz, s->bins, s->fld1, s->fld2);
+    if (s->interp_lnr > 0.0 && adj_bunch == 0) {
+      const uint32_t nx = s->s.nx, nzz = s->s.nz;
+      const size_t n1 = (size_t)(nx + 1) * nx * nzz * 2, n2 = (size_t)(nzz + 1) * (nx + 1) * 2;
+      double* t1 = (double*)malloc(n1 * sizeof(double));
+      double* f0 = (double*)malloc(n2 * sizeof(double));
+      double* f1 = (double*)malloc(n2 * sizeof(double));
+      const double gm = 1.0 / sqrt(1.0 - s->beta * s->beta);
+      const double x = orc_log(s->s.dz * gm / s->s.dr) / s->interp_lnr;
+      double* f2 = (double*)malloc(n2 * sizeof(double));
+      if (s->interp_order == 2) {
+        /* three classes around the nearest, Lagrange weights */
+        const double c = floor(x + 0.5), t = x - c;
+        const double w[3] = {0.5 * t * (t - 1.0), (1.0 - t) * (1.0 + t), 0.5 * t * (t + 1.0)};
+        double* fs[3] = {f0, f1, f2};
+        for (int h = 0; h < 3; ++h) {
+          const double dzc = orc_exp((c + h - 1) * s->interp_lnr) * s->s.dr / gm;
+          orc_t_green_table(nx, nzz, s->s.dr, dzc, s->freq, s->beta, b->n,
+                            b->current * s->frac / b->mass, 0, t1);
+          orc_t_field(nx, nzz, s->bins, t1, fs[h]);
+        }
+        for (size_t i = 0; i < n2; ++i) s->fld2[i] = w[0] * f0[i] + w[1] * f1[i] + w[2] * f2[i];
+      } else {
+        const double c = floor(x), t = x - c;
+        for (int h = 0; h < 2; ++h) {
+          const double dzc = orc_exp((c + h) * s->interp_lnr) * s->s.dr / gm;
+          orc_t_green_table(nx, nzz, s->s.dr, dzc, s->freq, s->beta, b->n,
+                            b->current * s->frac / b->mass, 0, t1);
+          orc_t_field(nx, nzz, s->bins, t1, h ? f1 : f0);
+        }
+        for (size_t i = 0; i < n2; ++i) s->fld2[i] = (1.0 - t) * f0[i] + t * f1[i];
+      }
+      free(t1); free(f0); free(f1); free(f2);
+    }
+    if (s->fp32_table)
+      for (size_t i = 0; i < (size_t)(s->s.nz + 1) * (s->s.nx + 1) * 2; ++i)
+        s->fld2[i] = (double)(float)s->fld2[i];
     kick(s, b, r_length, 1.0, 1.0, 1.0, 1.0);
   } else {
     double rq = s->s.good_cnt_old ? (double)(good_cnt / s->s.good_cnt_old) : 0.0;

@@ -137,6 +137,11 @@ void orc_spch_set_remesh_threshold(orc_spch* s, double v);
 void orc_spch_set_adj_bunch(orc_spch* s, uint32_t v);
 void orc_spch_set_adj_bunch_cutoff_w(orc_spch* s, double v);
 void orc_spch_set_mesh_size_cutoff_w(orc_spch* s, double v);
+/* field table rounded to single precision (the reference's default build) */
+void orc_spch_set_fp32_table(orc_spch* s, int on);
+/* field interpolated in the log cell aspect between the classes (1+step)^c
+ * of the engine's grouped batch solve (FP64); 0 = the reference field */
+void orc_spch_set_aspect_interp(orc_spch* s, double step, int order);
 void orc_spch_set_mesh_size(orc_spch* s, uint32_t nr, uint32_t nz);
 void orc_spch_get_state(const orc_spch* s, orc_spch_state* st);
 /* Tables of the last remesh: fld1 (nr+1)*nr*nz double2 in reference layout

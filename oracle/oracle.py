@@ -136,6 +136,8 @@ def lib():
                   "orc_spch_set_adj_bunch_cutoff_w", "orc_spch_set_mesh_size_cutoff_w"):
             getattr(L, f).argtypes = [P, C.c_double]
         L.orc_spch_set_adj_bunch.argtypes = [P, C.c_uint32]
+        L.orc_spch_set_fp32_table.argtypes = [P, C.c_int]
+        L.orc_spch_set_aspect_interp.argtypes = [P, C.c_double, C.c_int]
         L.orc_spch_get_state.argtypes = [P, C.POINTER(SpchState)]
         L.orc_spch_get_tables.argtypes = [P, dp, dp, C.POINTER(C.c_int64)]
         L.orc_spch_start.restype = C.c_int
@@ -289,6 +291,11 @@ class SpaceCharge:
     def set_adj_bunch(self, v): lib().orc_spch_set_adj_bunch(self.h, int(v))
     def set_adj_bunch_cutoff_w(self, v): lib().orc_spch_set_adj_bunch_cutoff_w(self.h, v)
     def set_mesh_size_cutoff_w(self, v): lib().orc_spch_set_mesh_size_cutoff_w(self.h, v)
+
+    def set_fp32_table(self, on=True): lib().orc_spch_set_fp32_table(self.h, 1 if on else 0)
+
+    def set_aspect_interp(self, step, order=1):
+        lib().orc_spch_set_aspect_interp(self.h, step, order)
 
     def state(self):
         s = SpchState()

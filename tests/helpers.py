@@ -78,3 +78,6 @@ def compare_arrays(ga, oa, label=""):
             i = int(np.argmax(ga[k] != oa[k]))
             return f"{label}{k}: differ at {i}: gpu {ga[k][i]} oracle {oa[k][i]}"
     return None
+
+
+from hpsim_amd.metrics import beam_metrics, metric_errors  # noqa: E402,F401

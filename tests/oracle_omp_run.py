@@ -9,6 +9,8 @@ A SPEC.json names a member of the batched-scan configuration or a
 single-beam configuration with its own mesh:
     {"config": IDX, "num": N, "periods": P, "nr": NR, "nz": NZ}
     {"scan_member": K, "nbeams": B, "periods": P, "seed": S, "nr": NR, "nz": NZ}
+with optional "aspect_interp": [STEP, ORDER] (the field interpolated between
+the aspect classes of the engine's grouped batch solve, in FP64).
 
 Used by the tests as the checker; it touches no GPU.
 """
@@ -26,9 +28,14 @@ from oracle import oracle as O  # noqa: E402
 
 
 def _spec(path):
-    """(BeamSpec, element list, nr, nz, interval) of a JSON spec."""
+    """(BeamSpec, element list, nr, nz, interval, aspect interpolation) of a
+    JSON spec."""
     with open(path) as f:
         sp = json.load(f)
+    return _spec_parts(sp) + (sp.get("aspect_interp"),)
+
+
+def _spec_parts(sp):
     if "scan_member" in sp:
         bw = cfg.batched_scan(nbeams=sp["nbeams"], periods=sp["periods"], seed=sp.get("seed", 7))
         k = sp["scan_member"]
@@ -39,8 +46,9 @@ def _spec(path):
 
 
 def main():
+    interp = None
     if sys.argv[1].endswith(".json"):
-        b, elems, nr, nz, interval = _spec(sys.argv[1])
+        b, elems, nr, nz, interval, interp = _spec(sys.argv[1])
         out = sys.argv[2]
     else:
         idx, num, periods, out = int(sys.argv[1]), int(sys.argv[2]), int(sys.argv[3]), sys.argv[4]
@@ -55,6 +63,8 @@ def main():
     beam.set_frequency(b.freq)
     beam.set_ref_w(b.w0)
     sc = O.SpaceCharge(nr, nz, interval)
+    if interp:
+        sc.set_aspect_interp(*interp)
     O.Simulator(beam, elems, sc).simulate()
     st = sc.state()
     np.savez(out, threads=O.lib().orc_threads(), remesh_count=st["remesh_count"],

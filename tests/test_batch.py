@@ -185,3 +185,57 @@ def test_pv_scan_as_one_batch_matches_the_scan_loop(tmp_path):
         assert H.compare_arrays(got.arrays(), b.arrays(), f"value {v}: ") is None
     HPSim.set_db_epics(pv, str(v0), dbc, bl)
     dbc.close()
+
+
+def _run_twice(members, graph):
+    """save, simulate, restore, simulate: the first call tunes the field
+    launch shape (eager), the second replays the lattice periods as a HIP
+    graph when graph is on."""
+    from hpsim_amd import HPSim
+    objs = []
+    for spec, els in members:
+        b = HPSim.Beam(mass=spec.mass, charge=spec.charge, current=spec.current, num=spec.num)
+        b.set_distribution(*cfg.make_distribution(spec))
+        b.set_frequency(spec.freq)
+        b.set_ref_w(spec.w0)
+        b.save_initial_beam()
+        s = HPSim.SpaceCharge(nr=32, nz=128, interval=0.05)
+        objs.append((b, s, HPSim.Simulator(b, HPSim.BeamLine(elements=els), s)))
+    batch = HPSim.Batch([m for _, _, m in objs])
+    batch.set_graph(graph)
+    batch.simulate()
+    for b, _, _ in objs:
+        b.restore_initial_beam()
+    batch.simulate()
+    return objs, batch
+
+
+def test_batch_period_graph_replays_are_bit_exact():
+    """Four members on six periods: the second simulate() replays one lattice
+    period's launches as a HIP graph (checked by its replay count) and ends
+    bit-identical to the eager launches and to the oracle doing the same."""
+    from oracle import oracle as O
+    _, members = _members([1500, 4096, 6000, 2222], periods=6)
+    g_objs, g_batch = _run_twice(members, graph=True)
+    e_objs, e_batch = _run_twice(members, graph=False)
+    replays, per = g_batch.graph_info()
+    assert replays >= 4 and per >= 6, (replays, per)
+    assert e_batch.graph_info() == (0, 0)
+    for k, ((gb, gs, _), (eb, es, _)) in enumerate(zip(g_objs, e_objs)):
+        err = H.compare_arrays(gb.arrays(), eb.arrays(), f"member {k}, graph vs eager: ")
+        assert err is None, err
+        assert gs.state() == es.state()
+    ref = []
+    for spec, els in members:
+        b = O.Beam(spec.num, spec.mass, spec.charge, spec.current)
+        b.set_distribution(*cfg.make_distribution(spec))
+        b.set_frequency(spec.freq)
+        b.set_ref_w(spec.w0)
+        b.save_initial_beam()
+        s = O.SpaceCharge(32, 128, 0.05)
+        m = O.Simulator(b, els, s)
+        m.simulate()
+        b.restore_initial_beam()
+        m.simulate()
+        ref.append((b, s))
+    _check(g_objs, ref)

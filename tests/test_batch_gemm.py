@@ -1,0 +1,197 @@
+"""The grouped field solve of a batch (field mode 'gemm'): Green's tables
+normalised to dr = 1 on an aspect grid, each beam's field interpolated
+quadratically between its three nearest classes on its own mesh, all fields
+of a kick point as one FP32 matrix-core product.
+
+* the field table of every member equals the oracle's FP64 field of the same
+  histogram on the same mesh within the interpolation error, for every
+  launch shape of the product, and the mesh is the reference's;
+* short tracking follows the oracle closely (the implementation check);
+* at the batched-scan configuration's own sizes and lattice (40 periods,
+  members up to 1,048,576 particles) the transverse rms emittances,
+  transmission and output energy stay within 1 %, 0.1 % and 0.1 % of the
+  oracle (the accuracy of the method)."""
+import json
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from hpsim_amd import configs as cfg
+from tests import helpers as H
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+pytestmark = pytest.mark.gpu
+STEP = 0.05
+
+
+def _omp(tmp_path, spec, name, timeout=900):
+    sp = tmp_path / f"{name}.json"
+    out = tmp_path / f"{name}.npz"
+    sp.write_text(json.dumps(spec))
+    threads = max(4, min(16, os.cpu_count() or 4))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    subprocess.run([sys.executable, os.path.join(HERE, "oracle_omp_run.py"), str(sp), str(out)],
+                   check=True, timeout=timeout, env=env)
+    return dict(np.load(out))
+
+
+def _members(bw, pick):
+    from hpsim_amd import HPSim
+    objs = []
+    for k in pick:
+        spec = bw.beams[k]
+        b = HPSim.Beam(mass=spec.mass, charge=spec.charge, current=spec.current, num=spec.num)
+        b.set_distribution(*cfg.make_distribution(spec))
+        b.set_frequency(spec.freq)
+        b.set_ref_w(spec.w0)
+        s = HPSim.SpaceCharge(nr=bw.nr, nz=bw.nz, interval=bw.interval)
+        objs.append((b, s, HPSim.Simulator(b, HPSim.BeamLine(elements=bw.lattices[k]), s)))
+    return objs
+
+
+def _gemm_batch(objs, shape=None):
+    from hpsim_amd import HPSim
+    bt = HPSim.Batch([m for _, _, m in objs])
+    bt.set_field_mode("gemm", STEP)
+    if shape is not None:
+        bt.set_gemm_shape(shape)
+    return bt
+
+
+def _fp64_field(O, st, bins, num, spec, nr, nz):
+    """The oracle's FP64 Green's table and field sum on the member's mesh."""
+    import ctypes as C
+    L = O.lib()
+    beta = math.sqrt(1.0 - 1.0 / (st["gm_old"] * st["gm_old"]))
+    f1 = np.empty((nr + 1) * nr * nz * 2)
+    f2 = np.empty((nz + 1) * (nr + 1) * 2)
+    dp = C.POINTER(C.c_double)
+    L.orc_t_green_table(nr, nz, st["dr"], st["dz"], spec.freq, beta, num,
+                        spec.current * st["fraction"] / spec.mass, 0, f1.ctypes.data_as(dp))
+    L.orc_t_field(nr, nz, bins.ctypes.data_as(C.POINTER(C.c_int64)), f1.ctypes.data_as(dp),
+                  f2.ctypes.data_as(dp))
+    return f2
+
+
+@pytest.mark.parametrize("shape", list(range(8)))
+def test_gemm_field_equals_fp64_field_on_the_reference_mesh(shape, oracle_lib):
+    """First kick point of nine scan members (one remesh each): the grouped
+    product's field table against the oracle's FP64 table x charge sum of the
+    same histogram on the same mesh, which is the FP64 path's mesh."""
+    O = oracle_lib
+    bw = cfg.batched_scan(nbeams=256, periods=1)
+    nums = np.array([b.num for b in bw.beams])
+    order = np.argsort(nums)
+    pick = [int(order[i]) for i in (0, 1, 40, 90, 128, 170, 220, 250, 255)]
+    print(f"shape {shape}: members", [int(nums[k]) for k in pick], flush=True)
+    objs = _members(bw, pick)
+    bt = _gemm_batch(objs, shape)
+    bt.simulate(0, 0)
+    info = bt.gemm_info()
+    assert info["shape"] == shape and info["columns_last"] == 3 * len(pick)
+    ref_objs = _members(bw, pick)
+    from hpsim_amd import HPSim
+    HPSim.Batch([m for _, _, m in ref_objs]).simulate(0, 0)
+    worst = 0.0
+    for k, (b, s, _), (_, rs, _) in zip(pick, objs, ref_objs):
+        st = s.state()
+        assert st == rs.state(), k                     # the mesh is the reference's
+        _, f2, bins = s.tables()
+        ref = _fp64_field(O, st, bins, int(nums[k]), bw.beams[k], bw.nr, bw.nz)
+        err = np.linalg.norm(f2 - ref) / np.linalg.norm(ref)
+        worst = max(worst, err)
+        assert err < 5e-5, (k, err)
+    print(f"shape {shape}: worst relative L2 field error {worst:.2e}", flush=True)
+
+
+def test_gemm_tracking_follows_the_oracle(tmp_path):
+    """Eight scan members, two periods: the grouped solve against the FP64
+    oracle -- the two differ by the interpolated FP32 field only."""
+    bw = cfg.batched_scan(nbeams=256, periods=2)
+    nums = np.array([b.num for b in bw.beams])
+    order = np.argsort(nums)
+    pick = [int(order[i]) for i in (0, 30, 80, 128, 160, 200, 240, 255)]
+    objs = _members(bw, pick)
+    bt = _gemm_batch(objs)
+    bt.simulate()
+    for k, (b, s, _) in zip(pick, objs):
+        ref = _omp(tmp_path, {"scan_member": k, "nbeams": 256, "periods": 2}, f"m{k}")
+        g = b.arrays()
+        st = s.state()
+        assert st["remesh_count"] == ref["remesh_count"].item(), k
+        # a particle next to a discontinuity of the maps (mesh edge, bin
+        # boundary, aperture) may take the other branch after a small field
+        # difference, so the check is on the bulk and on the beam moments
+        assert ((g["loss"] != 0) != (ref["loss"] != 0)).mean() < 1e-4, k
+        for c in H.COORDS:
+            scale = max(np.abs(ref[c] - ref[c].mean()).max(), 1e-300)
+            d = np.abs(g[c] - ref[c]) / scale
+            print(f"member {k} {c}: median {np.median(d):.1e} 99.9% {np.quantile(d, 0.999):.1e} "
+                  f"max {d.max():.1e}", flush=True)
+            assert np.quantile(d, 0.999) < 1e-4, (k, c)
+        e = H.metric_errors(H.beam_metrics(g, bw.beams[k].mass),
+                            H.beam_metrics(ref, bw.beams[k].mass))
+        print(f"member {k}: {e}", flush=True)
+        assert max(e["ex"], e["ey"], e["w_avg"], e["transmission"]) < 1e-4, (k, e)
+
+
+def test_gemm_follows_the_oracle_with_the_same_interpolation(tmp_path):
+    """Six scan members over the whole 40-period lattice against the FP64
+    oracle that interpolates its own FP64 Green's tables between the same
+    aspect classes: what remains is the FP32 product, so the members' output
+    figures agree far inside the method's error to the reference."""
+    bw = cfg.batched_scan(nbeams=256, periods=40)
+    nums = np.array([b.num for b in bw.beams])
+    order = np.argsort(nums)
+    pick = [int(order[i]) for i in (0, 60, 120, 180, 240, 255)]
+    objs = _members(bw, pick)
+    bt = _gemm_batch(objs)
+    bt.simulate()
+    worst = {}
+    for k, (b, s, _) in zip(pick, objs):
+        ref = _omp(tmp_path, {"scan_member": k, "nbeams": 256, "periods": 40,
+                              "aspect_interp": [STEP, 2]}, f"i{k}")
+        e = H.metric_errors(H.beam_metrics(b.arrays(), bw.beams[k].mass),
+                            H.beam_metrics(ref, bw.beams[k].mass))
+        print(f"member {k} ({nums[k]} particles) vs interpolating oracle: " +
+              ", ".join(f"{n} {v:.2e}" for n, v in e.items()), flush=True)
+        for n, v in e.items():
+            worst[n] = max(worst.get(n, 0.0), v)
+    assert max(worst["ex"], worst["ey"]) < 1e-3 and worst["transmission"] < 1e-4, worst
+
+
+def test_gemm_scan_members_within_error_bounds_at_configured_sizes(tmp_path):
+    """Sixteen members of the 256-beam scan (the three largest, the smallest,
+    twelve spread over the size range) on the whole 40-period lattice in one
+    grouped batch, against the FP64 oracle: transverse rms emittances within
+    1 %, transmission within 0.1 %, output energy within 0.1 %."""
+    bw = cfg.batched_scan(nbeams=256, periods=40)
+    nums = np.array([b.num for b in bw.beams])
+    order = np.argsort(nums)
+    pick = [int(order[i]) for i in (255, 254, 253, 0)] + [int(order[i]) for i in
+                                                          range(10, 250, 20)]
+    objs = _members(bw, pick)
+    bt = _gemm_batch(objs)
+    bt.simulate()
+    info = bt.gemm_info()
+    print("gemm", info, flush=True)
+    rows = []
+    for k, (b, s, _) in zip(pick, objs):
+        ref = _omp(tmp_path, {"scan_member": k, "nbeams": 256, "periods": 40}, f"r{k}")
+        e = H.metric_errors(H.beam_metrics(b.arrays(), bw.beams[k].mass),
+                            H.beam_metrics(ref, bw.beams[k].mass))
+        rows.append((k, int(nums[k]), e))
+        print(f"member {k} ({nums[k]} particles): " +
+              ", ".join(f"{n} {v:.2e}" for n, v in e.items()), flush=True)
+    for k, n, e in rows:
+        assert max(e["ex"], e["ey"]) <= 0.01, (k, n, e)
+        assert e["transmission"] <= 0.001, (k, n, e)
+        assert e["w_avg"] <= 0.001, (k, n, e)
+    # the longitudinal rms emittance counts every transversely good particle,
+    # longitudinally lost ones included (beam.cpp:985-994), whose phases lie
+    # far out; it is stated, and bounded in the median over the members
+    assert np.median([e["ez"] for _, _, e in rows]) <= 0.01
