@@ -15,6 +15,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 _LIB_OMP = os.path.join(_HERE, "liboracle_omp.so")
+# HPSIM_ORACLE_SANITIZED=1 loads the ASan/UBSan build instead of the serial
+# library (the process must have the ASan runtime preloaded)
+_LIB_SAN = os.path.join(_HERE, "liboracle_san.so")
 _use_omp = False
 
 EL_TYPES = {"Drift": 1, "Quad": 2, "RFGap-DTL": 3, "RFGap-CCL": 4, "ApertureC": 5,
@@ -90,7 +93,13 @@ def lib():
     global _lib
     if _lib is None:
         path = _LIB_OMP if _use_omp else _LIB
-        if not os.path.exists(path):
+        if os.environ.get("HPSIM_ORACLE_SANITIZED") == "1":
+            if _use_omp:
+                raise RuntimeError("the sanitized oracle build is serial only")
+            path = _LIB_SAN
+            if not os.path.exists(path):
+                subprocess.run(["make", "-s", "-C", _HERE, "san"], check=True)
+        elif not os.path.exists(path):
             build()
         L = C.CDLL(path)
         P = C.c_void_p
