@@ -84,8 +84,9 @@ def parse():
                         "single beam; small: the 65,536-particle single beam on 40 periods")
     p.add_argument("--field", default=None, choices=("fp64", "gemm", "surrogate"),
                    help="batched field solve: fp64 (per-beam FP64 tables, bit-exact), gemm "
-                        "(grouped FP32 matrix-core product on aspect-grid meshes) or surrogate "
-                        "(the trained network, audited by gemm); default surrogate")
+                        "(grouped FP32 matrix-core product, Green's tables shared over aspect "
+                        "classes) or surrogate (the trained network, audited by gemm); default "
+                        "surrogate")
     p.add_argument("--beams", type=int, default=None, help="beams of a batched workload")
     p.add_argument("--slice-periods", type=int, default=None,
                    help="lattice periods per step of a batched workload (stress: 40, scan: all)")

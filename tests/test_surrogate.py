@@ -116,7 +116,7 @@ def test_gpu_network_matches_reference_forward_and_gradient():
             assert e < 2e-2, (r, e)
 
 
-def _train_small(steps=1500):
+def _train_small(steps=20000):
     """A small network trained on the grouped SCHEFF solves of two short
     tracking runs (scan members and beams of the accelerating lattice)."""
     from hpsim_amd import configs as cfg
@@ -131,7 +131,7 @@ def _train_small(steps=1500):
     hold = n // 10
     net.fit_norm(0, n - hold)
     before = net.eval(n - hold, hold)
-    losses = net.train(steps, batch=32, lr=5e-4, lr_end=5e-6, seed=1, first=0, count=n - hold)
+    losses = net.train(steps, batch=32, lr=1e-3, lr_end=1e-5, seed=1, first=0, count=n - hold)
     after = net.eval(n - hold, hold)
     return net, n, before, after, losses
 
