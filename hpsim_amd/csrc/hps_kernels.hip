@@ -617,6 +617,12 @@ constexpr uint32_t kWideBlock = kChunk;
 // at most this many chunks (k_segment would run at most two wavefronts per
 // SIMD; measured faster up to 512 chunks, slower at 1024)
 constexpr uint32_t kWideMaxChunks = 512;
+// Batches of at least this many chunks run the phase pass one chunk per
+// wavefront (MI355X, 262,144 chunks of the 64-beam stress batch: phase pass
+// 189 -> 164 ms per step); a lone beam of 1,024 chunks in that form has one
+// wave per SIMD and runs slower (7.3 -> 13.5 ms per step), so lone beams
+// keep a workgroup per chunk.
+constexpr uint32_t kPhaseWaveMinChunks = 8192;
 struct WideSmem {
   double q[kScQ][kWideBlock];   // contributions, then lane sums in [q][0, 256)
   double mx[kWideBlock];        // phi of particles not longitudinally flagged, else -inf
@@ -936,6 +942,75 @@ __global__ __launch_bounds__(kBlock) void k_phase_batch(const PhaseArgs* recs, c
   const uint32_t e = map[blockIdx.x];
   const PhaseArgs a = period_recs(recs, roff)[e >> kBatchLocalBits];
   phase_body(a, e & kBatchLocalMask);
+}
+
+// The same sums with one chunk per wavefront (batches of at least
+// kPhaseWaveMinChunks chunks): lane l stands for the canonical
+// lanes l, l + 64, l + 128 and l + 192 of phase_body, sums each one's four
+// particles in its order and folds them as levels 128 and 64 of
+// block_reduce's tree, then the same shuffle levels: the same bits, without
+// LDS or barriers, and all sixteen particles' loads issued before the first
+// relative phase.
+__device__ __forceinline__ void phase_wave_body(const PhaseArgs& a, uint32_t c, int l)
+{
+  const BeamArrays& b = a.b;
+  const Partials& pr = a.pr;
+  const double ref = a.st->phi_avg_good;
+  double ph[4][kPerLane];
+  uint32_t lo[4][kPerLane], ll[4][kPerLane];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int j = 0; j < (int)kPerLane; ++j) {
+      const uint32_t i = c * kChunk + j * kBlock + q * 64 + l;
+      const bool in = i < b.n;
+      ph[q][j] = in ? gld(b.phi, i) : 0.0;
+      lo[q][j] = in ? gld(b.loss, i) : 1u;
+      ll[q][j] = in ? gld(b.lloss, i) : 0u;
+    }
+  double v[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < (int)kPerLane; ++j) {
+      const uint32_t i = c * kChunk + j * kBlock + q * 64 + l;
+      const double pr_ = i < b.n ? relative_phi(ph[q][j], ref) : 0.0;
+      const bool tg = lo[q][j] == 0;
+      const bool g = is_good(lo[q][j], ll[q][j]);
+      acc[0] = acc[0] + (tg ? pr_ : 0.0);
+      const double vg = g ? pr_ : 0.0;
+      acc[1] = acc[1] + vg;
+      acc[2] = acc[2] + vg * vg;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[q][k] = acc[k];
+  }
+  double s[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) s[k] = (v[0][k] + v[2][k]) + (v[1][k] + v[3][k]);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[k] = s[k] + __shfl_down(s[k], o, 64);
+  }
+  if (l == 0) {
+    gst(pr.sums, c, s[0]);
+    gst(pr.sums, pr.mpad + c, s[1]);
+    gst(pr.sums, 2 * pr.mpad + c, s[2]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_phase_wave_batch(const PhaseArgs* recs,
+                                                             const uint32_t* map, uint32_t nch,
+                                                             const uint64_t* roff)
+{
+  const uint32_t k = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) +
+                                                    (threadIdx.x >> 6));
+  if (k >= nch) return;
+  const uint32_t e = map[k];
+  const PhaseArgs a = period_recs(recs, roff)[e >> kBatchLocalBits];
+  phase_wave_body(a, e & kBatchLocalMask, threadIdx.x & 63);
 }
 
 // One workgroup after k_phase: the phi_r moments, then the Scheff::Start
@@ -1675,7 +1750,15 @@ hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s)
 hipError_t launch_phase_batch(const PhaseArgs* recs, const uint32_t* map, uint32_t nblocks,
                               const uint64_t* roff, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_phase_batch, dim3(nblocks), dim3(kBlock), 0, s, recs, map, roff);
+  // nblocks: one per chunk of the batch.  Batches of at least
+  // kPhaseWaveMinChunks chunks run a wavefront per chunk (four chunks per
+  // workgroup); smaller ones a workgroup per chunk, which keeps more waves in
+  // flight when there are few chunks.
+  if (nblocks >= kPhaseWaveMinChunks)
+    hipLaunchKernelGGL(k_phase_wave_batch, dim3((nblocks + kBlock / 64 - 1) / (kBlock / 64)),
+                       dim3(kBlock), 0, s, recs, map, nblocks, roff);
+  else
+    hipLaunchKernelGGL(k_phase_batch, dim3(nblocks), dim3(kBlock), 0, s, recs, map, roff);
   return hipGetLastError();
 }
 

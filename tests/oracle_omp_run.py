@@ -9,6 +9,7 @@ A SPEC.json names a member of the batched-scan configuration or a
 single-beam configuration with its own mesh:
     {"config": IDX, "num": N, "periods": P, "nr": NR, "nz": NZ}
     {"scan_member": K, "nbeams": B, "periods": P, "seed": S, "nr": NR, "nz": NZ}
+    {"stress_member": K, "nbeams": B, "num": N, "periods": P, "nr": NR, "nz": NZ}
 with optional "aspect_interp": [STEP, ORDER] (the field interpolated between
 the aspect classes of the engine's grouped batch solve, in FP64).
 
@@ -36,6 +37,10 @@ def _spec(path):
 
 
 def _spec_parts(sp):
+    if "stress_member" in sp:
+        bw = cfg.combined_stress(nbeams=sp["nbeams"], num=sp["num"], periods=sp["periods"])
+        k = sp["stress_member"]
+        return bw.beams[k], bw.lattices[k], sp.get("nr", bw.nr), sp.get("nz", bw.nz), bw.interval
     if "scan_member" in sp:
         bw = cfg.batched_scan(nbeams=sp["nbeams"], periods=sp["periods"], seed=sp.get("seed", 7))
         k = sp["scan_member"]

@@ -6,7 +6,10 @@
   particles -- the size from which a batch member's rebuild grid has its
   deposit workgroups fill the Green's table -- run as one batch;
 * one 4,194,304-particle beam on the 64 x 256 mesh, the member shape of the
-  combined-stress configuration.
+  combined-stress configuration;
+* three members of the combined-stress configuration as one batch (12,288
+  chunks: the batch form of the phase pass that runs one chunk per
+  wavefront).
 
 Each is compared bit for bit with the OpenMP oracle (itself pinned to the
 serial oracle in test_full_size.py), run in a process of its own by
@@ -95,3 +98,29 @@ def test_stress_member_4194304_on_64x256_bit_exact(tmp_path):
     err = H.compare_arrays(gb.arrays(), ref, "4194304 on 64x256: ")
     assert err is None, err
     _check_state("4194304 on 64x256", gsc.state(), ref)
+
+
+def test_stress_members_as_one_batch_bit_exact(tmp_path):
+    """The first three beams of the combined-stress configuration (4,194,304
+    particles each, 64 x 256 mesh) on one period as one batch: 12,288
+    chunks, so the phase pass runs one chunk per wavefront."""
+    from hpsim_amd import HPSim
+    nb, periods = 3, 1
+    bw = cfg.combined_stress(nbeams=nb, num=4194304, periods=periods)
+    assert nb * (bw.beams[0].num // 1024) >= 8192
+    objs = []
+    for spec, lat in zip(bw.beams, bw.lattices):
+        b = HPSim.Beam(mass=spec.mass, charge=spec.charge, current=spec.current, num=spec.num)
+        b.set_distribution(*cfg.make_distribution(spec))
+        b.set_frequency(spec.freq)
+        b.set_ref_w(spec.w0)
+        s = HPSim.SpaceCharge(nr=bw.nr, nz=bw.nz, interval=bw.interval)
+        objs.append((b, s, HPSim.Simulator(b, HPSim.BeamLine(elements=lat), s)))
+    HPSim.Batch([m for _, _, m in objs]).simulate()
+    for k in (0, nb - 1):
+        b, s, _ = objs[k]
+        ref = _omp(tmp_path, {"stress_member": k, "nbeams": nb, "num": 4194304,
+                              "periods": periods}, f"stress{k}")
+        err = H.compare_arrays(b.arrays(), ref, f"stress member {k}: ")
+        assert err is None, err
+        _check_state(f"stress member {k}", s.state(), ref)
