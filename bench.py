@@ -4,20 +4,26 @@
 Default workload: the combined stress configuration -- 64 beams of
 4,194,304 particles (drawn on the device), the 400-period Drift/Quad/Drift/
 DTL-gap lattice, SCHEFF on a 64x256 mesh after every element, all beams in
-one batch whose fields come from the learned surrogate (audited against the
-exact grouped SCHEFF product every 20 kick points, fallback above 2 %
-relative field error; grouped product on the FP32 matrix cores when it
-runs).  One step = one simulate() over the next 40 lattice periods (160
-elements) of every beam; the beams go on from where the previous step ended
-and start again at the lattice entrance after the last period.  value =
-particles x elements x steps / wall time of the timed steps.
+one batch whose fields come from the grouped SCHEFF solve on the FP32
+matrix cores (Green's tables shared over aspect classes, the product of
+every remeshing beam's charges in one launch sequence; its launch shape,
+direct or in the frequency domain along z, chosen by timing).  One step =
+one simulate() over the next 40 lattice periods (160 elements) of every
+beam; the beams go on from where the previous step ended and start again
+at the lattice entrance after the last period.  value = particles x
+elements x steps / wall time of the timed steps.  The same run times the
+learned-surrogate form (network fields audited against the grouped solve
+every 20 kick points, fallback above 2 % relative field error) and reports
+it under field_surrogate with its own accuracy.
 
 The JSON line also carries
   roofline:      HBM roofline of the batched element-map kernel
                  (k_segment_batch), algorithmic bytes per launch / average
                  launch time from HIP events around every launch of one extra
                  step; roofline_gemm: the grouped field product's FP32 matrix-
-                 core rate (2 M K flops per simulation in a product);
+                 core rate (its algorithmic flops per product column: 2 M K for
+                 the direct product, the per-frequency product and transforms
+                 for the spectral one);
   accuracy:      rms emittance, transmission and mean output energy of a few
                  beams of the workload tracked through the whole lattice with
                  the benchmarked field mode, against the same beams with the
@@ -32,6 +38,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -45,6 +52,8 @@ GEMM_ASPECT_STEP = 0.05        # aspect grid of the grouped field solve
 SURROGATE_AUDIT_EVERY = 20     # kick points between audits of the surrogate
 SURROGATE_MAX_ERR = 0.02       # relative L2 field error that triggers the fallback
 SEG_BYTES_PER_PARTICLE = 104   # k_segment: reads 6 FP64 + 2 u32, writes 6 FP64
+SPECTRAL_SHAPE = 8             # product shape that runs in the frequency domain along z
+DEFAULT_FIELD = "gemm"         # field form of the batched workloads' headline figure
 # HBM bytes per k_segment launch at 1,048,576 particles, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE counter runs (calibrated on a plain device copy)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_segment_traffic_plain-fp64-1048576x200.md")
@@ -82,6 +91,9 @@ def parse():
                         "one batch (default); batched-scan: 256 beams of 16K-1M particles with "
                         "per-beam lattice settings in one batch; plain: the 1,048,576-particle "
                         "single beam; small: the 65,536-particle single beam on 40 periods")
+    p.add_argument("--compare-field", default=None, choices=("fp64", "gemm", "surrogate", "none"),
+                   help="second field form timed in the same run (default: the other of gemm "
+                        "and surrogate)")
     p.add_argument("--field", default=None, choices=("fp64", "gemm", "surrogate"),
                    help="batched field solve: fp64 (per-beam FP64 tables, bit-exact), gemm "
                         "(grouped FP32 matrix-core product, Green's tables shared over aspect "
@@ -269,19 +281,70 @@ def _plain_rates():
     return out
 
 
-def main_batched(args):
+def _load_net(w):
     from hpsim_amd import surrogate as S
+    path = S.weights_path(w.nr, w.nz)
+    if not os.path.exists(path):
+        sys.exit(f"bench.py: no trained surrogate for the {w.nr}x{w.nz} mesh ({path}); "
+                 f"train it with python -m hpsim_amd.surrogate --nr {w.nr} --nz {w.nz}, or "
+                 "pass --field gemm")
+    return S.load(path, max_batch=64)
 
-    w = _batch_workload(args)
-    mode = args.field or "surrogate"
-    net = None
+
+def _timed_batch(w, mode, net, sel, steps, warmup, eager, profile):
+    """All beams of the workload in one batch with the given field mode:
+    warmup + timed steps (each the next `sel` elements of the lattice, back
+    to the entrance after the last), then, with profile, one more step with
+    HIP events around every launch.  Returns the figures of the run."""
+    total = sum(b.num for b in w.beams)
+    nel = len(w.lattices[0])
+    _log(f"{len(w.beams)} beams, {total} particles, {nel} elements, field {mode}: "
+         "drawing beams on the device")
+    batch, sims = _make_batch(w, range(len(w.beams)), mode, net)
+    batch.set_graph(not eager)
+    nslice = nel // sel
+    pos = [0]
+
+    def step():
+        if pos[0] == 0:
+            for m in sims:
+                m.beam.restore_initial_beam()
+        batch.simulate(pos[0] * sel, pos[0] * sel + sel - 1)
+        pos[0] = (pos[0] + 1) % nslice
+
+    for k in range(warmup):
+        step()
+        _log(f"{mode}: warm-up step {k + 1} done")
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step()
+        _log(f"{mode}: timed step {k + 1} done")
+    dt = time.perf_counter() - t0
+    r = {"value": total * sel * steps / dt, "ms_per_step": dt / steps * 1e3, "steps": steps,
+         "warmup": warmup}
+    r["graph"] = batch.graph_info()
+    if profile:
+        gi0 = batch.gemm_info() if mode != "fp64" else None
+        batch.set_profiling(True)
+        step()
+        r["prof"] = batch.profile()
+        batch.set_profiling(False)
+        r["gemm"] = (gi0, batch.gemm_info()) if mode != "fp64" else None
+    r["transmitted"] = sum(m.beam.get_size() - m.beam.get_loss_num("t") for m in sims)
     if mode == "surrogate":
-        path = S.weights_path(w.nr, w.nz)
-        if not os.path.exists(path):
-            sys.exit(f"bench.py: no trained surrogate for the {w.nr}x{w.nz} mesh ({path}); "
-                     f"train it with python -m hpsim_amd.surrogate --nr {w.nr} --nz {w.nz}, or "
-                     "pass --field gemm")
-        net = S.load(path, max_batch=64)
+        from hpsim_amd import surrogate as S
+        r["surrogate_counters"] = S.info(batch)
+    del batch, sims
+    return r
+
+
+def main_batched(args):
+    w = _batch_workload(args)
+    mode = args.field or DEFAULT_FIELD
+    other = args.compare_field
+    if other is None:
+        other = "surrogate" if mode == "gemm" else ("gemm" if mode == "surrogate" else "none")
+    net = _load_net(w) if "surrogate" in (mode, other) else None
     total = sum(b.num for b in w.beams)
     nel = len(w.lattices[0])
     slice_periods = args.slice_periods or (40 if args.workload == "stress" else nel // 4)
@@ -289,40 +352,15 @@ def main_batched(args):
     if nel % sel:
         sys.exit(f"bench.py: {nel} elements are not a whole number of {slice_periods}-period "
                  "slices")
-    _log(f"{len(w.beams)} beams, {total} particles, {nel} elements, field {mode}: "
-         "drawing beams on the device")
-    batch, sims = _make_batch(w, range(len(w.beams)), mode, net)
-    batch.set_graph(not args.eager)
-    nslice = nel // sel
-    pos = [0]
-
-    def step():
-        # the next slice of the lattice; back to the entrance after the last
-        if pos[0] == 0:
-            for m in sims:
-                m.beam.restore_initial_beam()
-        batch.simulate(pos[0] * sel, pos[0] * sel + sel - 1)
-        pos[0] = (pos[0] + 1) % nslice
-
-    for k in range(args.warmup):
-        step()
-        _log(f"warm-up step {k + 1} done")
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step()
-        _log(f"timed step {k + 1} done")
-    dt = time.perf_counter() - t0
-    replays, per_period = batch.graph_info()
-    gi0 = batch.gemm_info() if mode != "fp64" else None
-    batch.set_profiling(True)
-    step()
-    prof = batch.profile()
-    batch.set_profiling(False)
-    gi1 = batch.gemm_info() if mode != "fp64" else None
+    run = _timed_batch(w, mode, net, sel, args.steps, args.warmup, args.eager, True)
+    dt = run["ms_per_step"] * args.steps / 1e3
+    replays, per_period = run["graph"]
+    prof = run["prof"]
+    gi0, gi1 = run["gemm"] or (None, None)
     seg_ms, seg_n = prof["segment"]
     avg_s = seg_ms / 1e3 / max(seg_n, 1)
     achieved = SEG_BYTES_PER_PARTICLE * total / avg_s / 1e9
-    transmitted = sum(m.beam.get_size() - m.beam.get_loss_num("t") for m in sims)
+    transmitted = run["transmitted"]
     out = {
         "metric": "particle-element pushes/s incl. space charge",
         "value": total * sel * args.steps / dt,
@@ -356,19 +394,47 @@ def main_batched(args):
     if mode != "fp64":
         prod_ms, prod_n = prof["field_product"]
         cols = gi1["product_columns"] - gi0["product_columns"]
-        flops = 2.0 * (2 * (w.nr + 1) * (w.nz + 1)) * (w.nr * w.nz) * cols
+        direct = 2.0 * (2 * (w.nr + 1) * (w.nz + 1)) * (w.nr * w.nz) * cols
+        spectral = gi1["shape"] == SPECTRAL_SHAPE
+        if spectral:
+            # per column: the per-frequency complex product as a real product of
+            # 2 R x 2 nr (R = 2 (nr + 1) field rows) at nz + 1 frequencies, plus the
+            # radix-2 transforms (nr / 2 forward, nr + 1 inverse, 5 L log2 L each)
+            ln = 2 * w.nz
+            fft = 5.0 * ln * math.log2(ln) * (w.nr // 2 + w.nr + 1)
+            flops = (2.0 * (4 * (w.nr + 1)) * (2 * w.nr) * (w.nz + 1) + fft) * cols
+        else:
+            flops = direct
+        rate = flops / (prod_ms / 1e3) / 1e12 if prod_ms > 0 else None
         out["roofline_gemm"] = {
-            "bound": "mfma", "kernel": "k_gemm_field (FP32 MFMA 16x16x4)",
-            "achieved": flops / (prod_ms / 1e3) / 1e12 if prod_ms > 0 else None,
-            "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": (flops / (prod_ms / 1e3) / 1e12 / FP32_MFMA_PEAK_TFLOPS
-                     if prod_ms > 0 else None),
+            "bound": "mfma",
+            "kernel": ("k_fft_product (per-frequency product on FP32 MFMA 16x16x4) with the "
+                       "k_fft_* transforms" if spectral else "k_gemm_field (FP32 MFMA 16x16x4)"),
+            "achieved": rate, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": rate / FP32_MFMA_PEAK_TFLOPS if rate else None,
+            "flops_per_column": flops / max(cols, 1),
+            "direct_product_flops_per_column": direct / max(cols, 1),
+            "direct_equivalent_tflops": direct / (prod_ms / 1e3) / 1e12 if prod_ms > 0 else None,
             "launches": prod_n, "product_columns": cols,
             "shape": gi1["shape"], "shape_candidate_ms": gi1["times_ms"],
             "class_tables_built": gi1["tables_built"]}
+    if "surrogate_counters" in run:
+        out["surrogate_counters"] = run["surrogate_counters"]
     if args.validate_beams > 0:
         _log("accuracy run")
         out["accuracy"] = _accuracy(w, mode, net, args.validate_beams)
+    if other != "none":
+        # the other field form on the same box: a shorter timed run and its
+        # accuracy against the same FP64 reference
+        o = _timed_batch(w, other, net, sel, min(args.steps, 3), 1, args.eager, False)
+        cmp = {"value": o["value"], "ms_per_step": o["ms_per_step"], "steps": o["steps"],
+               "warmup": o["warmup"], "ratio_to_value": o["value"] / out["value"]}
+        if "surrogate_counters" in o:
+            cmp["surrogate_counters"] = o["surrogate_counters"]
+        if args.validate_beams > 0:
+            _log(f"accuracy run ({other})")
+            cmp["accuracy"] = _accuracy(w, other, net, args.validate_beams)
+        out["field_" + other] = cmp
     if not args.no_plain:
         _log("plain FP64 single-beam runs")
         plain = _plain_rates()
@@ -389,7 +455,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent beams "
-              "shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
+              "would shard across GPUs by beam (one batch per GPU, no communication)",
+              file=sys.stderr)
         sys.exit(2)
     if args.workload in ("batched-scan", "stress"):
         return main_batched(args)

@@ -25,8 +25,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libhpsim_amd.so")
-SOURCES = ["hps_kernels.hip", "hps_engine.hip", "hps_field_gemm.hip", "hps_unet.hip",
-           "hps_surrogate.hip"]
+SOURCES = ["hps_kernels.hip", "hps_engine.hip", "hps_field_gemm.hip", "hps_field_fft.hip",
+           "hps_unet.hip", "hps_surrogate.hip"]
 EXTRA_FLAGS = {"hps_kernels.hip": ["-mllvm", "-disable-machine-licm"]}
 ARCH = os.environ.get("HPSIM_OFFLOAD_ARCH", "gfx950")
 

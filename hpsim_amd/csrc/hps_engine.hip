@@ -1589,6 +1589,7 @@ struct hps_batch {
   SurCtl sur_cfg{};
   SurCtl* d_sur = nullptr;
   float *d_xinf = nullptr, *d_uinf = nullptr;
+  FftBufs fft{};                       // spectral product buffers (tw null: unavailable)
   uint32_t gemm_shape = UINT32_MAX;    // chosen product shape (UINT32_MAX: tune)
   uint32_t gemm_shape_key[3] = {0, 0, 0};
   std::vector<float> gemm_times;
@@ -1655,6 +1656,9 @@ static void free_gemm(hps_batch* bt)
   (void)hipFree(bt->d_tables); (void)hipFree(bt->d_q); (void)hipFree(bt->d_w);
   (void)hipFree(bt->mm.pos_exact); (void)hipFree(bt->mm.pos_nn); (void)hipFree(bt->mm.fallback);
   (void)hipFree(bt->mm.qtot); (void)hipFree(bt->d_xinf); (void)hipFree(bt->d_uinf);
+  (void)hipFree(bt->fft.tw); (void)hipFree(bt->fft.shat); (void)hipFree(bt->fft.spec_class);
+  (void)hipFree(bt->fft.list); (void)hipFree(bt->fft.qhat); (void)hipFree(bt->fft.what);
+  bt->fft = FftBufs{};
   bt->mm = MemberMap{};
   bt->d_xinf = nullptr; bt->d_uinf = nullptr;
   bt->d_mem = nullptr; bt->d_gs = nullptr; bt->d_perm = nullptr;
@@ -1691,6 +1695,25 @@ static int alloc_gemm(hps_batch* bt, uint32_t nr, uint32_t nz, uint32_t B, hipSt
   GemmState g{};
   for (uint32_t k = 0; k < kGemmSlots; ++k) g.slot_class[k] = INT32_MIN;
   HPS_CHECK(hipMemcpy(bt->d_gs, &g, sizeof g, hipMemcpyHostToDevice));
+  if (fft_product_supported(nr, nz)) {
+    // spectral product: twiddles in FP64 rounded to FP32, empty spectrum cache
+    const size_t L = 2 * (size_t)nz, F = nz + 1, R = 2 * (size_t)(nr + 1);
+    std::vector<float2> tw(L / 2);
+    for (size_t k = 0; k < L / 2; ++k) {
+      const double a = -2.0 * M_PI * (double)k / (double)L;
+      tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    std::vector<int32_t> none(kGemmSlots, INT32_MIN);
+    HPS_CHECK(hipMalloc(&bt->fft.tw, tw.size() * sizeof(float2)));
+    HPS_CHECK(hipMemcpy(bt->fft.tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HPS_CHECK(hipMalloc(&bt->fft.shat, kGemmSlots * F * R * nr * sizeof(float2)));
+    HPS_CHECK(hipMalloc(&bt->fft.spec_class, kGemmSlots * sizeof(int32_t)));
+    HPS_CHECK(hipMemcpy(bt->fft.spec_class, none.data(), kGemmSlots * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+    HPS_CHECK(hipMalloc(&bt->fft.list, (kGemmSlots + 1) * sizeof(uint32_t)));
+    HPS_CHECK(hipMalloc(&bt->fft.qhat, kGemmCols * (size_t)B * F * nr * sizeof(float2)));
+    HPS_CHECK(hipMalloc(&bt->fft.what, kGemmCols * (size_t)B * F * R * sizeof(float2)));
+  }
   bt->gemm_key[0] = nr; bt->gemm_key[1] = nz; bt->gemm_key[2] = B;
   bt->gemm_key_lnr = bt->aspect_lnr;
   return 0;
@@ -1702,12 +1725,22 @@ static int alloc_gemm(hps_batch* bt, uint32_t nr, uint32_t nz, uint32_t B, hipSt
 static int tune_gemm(hps_batch* bt, uint32_t nr, uint32_t nz, uint32_t B, hipStream_t s)
 {
   const uint32_t n = gemm_shapes();
+  const FftBufs* fb = bt->fft.tw ? &bt->fft : nullptr;
   std::vector<hipEvent_t> ev(2 * n);
   for (auto& e : ev) HPS_CHECK(hipEventCreate(&e));
-  HPS_CHECK(launch_gemm_product(bt->d_gs, bt->d_tables, bt->d_q, bt->d_w, nr, nz, 0, bt->cus, s));
+  // untimed first launches: the direct product, and the spectral one, whose
+  // first launch also transforms the class tables in use (kept for later)
+  HPS_CHECK(launch_gemm_product(bt->d_gs, bt->d_tables, bt->d_q, bt->d_w, nr, nz, 0, bt->cus,
+                                fb, s));
+  for (uint32_t k = 0; k < n; ++k)
+    if (gemm_shape(k).spectral && fb)
+      HPS_CHECK(launch_gemm_product(bt->d_gs, bt->d_tables, bt->d_q, bt->d_w, nr, nz, k,
+                                    bt->cus, fb, s));
   for (uint32_t k = 0; k < n; ++k) {
     HPS_CHECK(hipEventRecord(ev[2 * k], s));
-    HPS_CHECK(launch_gemm_product(bt->d_gs, bt->d_tables, bt->d_q, bt->d_w, nr, nz, k, bt->cus, s));
+    if (!gemm_shape(k).spectral || fb)
+      HPS_CHECK(launch_gemm_product(bt->d_gs, bt->d_tables, bt->d_q, bt->d_w, nr, nz, k,
+                                    bt->cus, fb, s));
     HPS_CHECK(hipEventRecord(ev[2 * k + 1], s));
   }
   HPS_CHECK(hipEventSynchronize(ev[2 * n - 1]));
@@ -1715,6 +1748,7 @@ static int tune_gemm(hps_batch* bt, uint32_t nr, uint32_t nz, uint32_t B, hipStr
   uint32_t best = 0;
   for (uint32_t k = 0; k < n; ++k) {
     HPS_CHECK(hipEventElapsedTime(&bt->gemm_times[k], ev[2 * k], ev[2 * k + 1]));
+    if (gemm_shape(k).spectral && !fb) continue;   // not available for this mesh
     if (bt->gemm_times[k] < bt->gemm_times[best]) best = k;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
@@ -2077,7 +2111,10 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
       a.sc = m->sc->dev; a.fld1 = m->sc->fld1; a.b = m->b->d; a.st = m->b->st; a.bins = m->sc->bins;
       // the grouped field solve has no per-beam Green's table
       a.greens_blocks = bt->field_mode == 1 ? 0 : gb;
-      a.dep_blocks = m->b->blck_size / 4 != 0 ? deposit_blocks(m->b->n, 4) : 0;
+      // deposit-only rebuilds (grouped solve) take fuller deposit workgroups
+      a.dep_blocks = m->b->blck_size / 4 == 0 ? 0
+                     : bt->field_mode == 1 ? deposit_blocks_batch(m->b->n)
+                                           : deposit_blocks(m->b->n, 4);
       const uint32_t nb = sc_build_blocks(a);
       for (uint32_t c = 0; c < nb; ++c)
         build_map.push_back(((uint32_t)k << kBatchLocalBits) | c);
@@ -2200,6 +2237,8 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
                                         bt->d_tables, bt->d_q, nr, nz, bt->aspect_lnr, bt->cus,
                                         s));
           if (bt->gemm_shape == UINT32_MAX && tune_gemm(bt, nr, nz, (uint32_t)B, s)) return -1;
+          if (gemm_shape(bt->gemm_shape).spectral && !bt->fft.tw)
+            return fail("the spectral product needs an even nr and a power-of-two nz");
           if (bt->profile) {   // the product alone (family 6), inside the field family
             while (bt->evpool.size() < nev + 2) {
               hipEvent_t e;
@@ -2210,7 +2249,8 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
             HPS_CHECK(hipEventRecord(bt->evpool[nev], s));
           }
           HPS_CHECK(launch_gemm_product(bt->d_gs, bt->d_tables, bt->d_q, bt->d_w, nr, nz,
-                                        bt->gemm_shape, bt->cus, s));
+                                        bt->gemm_shape, bt->cus,
+                                        bt->fft.tw ? &bt->fft : nullptr, s));
           if (bt->profile) {
             HPS_CHECK(hipEventRecord(bt->evpool[nev + 1], s));
             nev += 2;
@@ -2369,7 +2409,7 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
     const void* ptrs[] = {bt->d_rec, d_chunk_map, d_build, d_build_map, d_field, bt->d_mem,
                           bt->d_gs, bt->d_perm, bt->d_tables, bt->d_q, bt->d_w, bt->d_ctl,
                           bt->sur, bt->capture, bt->d_sur, bt->d_xinf, cap_data.x,
-                          bt->mm.pos_exact};
+                          bt->mm.pos_exact, bt->fft.shat, bt->fft.qhat, bt->fft.what};
     put(ptrs, sizeof ptrs);
     const uint32_t dims[] = {(uint32_t)B, nr, nz, total_chunks, build_blocks, bt->field_shape,
                              bt->gemm_shape, bt->cus, gemm ? 1u : 0u};

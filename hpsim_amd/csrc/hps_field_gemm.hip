@@ -583,10 +583,12 @@ uint32_t gemm_shapes() { return kGemmShapes; }
 
 GemmShape gemm_shape(uint32_t i)
 {
-  // (beams per tile, K splits, workgroups per CU of the persistent grid)
+  // (beams per tile, K splits, workgroups per CU of the persistent grid,
+  // spectral): eight shapes of the direct product, then the product in the
+  // frequency domain along z
   static const GemmShape shapes[kGemmShapes] = {
-      {64, 1, 2}, {64, 2, 2}, {64, 4, 2}, {32, 1, 2}, {32, 2, 2}, {32, 4, 2},
-      {64, 2, 4}, {32, 4, 4}};
+      {64, 1, 2, 0}, {64, 2, 2, 0}, {64, 4, 2, 0}, {32, 1, 2, 0}, {32, 2, 2, 0}, {32, 4, 2, 0},
+      {64, 2, 4, 0}, {32, 4, 4, 0}, {32, 1, 4, 1}};
   return shapes[i < kGemmShapes ? i : 0];
 }
 
@@ -640,9 +642,13 @@ hipError_t launch_capture(const GemmMember* mem, uint32_t nmem, const GemmState*
 
 hipError_t launch_gemm_product(const GemmState* gs, const float* tables, const float* q, float* w,
                                uint32_t nr, uint32_t nz, uint32_t shape, uint32_t cus,
-                               hipStream_t s)
+                               const FftBufs* fb, hipStream_t s)
 {
   const GemmShape sh = gemm_shape(shape);
+  if (sh.spectral) {
+    if (!fb) return hipErrorInvalidValue;
+    return launch_fft_product(gs, tables, q, w, *fb, nr, nz, cus, s);
+  }
   const dim3 grid(cus * sh.per_cu), block(kThreads);
   if (sh.nt == 64)
     hipLaunchKernelGGL(k_gemm_field<64>, grid, block, 0, s, gs, tables, q, w, nr, nz, sh.splits);

@@ -131,10 +131,12 @@ hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s);
 // besides its histogram.
 constexpr uint32_t kDepositBlock = 1024;
 constexpr uint32_t kDepositMaxBlocks = 256;
+constexpr uint32_t kDepositMaxBlocksBatch = 128;
 constexpr size_t kBlockLdsReserve = 4 * kDepositBlock;
 // deposit workgroups for n particles, per_thread particles per thread (at most
 // kDepositMaxBlocks workgroups)
 uint32_t deposit_blocks(uint32_t n, uint32_t per_thread);
+uint32_t deposit_blocks_batch(uint32_t n);
 // Green's table + charge deposit (deposit = false for beams whose reference
 // deposit launch is empty).
 hipError_t launch_sc_build(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, BeamArrays b,
@@ -197,8 +199,9 @@ struct GemmShape {
   uint32_t nt;        // beams per tile (32 or 64)
   uint32_t splits;    // K splits (source-ring ranges)
   uint32_t per_cu;    // persistent workgroups per CU
+  uint32_t spectral;  // 1: the product in the frequency domain along z (hps_field_fft.hip)
 };
-constexpr uint32_t kGemmShapes = 8;
+constexpr uint32_t kGemmShapes = 9;
 uint32_t gemm_shapes();
 GemmShape gemm_shape(uint32_t i);
 constexpr uint32_t kGemmMaxSplits = 4;
@@ -240,9 +243,28 @@ static_assert(offsetof(SurCtl, kick) == kSurCtlConfigBytes, "SurCtl configuratio
 hipError_t launch_gemm_prepare(const GemmMember* mem, uint32_t nmem, GemmState* gs,
                                uint32_t* perm, MemberMap mm, SurCtl* ctl, float* tables, float* q,
                                uint32_t nr, uint32_t nz, double lnr, uint32_t cus, hipStream_t s);
+// Buffers of the spectral product (one K split): twiddles exp(-2 pi i k / L)
+// for k < L / 2 (L = 2 nz), the slots' table spectra [slot][f][row][rs]
+// (f = 0 .. nz, row = 2 rp + comp), the class each slot's spectrum belongs
+// to, a worklist of slots to transform, and per product column the charge
+// spectra [p][f][rs] and field spectra [p][f][row].
+struct FftBufs {
+  float2* tw;
+  float2* shat;
+  int32_t* spec_class;
+  uint32_t* list;
+  float2* qhat;
+  float2* what;
+};
+bool fft_product_supported(uint32_t nr, uint32_t nz);
+hipError_t launch_fft_product(const GemmState* gs, const float* tables, const float* q, float* w,
+                              const FftBufs& fb, uint32_t nr, uint32_t nz, uint32_t cus,
+                              hipStream_t s);
+// fb: null when the spectral shape is not available (then only the direct
+// shapes may be asked for)
 hipError_t launch_gemm_product(const GemmState* gs, const float* tables, const float* q, float* w,
                                uint32_t nr, uint32_t nz, uint32_t shape, uint32_t cus,
-                               hipStream_t s);
+                               const FftBufs* fb, hipStream_t s);
 hipError_t launch_gemm_store(const GemmMember* mem, uint32_t nmem, const GemmState* gs,
                              MemberMap mm, const float* w, const float* y, const SurCtl* ctl,
                              uint32_t nr, uint32_t nz, double lnr, uint32_t shape, hipStream_t s);

@@ -1790,6 +1790,22 @@ uint32_t deposit_blocks(uint32_t n, uint32_t per_thread)
   return grid;
 }
 
+// Deposit workgroups of a batch member whose rebuild is the deposit alone
+// (grouped field solve, no per-beam Green's table): four particles per
+// thread up to kDepositMaxBlocksBatch workgroups.  Members of more than
+// 524,288 particles deposit more particles per workgroup, which spreads the
+// zeroing and flush of each workgroup's LDS histogram over more particles
+// (64 members of 4,194,304 particles on the 64 x 256 mesh: deposit 261 ->
+// 242 ms per 160-element step at 128 workgroups per member; 294 at 512,
+// 360 at 1,024).
+uint32_t deposit_blocks_batch(uint32_t n)
+{
+  uint32_t grid = (n + 4 * kDepositBlock - 1) / (4 * kDepositBlock);
+  if (grid < 1) grid = 1;
+  if (grid > kDepositMaxBlocksBatch) grid = kDepositMaxBlocksBatch;
+  return grid;
+}
+
 uint32_t greens_blocks(uint32_t nr, uint32_t nz)
 {
   return ((nr + 1) * nz * nr + kDepositBlock - 1) / kDepositBlock;

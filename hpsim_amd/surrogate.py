@@ -229,6 +229,21 @@ def training_workloads(nr=32, nz=128, seeds=(101, 102, 103), nbeams=64, ramp_bea
     return out
 
 
+def stress_workloads(nr=64, nz=256, seeds=(23,), nbeams=16, num=1048576, periods=400):
+    """Tracking runs on the combined-stress lattice (the 65,536-particle
+    configuration's lattice extended to `periods` periods, unperturbed):
+    beams drawn with seeds other than the benchmarked workload's, so the
+    training data covers the long, lossy lattice the stress workload
+    tracks."""
+    from . import configs as cfg
+    out = []
+    for s in seeds:
+        bw = cfg.combined_stress(nbeams=nbeams, num=num, periods=periods, seed=s)
+        bw.nr, bw.nz = nr, nz
+        out.append(bw)
+    return out
+
+
 def collect(net, workloads, aspect_step=0.05, log=None):
     """Tracks every workload as one grouped batch with the capture on: each
     remesh appends (charge, exact field) to net's dataset."""
@@ -266,21 +281,38 @@ def weights_path(nr, nz):
 
 def train_network(nr=32, nz=128, widths=DEFAULT_WIDTHS, nbot=DEFAULT_NBOT, steps=20000,
                   batch=64, lr=5e-4, lr_end=5e-6, holdout=0.05, seed=1, workloads=None,
-                  capacity=250000, chunk=20000, log=print):
+                  capacity=250000, chunk=20000, log=print, init=None, time_limit=None,
+                  checkpoint=None):
     """Data from the engine's own SCHEFF solves, then Adam with a cosine
     learning rate from lr to lr_end over all steps, run in chunks of
     `chunk` steps (the held-out error logged after each); returns (net,
     held-out relative L2 errors (a random 5 % of the samples), per-step
-    losses)."""
+    losses).  init: weights file to start from (its normalisation is kept,
+    widths and depth come from the file); time_limit: stop after the chunk
+    that passes this many seconds of training; checkpoint: file the weights
+    are saved to after every chunk."""
     import math
     import time
+    if init is not None:
+        d = np.load(init, allow_pickle=False)
+        nbot = int(d["dims"][2])
+        widths = tuple(int(v) for v in d["dims"][3:7])
     net = Network(nr=nr, nz=nz, widths=widths, nbot=nbot, max_batch=batch, seed=seed)
     net.reserve(capacity)
     collect(net, workloads if workloads is not None else training_workloads(nr, nz), log=log)
     net.shuffle(seed)
     n = net.size()
     n_hold = max(1, int(n * holdout))
-    net.fit_norm(0, n - n_hold)
+    if init is not None:
+        net.set_params(d["params"])
+        net.set_norm(*(float(v) for v in d["norm"]))
+        if log:
+            e = net.eval(n - n_hold, n_hold)
+            log(f"start ({init}): held-out relative L2 error median {np.median(e):.4f}, "
+                f"95th percentile {np.percentile(e, 95):.4f}")
+    else:
+        net.fit_norm(0, n - n_hold)
+
     def lr_at(s):
         return lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * s / steps))
 
@@ -294,6 +326,12 @@ def train_network(nr=32, nz=128, widths=DEFAULT_WIDTHS, nbot=DEFAULT_NBOT, steps
             e = net.eval(n - n_hold, n_hold)
             log(f"{s0 + k} steps ({time.time() - t0:.0f} s): held-out relative L2 error "
                 f"median {np.median(e):.4f}, 95th percentile {np.percentile(e, 95):.4f}")
+        if checkpoint:
+            save(net, checkpoint, widths, nbot)
+        if time_limit is not None and time.time() - t0 > time_limit:
+            if log:
+                log(f"time limit: stopped after {s0 + k} of {steps} steps")
+            break
     losses = np.concatenate(parts)
     err = net.eval(n - n_hold, n_hold)
     if log:
@@ -313,10 +351,20 @@ if __name__ == "__main__":
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--init", default=None, help="weights file to start from (fine-tuning)")
+    ap.add_argument("--stress-beams", type=int, default=0,
+                    help="also capture pairs from this many beams on the stress lattice")
+    ap.add_argument("--stress-num", type=int, default=1048576)
+    ap.add_argument("--chunk", type=int, default=20000)
+    ap.add_argument("--time-limit", type=float, default=None)
     a = ap.parse_args()
-    net, err, _ = train_network(a.nr, a.nz, steps=a.steps, lr=a.lr, lr_end=a.lr * 0.01,
-                                batch=a.batch, log=lambda s: print(s, flush=True))
     out = a.out or weights_path(a.nr, a.nz)
-    os.makedirs(os.path.dirname(out), exist_ok=True)
-    save(net, out, DEFAULT_WIDTHS, DEFAULT_NBOT)
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+    wls = training_workloads(a.nr, a.nz)
+    if a.stress_beams:
+        wls += stress_workloads(a.nr, a.nz, nbeams=a.stress_beams, num=a.stress_num)
+    net, err, _ = train_network(a.nr, a.nz, steps=a.steps, lr=a.lr, lr_end=a.lr * 0.01,
+                                batch=a.batch, log=lambda s: print(s, flush=True),
+                                workloads=wls, init=a.init, chunk=a.chunk,
+                                time_limit=a.time_limit, checkpoint=out)
     print(f"saved {out}")
