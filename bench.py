@@ -54,20 +54,48 @@ SURROGATE_MAX_ERR = 0.02       # relative L2 field error that triggers the fallb
 SEG_BYTES_PER_PARTICLE = 104   # k_segment: reads 6 FP64 + 2 u32, writes 6 FP64
 SPECTRAL_SHAPE = 8             # product shape that runs in the frequency domain along z
 DEFAULT_FIELD = "gemm"         # field form of the batched workloads' headline figure
-# HBM bytes per k_segment launch at 1,048,576 particles, measured with rocprofv3
-# FETCH_SIZE / WRITE_SIZE counter runs (calibrated on a plain device copy)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_segment_traffic_plain-fp64-1048576x200.md")
+MI355X_CUS = 256
+MI355X_CLOCK_HZ = 2.4e9
+# HBM bytes per segment-kernel launch, measured with rocprofv3 FETCH_SIZE /
+# WRITE_SIZE counter runs (calibrated on a plain device copy), and the segment
+# kernel's VALU instructions per launch, for the launch shapes they were
+# measured at
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC_FILES = {
+    ("plain-fp64-1048576x200", 1048576):
+        os.path.join(PROFILES, "k_segment_traffic_plain-fp64-1048576x200.md"),
+    ("stress-64x4194304x400", 268435456):
+        os.path.join(PROFILES, "k_segment_batch_traffic_stress-64x4194304x400.md"),
+}
+VALU_FILES = {
+    ("stress-64x4194304x400", 268435456):
+        os.path.join(PROFILES, "counters_stress-64x4194304x400.md"),
+}
 
 
-def measured_traffic(num):
-    """Traffic per k_segment launch from the committed counter summary, for
-    the particle count it was measured at; None otherwise."""
-    if num != 1048576 or not os.path.exists(TRAFFIC_FILE):
+def _profile_value(files, key, field):
+    path = files.get(key)
+    if path is None or not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
-        if line.startswith("traffic_bytes_per_launch:"):
+    for line in open(path):
+        if line.startswith(field + ":"):
             return int(line.split(":", 1)[1])
     return None
+
+
+def measured_traffic(workload, num):
+    """HBM bytes per segment-kernel launch from the committed counter summary
+    of this workload and particle count; None when not measured."""
+    return _profile_value(TRAFFIC_FILES, (workload, num), "traffic_bytes_per_launch")
+
+
+def valu_issue(workload, num, avg_launch_s):
+    """Fraction of the segment kernel's time its VALU instructions (committed
+    counter summary) need at one wave64 instruction per CU per clock."""
+    n = _profile_value(VALU_FILES, (workload, num), "valu_insts_per_launch")
+    if n is None:
+        return None
+    return n / (MI355X_CUS * MI355X_CLOCK_HZ) / avg_launch_s
 
 
 def parse():
@@ -382,7 +410,9 @@ def main_batched(args):
                    "elements_per_step": sel, "mesh": f"{w.nr}x{w.nz}", "field": mode,
                    "transmitted_after_profiled_step": transmitted},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(w.name, total),
+                     "valu_issue_frac": valu_issue(w.name, total, avg_s),
                      "kernel": "k_segment_batch", "avg_launch_us": avg_s * 1e6,
                      "launches": seg_n, "bytes_per_launch": SEG_BYTES_PER_PARTICLE * total,
                      "timing": "HIP events around every launch of one extra (eager) step"},
@@ -521,7 +551,7 @@ def main():
                    "mesh": f"{wl.nr}x{wl.nz}", "sc_kicks_per_step": prof["phase_final"][1],
                    "transmitted": b.num - lost},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(b.num),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name, b.num),
                      # beams of at most 512 chunks of 1024 particles run the
                      # one-particle-per-thread form of the segment kernel
                      "kernel": "k_segment_wide" if b.num <= 512 * 1024 else "k_segment",
