@@ -283,8 +283,12 @@ int hps_batch_graph_info(hps_batch* bt, uint64_t* replays, uint32_t* period_laun
  * third order in the step (below 5e-6 of the field at 0.05) plus FP32
  * rounding.  No reference counterpart. */
 int hps_batch_set_field_mode(hps_batch* bt, int mode, double aspect_step);
-/* Launch shape of the grouped product (index < 8, or UINT32_MAX: autotune
- * at the first rebuild, the default). */
+/* Launch shape of the grouped product (index < 9, or UINT32_MAX: autotune
+ * at the first rebuild, the default).  Shapes 0-7 run the direct product
+ * (columns per tile, K splits, workgroups per CU); shape 8 runs it in the
+ * frequency domain along z (radix-2 transforms of length 2 nz, the
+ * per-frequency complex product on the matrix cores; needs an even nr and
+ * a power-of-two nz, else simulate fails when it is asked for). */
 int hps_batch_set_gemm_shape(hps_batch* bt, uint32_t shape);
 /* info[0] chosen shape, [1] class tables built, [2] product columns that
  * used another class's table (cache conflicts), [3] product columns of the
