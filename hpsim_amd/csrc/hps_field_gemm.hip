@@ -86,15 +86,32 @@ __global__ __launch_bounds__(kThreads) void k_gemm_schedule(const GemmMember* __
 {
   __shared__ uint32_t count[kGemmSlots];
   __shared__ uint32_t claimed[kGemmSlots];
-  __shared__ int32_t mslot[kGemmCols * kGemmMaxMembers];
+  static_assert(kGemmSlots <= 127, "slot numbers are kept in int8");
+  __shared__ int8_t mslot[kGemmCols * kGemmMaxMembers];
+  __shared__ int32_t mclass[kGemmMaxMembers];   // first class c - 1 of a remeshing member
+  __shared__ uint8_t mrem[kGemmMaxMembers];     // the member remeshes at this kick point
+  __shared__ uint32_t any_rem;
   const int t = threadIdx.x;
   if (t < (int)kGemmSlots) { count[t] = 0; claimed[t] = 0; }
+  if (t == 0) any_rem = 0;
+  __syncthreads();
+  // every member's remesh flag and aspect class, loaded by all threads (one
+  // dependent pair of loads per member); thread 0 then walks them in order
+  for (uint32_t m = t; m < nmem; m += kThreads) {
+    const ScDev* sc = mem[m].sc;
+    const uint32_t rem = sc->remesh;
+    mrem[m] = rem ? 1 : 0;
+    if (rem) {
+      double wl[3];
+      mclass[m] = aspect_lagrange(sc->log_aspect, lnr, wl) - 1;
+      any_rem = 1;
+    }
+  }
   __syncthreads();
   if (t != 0) return;
   bool audit = false;
   if (ctl) {
-    uint32_t any = 0;
-    for (uint32_t m = 0; m < nmem && !any; ++m) any = mem[m].sc->remesh;
+    const uint32_t any = any_rem;
     if (ctl->audit_every && ctl->kick % ctl->audit_every == 0) ctl->pending = 1;
     ctl->kick += 1;
     audit = ctl->pending && any;
@@ -103,14 +120,12 @@ __global__ __launch_bounds__(kThreads) void k_gemm_schedule(const GemmMember* __
   }
   uint32_t nbuild = 0, conflicts = gs->conflicts, nn = 0, ncol = 0;
   for (uint32_t m = 0; m < nmem; ++m) {
-    const ScDev* sc = mem[m].sc;
     mm.pos_nn[m] = -1;
     for (uint32_t h = 0; h < kGemmCols; ++h) mslot[kGemmCols * m + h] = -1;
-    if (!sc->remesh) continue;
+    if (!mrem[m]) continue;
     if (ctl) mm.pos_nn[m] = (int32_t)nn++;
     if (ctl && !audit && !mm.fallback[m]) continue;
-    double wl[3];
-    const int32_t c0 = aspect_lagrange(sc->log_aspect, lnr, wl) - 1;
+    const int32_t c0 = mclass[m];
     for (uint32_t h = 0; h < kGemmCols; ++h) {
       const int32_t c = c0 + (int32_t)h;
       const int32_t slot = (int32_t)((uint32_t)c & (kGemmSlots - 1));
@@ -124,7 +139,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_schedule(const GemmMember* __
       }
       claimed[slot] = 1;
       count[slot] += 1;
-      mslot[kGemmCols * m + h] = slot;
+      mslot[kGemmCols * m + h] = (int8_t)slot;
       ++ncol;
     }
   }
