@@ -2113,7 +2113,7 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
       a.greens_blocks = bt->field_mode == 1 ? 0 : gb;
       // deposit-only rebuilds (grouped solve) take fuller deposit workgroups
       a.dep_blocks = m->b->blck_size / 4 == 0 ? 0
-                     : bt->field_mode == 1 ? deposit_blocks_batch(m->b->n)
+                     : bt->field_mode == 1 ? deposit_blocks_batch(m->b->n, nr, nz)
                                            : deposit_blocks(m->b->n, 4);
       const uint32_t nb = sc_build_blocks(a);
       for (uint32_t c = 0; c < nb; ++c)

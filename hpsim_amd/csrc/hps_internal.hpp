@@ -132,11 +132,12 @@ hipError_t launch_phase_final(const PhaseArgs& a, hipStream_t s);
 constexpr uint32_t kDepositBlock = 1024;
 constexpr uint32_t kDepositMaxBlocks = 256;
 constexpr uint32_t kDepositMaxBlocksBatch = 128;
+constexpr uint32_t kDepositMaxBlocksBig = 32;
 constexpr size_t kBlockLdsReserve = 4 * kDepositBlock;
 // deposit workgroups for n particles, per_thread particles per thread (at most
 // kDepositMaxBlocks workgroups)
 uint32_t deposit_blocks(uint32_t n, uint32_t per_thread);
-uint32_t deposit_blocks_batch(uint32_t n);
+uint32_t deposit_blocks_batch(uint32_t n, uint32_t nr, uint32_t nz);
 // Green's table + charge deposit (deposit = false for beams whose reference
 // deposit launch is empty).
 hipError_t launch_sc_build(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, BeamArrays b,

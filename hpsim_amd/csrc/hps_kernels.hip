@@ -862,7 +862,14 @@ __device__ __forceinline__ void phase_body(const PhaseArgs& a, uint32_t c)
     const uint32_t i = c * kChunk + j * kBlock + t;
     double pr_ = 0.0;
     uint32_t loss = 1, lloss = 0;
-    if (i < b.n) { loss = b.loss[i]; lloss = b.lloss[i]; pr_ = relative_phi(b.phi[i], ref); }
+    if (i < b.n) {
+      loss = b.loss[i];
+      lloss = b.lloss[i];
+      // the relative phase enters the sums only for loss == 0; a lost
+      // particle keeps its last phase, which falls ever further behind the
+      // reference, so its exact remainder would take the longest
+      if (loss == 0) pr_ = relative_phi(b.phi[i], ref);
+    }
     const bool tg = (i < b.n) && loss == 0;
     const bool g = (i < b.n) && is_good(loss, lloss);
     acc[0] = acc[0] + (tg ? pr_ : 0.0);
@@ -896,7 +903,11 @@ __global__ __launch_bounds__(kWideBlock) void k_phase_wide(PhaseArgs a)
   const uint32_t i = c * kChunk + (uint32_t)t;
   double pr_ = 0.0;
   uint32_t loss = 1, lloss = 0;
-  if (i < b.n) { loss = b.loss[i]; lloss = b.lloss[i]; pr_ = relative_phi(b.phi[i], ref); }
+  if (i < b.n) {
+    loss = b.loss[i];
+    lloss = b.lloss[i];
+    if (loss == 0) pr_ = relative_phi(b.phi[i], ref);   // used only for loss == 0
+  }
   const bool tg = (i < b.n) && loss == 0;
   const bool g = (i < b.n) && is_good(loss, lloss);
   const double vg = g ? pr_ : 0.0;
@@ -974,8 +985,8 @@ __device__ __forceinline__ void phase_wave_body(const PhaseArgs& a, uint32_t c, 
     double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < (int)kPerLane; ++j) {
-      const uint32_t i = c * kChunk + j * kBlock + q * 64 + l;
-      const double pr_ = i < b.n ? relative_phi(ph[q][j], ref) : 0.0;
+      // used only for loss == 0 (beyond n, loss reads 1)
+      const double pr_ = lo[q][j] == 0 ? relative_phi(ph[q][j], ref) : 0.0;
       const bool tg = lo[q][j] == 0;
       const bool g = is_good(lo[q][j], ll[q][j]);
       acc[0] = acc[0] + (tg ? pr_ : 0.0);
@@ -1792,17 +1803,21 @@ uint32_t deposit_blocks(uint32_t n, uint32_t per_thread)
 
 // Deposit workgroups of a batch member whose rebuild is the deposit alone
 // (grouped field solve, no per-beam Green's table): four particles per
-// thread up to kDepositMaxBlocksBatch workgroups.  Members of more than
-// 524,288 particles deposit more particles per workgroup, which spreads the
-// zeroing and flush of each workgroup's LDS histogram over more particles
-// (64 members of 4,194,304 particles on the 64 x 256 mesh: deposit 261 ->
-// 242 ms per 160-element step at 128 workgroups per member; 294 at 512,
-// 360 at 1,024).
-uint32_t deposit_blocks_batch(uint32_t n)
+// thread, at most kDepositMaxBlocksBatch workgroups, or kDepositMaxBlocksBig
+// when the histogram takes more than 64 KB of LDS (one workgroup per CU).
+// Fewer, fuller workgroups spread the zeroing and flush of each workgroup's
+// histogram over more particles: 64 members of 4,194,304 particles on the
+// 64 x 256 mesh, deposit per 160-element step 360 / 294 / 261 / 241 / 233 /
+// 227 ms at 1,024 / 512 / 256 / 128 / 64 / 32 workgroups per member; the
+// 256-beam scan on 32 x 128 takes 84 ms at 128 and 91 ms at 32.
+uint32_t deposit_blocks_batch(uint32_t n, uint32_t nr, uint32_t nz)
 {
+  const uint32_t cap = (size_t)nr * nz * sizeof(unsigned long long) > 65536
+                           ? kDepositMaxBlocksBig
+                           : kDepositMaxBlocksBatch;
   uint32_t grid = (n + 4 * kDepositBlock - 1) / (4 * kDepositBlock);
   if (grid < 1) grid = 1;
-  if (grid > kDepositMaxBlocksBatch) grid = kDepositMaxBlocksBatch;
+  if (grid > cap) grid = cap;
   return grid;
 }
 
