@@ -1650,6 +1650,41 @@ static PeriodRun find_period_run(const std::vector<PeriodKey>& k)
   return best;
 }
 
+// Chunk map of a batch's segment and phase launches.  The workgroups of a
+// launch go to the part's eight XCDs round robin by index (workgroup b on
+// XCD b mod 8) and each XCD has its own L2, so the members are dealt to
+// eight groups with balanced chunk totals (largest member first, each to the
+// lightest group) and block 8 j + x takes the j-th chunk of group x: all
+// chunks of a member run on one XCD, whose L2 then holds the field tables
+// the kicks gather from for its members only.  Any order gives the same
+// results (every chunk writes its own partial sums).
+static std::vector<uint32_t> xcd_chunk_map(const std::vector<uint32_t>& nch)
+{
+  constexpr uint32_t kXcds = 8;
+  const size_t B = nch.size();
+  std::vector<size_t> order(B);
+  for (size_t k = 0; k < B; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return nch[a] > nch[b]; });
+  std::vector<std::vector<uint32_t>> grp(kXcds);
+  std::vector<uint64_t> load(kXcds, 0);
+  size_t total = 0;
+  for (size_t k : order) {
+    uint32_t x = 0;
+    for (uint32_t y = 1; y < kXcds; ++y)
+      if (load[y] < load[x]) x = y;
+    for (uint32_t c = 0; c < nch[k]; ++c) grp[x].push_back(((uint32_t)k << kBatchLocalBits) | c);
+    load[x] += nch[k];
+    total += nch[k];
+  }
+  std::vector<uint32_t> map;
+  map.reserve(total);
+  for (size_t j = 0; map.size() < total; ++j)
+    for (uint32_t x = 0; x < kXcds; ++x)
+      if (j < grp[x].size()) map.push_back(grp[x][j]);
+  return map;
+}
+
 static void free_gemm(hps_batch* bt)
 {
   (void)hipFree(bt->d_mem); (void)hipFree(bt->d_gs); (void)hipFree(bt->d_perm);
@@ -2096,11 +2131,10 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
   const uint32_t nr = sc_on ? m0->sc->cfg.nx : 0, nz = sc_on ? m0->sc->cfg.nz : 0;
   if (bt->field_key[0] != nr || bt->field_key[1] != nz || bt->field_key[2] != (uint32_t)B)
     bt->field_shape = bt->autotune ? UINT32_MAX : field_batch_default_shape(nr, (uint32_t)B);
-  std::vector<uint32_t> chunk_map, build_map;
-  for (size_t k = 0; k < B; ++k) {
-    const uint32_t nch = bt->sims[k]->b->pr.nch;
-    for (uint32_t c = 0; c < nch; ++c) chunk_map.push_back(((uint32_t)k << kBatchLocalBits) | c);
-  }
+  std::vector<uint32_t> build_map;
+  std::vector<uint32_t> nchs(B);
+  for (size_t k = 0; k < B; ++k) nchs[k] = bt->sims[k]->b->pr.nch;
+  const std::vector<uint32_t> chunk_map = xcd_chunk_map(nchs);
   std::vector<BuildArgs> build(B);
   std::vector<FieldArgs> field(B);
   if (sc_on) {

@@ -1016,8 +1016,13 @@ __global__ __launch_bounds__(kBlock) void k_phase_wave_batch(const PhaseArgs* re
                                                              const uint32_t* map, uint32_t nch,
                                                              const uint64_t* roff)
 {
-  const uint32_t k = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) +
-                                                    (threadIdx.x >> 6));
+  // the chunk map deals each member's chunks to one XCD's positions (8 j + x
+  // for XCD x); workgroup b runs on XCD b mod 8, so its four waves take
+  // positions 32 (b / 8) + 8 w + b mod 8, w < 4: every position once, each
+  // on its XCD
+  static_assert(kBlock == 256, "four waves per workgroup");
+  const uint32_t b = blockIdx.x, w = threadIdx.x >> 6;
+  const uint32_t k = __builtin_amdgcn_readfirstlane(32u * (b >> 3) + 8u * w + (b & 7u));
   if (k >= nch) return;
   const uint32_t e = map[k];
   const PhaseArgs a = period_recs(recs, roff)[e >> kBatchLocalBits];
@@ -1310,11 +1315,11 @@ __device__ __forceinline__ bool deposit_one(double x, double y, double phi, cons
 
 // The deposit of a workgroup's share of the particles (strided by the
 // deposit grid) into its LDS histogram, flushed into the global one with
-// 64-bit integer atomics.  kPrefetch: each thread loads its next particle
-// (flags and coordinates, unconditionally) before depositing the current
-// one (the batched rebuilds); the histogram is the same (integer sums), only
-// the load order differs.
-template <bool kPrefetch>
+// 64-bit integer atomics.  Each thread loads its next particle (flags and
+// coordinates, unconditionally) before depositing the current one; the
+// histogram is the same (integer sums), only the load order differs (the
+// 1,048,576-particle beam: 14.7 -> 14.2 ms per 800-element step; 64 beams
+// of 4,194,304 particles on 64 x 256: 226 -> 185 ms per 160-element step).
 __device__ __forceinline__ void deposit_block(const BeamArrays& b, const hps_beam_stats* st,
                                               ScDev* sc, int64_t* bins, uint32_t blk,
                                               uint32_t nblk, unsigned long long* hist,
@@ -1327,31 +1332,24 @@ __device__ __forceinline__ void deposit_block(const BeamArrays& b, const hps_bea
   __syncthreads();
   uint32_t count = 0;
   const uint32_t stride = nblk * kDepositBlock;
-  if (kPrefetch) {
-    uint32_t np = blk * kDepositBlock + t;
-    bool have = np < b.n;
-    uint32_t lo = 1, ll = 0;
-    double x = 0.0, y = 0.0, ph = 0.0;
+  uint32_t np = blk * kDepositBlock + t;
+  bool have = np < b.n;
+  uint32_t lo = 1, ll = 0;
+  double x = 0.0, y = 0.0, ph = 0.0;
+  if (have) {
+    lo = gld(b.loss, np); ll = gld(b.lloss, np);
+    x = gld(b.x, np); y = gld(b.y, np); ph = gld(b.phi, np);
+  }
+  while (have) {
+    const uint32_t clo = lo, cll = ll;
+    const double cx = x, cy = y, cph = ph;
+    np += stride;
+    have = np < b.n;
     if (have) {
       lo = gld(b.loss, np); ll = gld(b.lloss, np);
       x = gld(b.x, np); y = gld(b.y, np); ph = gld(b.phi, np);
     }
-    while (have) {
-      const uint32_t clo = lo, cll = ll;
-      const double cx = x, cy = y, cph = ph;
-      np += stride;
-      have = np < b.n;
-      if (have) {
-        lo = gld(b.loss, np); ll = gld(b.lloss, np);
-        x = gld(b.x, np); y = gld(b.y, np); ph = gld(b.phi, np);
-      }
-      if (clo == 0 && cll == 0 && deposit_one(cx, cy, cph, d, hist)) ++count;
-    }
-  } else {
-    for (uint32_t np = blk * kDepositBlock + t; np < b.n; np += stride) {
-      if (!(b.loss[np] == 0 && b.lloss[np] == 0)) continue;
-      if (deposit_one(b.x[np], b.y[np], b.phi[np], d, hist)) ++count;
-    }
+    if (clo == 0 && cll == 0 && deposit_one(cx, cy, cph, d, hist)) ++count;
   }
   __syncthreads();
   for (uint32_t i = t; i < nb; i += kDepositBlock) {
@@ -1378,7 +1376,6 @@ __device__ __forceinline__ void deposit_block(const BeamArrays& b, const hps_bea
 // combined).  Both roles read only the control step's results; all exit
 // when there is no remesh.
 
-template <bool kPrefetch>
 __device__ __forceinline__ void sc_build_body(const BuildArgs& a, uint32_t nr, uint32_t nz,
                                               uint32_t blk, unsigned long long* hist,
                                               uint32_t* ured)
@@ -1387,25 +1384,23 @@ __device__ __forceinline__ void sc_build_body(const BuildArgs& a, uint32_t nr, u
   if (a.dep_blocks >= a.greens_blocks) {
     if (blk < a.greens_blocks)
       greens_entry(a.sc, a.fld1, nr, nz, a.b.n, a.b.mass, blk * kDepositBlock + threadIdx.x);
-    deposit_block<kPrefetch>(a.b, a.st, a.sc, a.bins, blk, a.dep_blocks, hist, ured);
+    deposit_block(a.b, a.st, a.sc, a.bins, blk, a.dep_blocks, hist, ured);
     return;
   }
   if (blk < a.greens_blocks) {
     greens_entry(a.sc, a.fld1, nr, nz, a.b.n, a.b.mass, blk * kDepositBlock + threadIdx.x);
     return;
   }
-  deposit_block<kPrefetch>(a.b, a.st, a.sc, a.bins, blk - a.greens_blocks, a.dep_blocks, hist,
-                           ured);
+  deposit_block(a.b, a.st, a.sc, a.bins, blk - a.greens_blocks, a.dep_blocks, hist, ured);
 }
 
 __global__ __launch_bounds__(kDepositBlock) void k_sc_build(BuildArgs a, uint32_t nr, uint32_t nz)
 {
   extern __shared__ unsigned long long hist[];
   __shared__ uint32_t ured[kDepositBlock];
-  sc_build_body<false>(a, nr, nz, blockIdx.x, hist, ured);
+  sc_build_body(a, nr, nz, blockIdx.x, hist, ured);
 }
 
-template <bool kPrefetch>
 __global__ __launch_bounds__(kDepositBlock) void k_sc_build_batch(const BuildArgs* recs,
                                                                   const uint32_t* map,
                                                                   uint32_t nr, uint32_t nz)
@@ -1414,7 +1409,7 @@ __global__ __launch_bounds__(kDepositBlock) void k_sc_build_batch(const BuildArg
   __shared__ uint32_t ured[kDepositBlock];
   const uint32_t e = map[blockIdx.x];
   const BuildArgs a = recs[e >> kBatchLocalBits];
-  sc_build_body<kPrefetch>(a, nr, nz, e & kBatchLocalMask, hist, ured);
+  sc_build_body(a, nr, nz, e & kBatchLocalMask, hist, ured);
 }
 
 // Field table E(zp, rp) = sum over bins of charge * Green's value,
@@ -1827,8 +1822,8 @@ hipError_t launch_phase_batch(const PhaseArgs* recs, const uint32_t* map, uint32
   // workgroup); smaller ones a workgroup per chunk, which keeps more waves in
   // flight when there are few chunks.
   if (nblocks >= kPhaseWaveMinChunks)
-    hipLaunchKernelGGL(k_phase_wave_batch, dim3((nblocks + kBlock / 64 - 1) / (kBlock / 64)),
-                       dim3(kBlock), 0, s, recs, map, nblocks, roff);
+    hipLaunchKernelGGL(k_phase_wave_batch, dim3((nblocks + 31) / 32 * 8), dim3(kBlock), 0, s,
+                       recs, map, nblocks, roff);
   else
     hipLaunchKernelGGL(k_phase_batch, dim3(nblocks), dim3(kBlock), 0, s, recs, map, roff);
   return hipGetLastError();
@@ -1907,12 +1902,9 @@ hipError_t launch_sc_build(ScDev* sc, double* fld1, uint32_t nr, uint32_t nz, Be
 hipError_t launch_sc_build_batch(const BuildArgs* recs, const uint32_t* map, uint32_t nblocks,
                                  uint32_t nr, uint32_t nz, hipStream_t s)
 {
-  // batches deposit with the next particle's loads in flight (64 beams of
-  // 4,194,304 particles on 64 x 256: deposit 226 -> 185 ms per 160-element
-  // step; the 256-beam scan on 32 x 128: 88 -> 79 ms)
   const size_t lds = (size_t)nr * nz * sizeof(unsigned long long);
-  hipLaunchKernelGGL(k_sc_build_batch<true>, dim3(nblocks), dim3(kDepositBlock), lds, s, recs,
-                     map, nr, nz);
+  hipLaunchKernelGGL(k_sc_build_batch, dim3(nblocks), dim3(kDepositBlock), lds, s, recs, map, nr,
+                     nz);
   return hipGetLastError();
 }
 
