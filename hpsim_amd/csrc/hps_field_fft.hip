@@ -46,6 +46,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int kFftThreads = 256;
 constexpr int kFftLdsComplex = 4096;   // complex entries of a workgroup's transform rows (32 KB)
 constexpr int kProdNT = 32;            // columns per product tile
+constexpr int kProdMaxK = 128;         // 2 nr: the real K of the product (nr <= 64)
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_product(const GemmState* __
                                                              uint32_t nr, uint32_t nz)
 {
   __shared__ uint32_t s_start[kGemmSlots], s_count[kGemmSlots];
-  __shared__ float b_lds[kProdNT * (2 * kFftLdsComplex / 32 + 1)];
+  __shared__ float b_lds[kProdNT * (kProdMaxK + 1)];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   if (t < (int)kGemmSlots) { s_start[t] = gs->slot_start[t]; s_count[t] = gs->slot_count[t]; }
   __syncthreads();
@@ -351,7 +352,7 @@ bool fft_product_supported(uint32_t nr, uint32_t nz)
 {
   const uint32_t L = 2 * nz;
   return nr >= 2 && nr % 2 == 0 && (L & (L - 1)) == 0 && L >= 8 &&
-         L <= (uint32_t)kFftLdsComplex && 4 * nr <= 2 * (uint32_t)kFftLdsComplex / 32;
+         L <= (uint32_t)kFftLdsComplex && 2 * nr <= (uint32_t)kProdMaxK;
 }
 
 hipError_t launch_fft_product(const GemmState* gs, const float* tables, const float* q, float* w,
