@@ -134,6 +134,34 @@ def test_spectral_product_equals_direct_product_at_64x256():
         assert err < 1e-5, (k, err)
 
 
+def test_spectral_shape_needs_a_power_of_two_mesh(oracle_lib):
+    """On a 16 x 192 mesh (nz a multiple of 64 but not a power of two) the
+    frequency-domain shape is not available: asking for it fails with an
+    error, and the autotuner keeps a direct shape whose field is the FP64
+    field within the interpolation error."""
+    from hpsim_amd import HPSim
+    O = oracle_lib
+    bw = cfg.batched_scan(nbeams=256, periods=1)
+    bw.nr, bw.nz = 16, 192
+    nums = np.array([b.num for b in bw.beams])
+    pick = [int(k) for k in np.argsort(nums)[[0, 128]]]
+    objs = _members(bw, pick)
+    bt = _gemm_batch(objs, 8)
+    with pytest.raises(RuntimeError, match="spectral product"):
+        bt.simulate(0, 0)
+    objs = _members(bw, pick)
+    bt = _gemm_batch(objs)
+    bt.simulate(0, 0)
+    info = bt.gemm_info()
+    assert info["shape"] is not None and info["shape"] < 8, info
+    for k, (b, s, _) in zip(pick, objs):
+        st = s.state()
+        _, f2, bins = s.tables()
+        ref = _fp64_field(O, st, bins, int(nums[k]), bw.beams[k], bw.nr, bw.nz)
+        err = np.linalg.norm(f2 - ref) / np.linalg.norm(ref)
+        assert err < 5e-5, (k, err)
+
+
 def test_gemm_tracking_follows_the_oracle(tmp_path):
     """Eight scan members, two periods: the grouped solve against the FP64
     oracle -- the two differ by the interpolated FP32 field only."""
