@@ -172,9 +172,13 @@ def test_cross_chunk_tree_widths_bit_exact(num):
         assert gs[k] == os_[k], (k, gs[k], os_[k])
 
 
-def test_aperture_elements_and_rotation_steerer():
+@pytest.mark.parametrize("num", [4096, 600000])
+def test_aperture_elements_and_rotation_steerer(num):
+    """Aperture elements (their centroid and loss-count statistics), a
+    rotation and a steerer inside the lattice: 4 chunks (one particle per
+    thread) and 586 chunks (four particles per thread)."""
     from hpsim_amd import configs as cfg
-    wl = H.workload(0, num=4096, periods=3, sc=True)
+    wl = H.workload(0, num=num, periods=3, sc=True)
     elems = cfg.build_lattice(wl.lattice)
     extra = [cfg.Element("AC", "ApertureC", aperture=0.003, aperture_in=True),
              cfg.Element("ROT", "Rotation", angle=0.3),
