@@ -5,6 +5,10 @@ multiply-add; the kernels call fma() explicitly where the canonical order
 uses it) and -fno-fast-math.  Packed FP32 instruction selection is switched
 off for all device code.
 
+hps_push_fast.hip builds hps_kernels.hip a second time with reciprocal
+division (-fapprox-func -freciprocal-math) for the batches whose field comes
+from the FP32 product or the surrogate; its header states the bounds.
+
 hps_kernels.hip is built without machine-level loop-invariant code motion:
 the tracking kernels use ~50 FP64 polynomial and range-reduction constants
 inside their particle and element loops, and hoisting their materialisation
@@ -25,9 +29,11 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libhpsim_amd.so")
-SOURCES = ["hps_kernels.hip", "hps_engine.hip", "hps_field_gemm.hip", "hps_field_fft.hip",
-           "hps_unet.hip", "hps_surrogate.hip"]
-EXTRA_FLAGS = {"hps_kernels.hip": ["-mllvm", "-disable-machine-licm"]}
+SOURCES = ["hps_kernels.hip", "hps_push_fast.hip", "hps_engine.hip", "hps_field_gemm.hip",
+           "hps_field_fft.hip", "hps_unet.hip", "hps_surrogate.hip"]
+EXTRA_FLAGS = {"hps_kernels.hip": ["-mllvm", "-disable-machine-licm"],
+               "hps_push_fast.hip": ["-mllvm", "-disable-machine-licm", "-fapprox-func",
+                                     "-freciprocal-math"]}
 ARCH = os.environ.get("HPSIM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -2249,8 +2249,12 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
     const void* rec = bt->d_rec + l.rec;
     switch (l.kind) {
       case L_SEG:
-        HPS_CHECK(launch_segment_batch((const SegArgs*)rec, d_chunk_map, total_chunks,
-                                       l.op_class, roff, s));
+        if (gemm)
+          HPS_CHECK(fast::launch_segment_batch((const SegArgs*)rec, d_chunk_map, total_chunks,
+                                               l.op_class, roff, s));
+        else
+          HPS_CHECK(launch_segment_batch((const SegArgs*)rec, d_chunk_map, total_chunks,
+                                         l.op_class, roff, s));
         break;
       case L_SEG_FINAL:
         HPS_CHECK(launch_seg_final_batch((const SegArgs*)rec, (uint32_t)B, roff, s));
@@ -2262,7 +2266,10 @@ extern "C" int hps_batch_simulate(hps_batch* bt, int32_t start, int32_t end)
         HPS_CHECK(launch_phase_final_batch((const PhaseArgs*)rec, (uint32_t)B, roff, s));
         break;
       case L_BUILD:
-        HPS_CHECK(launch_sc_build_batch(d_build, d_build_map, build_blocks, nr, nz, s));
+        if (gemm)
+          HPS_CHECK(fast::launch_sc_build_batch(d_build, d_build_map, build_blocks, nr, nz, s));
+        else
+          HPS_CHECK(launch_sc_build_batch(d_build, d_build_map, build_blocks, nr, nz, s));
         break;
       default: {
         if (gemm) {

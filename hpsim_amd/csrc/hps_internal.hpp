@@ -163,6 +163,16 @@ hipError_t launch_phase_final_batch(const PhaseArgs* recs, uint32_t nmem, const 
 hipError_t launch_roff(uint64_t* roff, uint64_t v, bool add, hipStream_t s);
 hipError_t launch_sc_build_batch(const BuildArgs* recs, const uint32_t* map, uint32_t nblocks,
                                  uint32_t nr, uint32_t nz, hipStream_t s);
+// The same batched push and rebuild with reciprocal FP64 division
+// (hps_push_fast.hip), for batches in the grouped field solve (field mode 1,
+// with or without a surrogate), whose results are held to error bounds rather
+// than to the oracle's bits.
+namespace fast {
+hipError_t launch_segment_batch(const SegArgs* recs, const uint32_t* map, uint32_t nblocks,
+                                int op_class, const uint64_t* roff, hipStream_t s);
+hipError_t launch_sc_build_batch(const BuildArgs* recs, const uint32_t* map, uint32_t nblocks,
+                                 uint32_t nr, uint32_t nz, hipStream_t s);
+}  // namespace fast
 // Field sum of a batch in launch shape `shape` < field_batch_shapes() (any
 // larger value: the default for the batch size); every shape gives the same
 // bits, so a caller may time them and keep the fastest.

@@ -23,6 +23,13 @@
 
 namespace hps {
 
+// hps_push_fast.hip builds this file a second time with HPS_FAST_PUSH
+// defined and reciprocal division; that build's kernels and launchers live
+// in hps::fast.
+#ifdef HPS_FAST_PUSH
+namespace fast {
+#endif
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -2132,5 +2139,9 @@ hipError_t launch_probe(int fn, const double* x, const double* y, double* out, u
   hipLaunchKernelGGL(k_probe, dim3(blocks_for(n)), dim3(kBlock), 0, s, fn, x, y, out, n);
   return hipGetLastError();
 }
+
+#ifdef HPS_FAST_PUSH
+}  // namespace fast
+#endif
 
 }  // namespace hps
