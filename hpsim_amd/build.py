@@ -6,8 +6,9 @@ uses it) and -fno-fast-math.  Packed FP32 instruction selection is switched
 off for all device code.
 
 hps_push_fast.hip builds hps_kernels.hip a second time with reciprocal
-division (-fapprox-func -freciprocal-math) for the batches whose field comes
-from the FP32 product or the surrogate; its header states the bounds.
+division (-fapprox-func -freciprocal-math) and square roots from v_rsq_f64
+for the batches whose field comes from the FP32 product or the surrogate;
+its header states the bounds.
 
 hps_kernels.hip is built without machine-level loop-invariant code motion:
 the tracking kernels use ~50 FP64 polynomial and range-reduction constants

@@ -202,7 +202,7 @@ __device__ __forceinline__ double bessi0c(double r_x)
           y * (0.360768e-1 + y * 0.45813e-2)))));
   } else {
     y = 3.75 / ax;
-    ans = (hps::exp(ax) * (1.0 / ::sqrt(ax))) * (0.39894228 + y * (0.1328592e-1 +
+    ans = (hps::exp(ax) * (1.0 / psqrt(ax))) * (0.39894228 + y * (0.1328592e-1 +
           y * (0.225319e-2 + y * (-0.157565e-2 + y * (0.916281e-2 + y * (-0.2057706e-1 +
           y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))));
   }
@@ -224,7 +224,7 @@ __device__ __forceinline__ double bessi1c(double r_x)
     ans = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
     ans = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 + y * (0.163801e-2 + y *
           (-0.1031555e-1 + y * ans))));
-    ans = ans * (hps::exp(ax) * (1.0 / ::sqrt(ax)));
+    ans = ans * (hps::exp(ax) * (1.0 / psqrt(ax)));
   }
   return r_x < 0.0 ? -ans : ans;
 }
@@ -250,7 +250,7 @@ __device__ __forceinline__ void bessel_all(double r_x, double& b0, double& b1, d
          y * (0.301532e-2 + y * 0.32411e-3))))));
   } else {
     const double y = 3.75 / ax;
-    const double f = hps::exp(ax) * (1.0 / ::sqrt(ax));
+    const double f = hps::exp(ax) * (1.0 / psqrt(ax));
     b0 = f * (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 +
          y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1 +
          y * 0.392377e-2))))))));
@@ -294,7 +294,7 @@ __device__ __forceinline__ void drift(Particle& p, double h, double aper, double
     p.x = x;
     p.y = y;
     const double gm1 = p.w / mass;
-    const double bt = ::sqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
+    const double bt = psqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
     p.phi += TWOPI * h / (wavelen * bt);
   }
 }
@@ -315,9 +315,9 @@ __device__ __forceinline__ void half_quad(Particle& p, double length, double ape
     p.y = y + yp * length * 0.5;
   } else {
     const double gamma1 = w / mass;
-    const double bg = ::sqrt(gamma1 * (gamma1 + 2.0) * (1.0 + xp * xp + yp * yp));
+    const double bg = psqrt(gamma1 * (gamma1 + 2.0) * (1.0 + xp * xp + yp * yp));
     const double absqg = qg > 0.0 ? qg : -qg;
-    const double k = ::sqrt(absqg / (bg * mass) * CLIGHT);
+    const double k = psqrt(absqg / (bg * mass) * CLIGHT);
     const double inv_k = 1.0 / k;
     const double kds = length * 0.25 * k;
     double s, c;
@@ -342,7 +342,7 @@ __device__ __forceinline__ void half_quad(Particle& p, double length, double ape
     p.loss = elem;
   } else {
     const double gm1 = w / mass;
-    const double bt = ::sqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
+    const double bt = psqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
     p.phi += TWOPI * length * 0.5 / (wavelen * bt);
   }
 }
@@ -357,12 +357,12 @@ __device__ __forceinline__ void gap_first_half(Particle& p, const G& gap, const 
   p.y = y + u.dd1 * yp;
   if (gap.amplitude == 0.0) {
     const double gm1 = w / mass;
-    const double bt = ::sqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
+    const double bt = psqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
     p.phi += u.twopi_dd1 / (u.wave_len * bt);
   } else {
     const double beta_in = u.beta_in1;
     const double gam = w / mass;
-    const double beta = ::sqrt((gam + 2.0) * gam) / (gam + 1.0);
+    const double beta = psqrt((gam + 2.0) * gam) / (gam + 1.0);
     if (!ccl) {
       const double dphi_qlen1 = u.twopi_half_q1 / (beta * u.wave_len);
       double dphi_clen = 0.0;
@@ -386,7 +386,7 @@ __device__ __forceinline__ void gap_second_half(Particle& p, const G& gap, const
   double w = p.w;
   const double beta_in = u.beta_in2;
   double gam = w * inv_mass;
-  const double bg1 = ::sqrt(gam * (gam + 2.0));
+  const double bg1 = psqrt(gam * (gam + 2.0));
   double beta = bg1 / (gam + 1.0);
   double t_fac, tp_fac, sp_fac;
   ttf(beta, gap, t_fac, tp_fac, sp_fac);
@@ -394,7 +394,7 @@ __device__ __forceinline__ void gap_second_half(Particle& p, const G& gap, const
     p.x = x + u.dd2 * xp;
     p.y = y + u.dd2 * yp;
     const double gm1 = p.w / mass;
-    const double bt = ::sqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
+    const double bt = psqrt(gm1 * (gm1 + 2.0)) / (gm1 + 1.0);
     p.phi += u.twopi_dd2 / (wavelen * bt);
     return;
   }
@@ -423,7 +423,7 @@ __device__ __forceinline__ void gap_second_half(Particle& p, const G& gap, const
   const double rs_p = x * x + y * y;
   const double rrp = x * xp + y * yp;
   double rpris = 0.0, bes1covr;
-  const double ris = ::sqrt(rs_p);
+  const double ris = psqrt(rs_p);
   const double fkrris = fkr * ris;
   double b0c, b1c, b1p;
   bessel_all(fkrris, b0c, b1c, b1p);
@@ -433,7 +433,7 @@ __device__ __forceinline__ void gap_second_half(Particle& p, const G& gap, const
                        tw / gmg * fkr * b1p - tw * gmg * bes1covr) * rrp * sinp1);
   w += delte;
   gam = w * inv_mass;
-  const double bg = ::sqrt(gam * (gam + 2.0));
+  const double bg = psqrt(gam * (gam + 2.0));
   beta = bg / (gam + 1.0);
   const double bgr = bg1 / bg;
   const double onema = 1.0 - (con5nex * bes1covr + con5nex1 * b1p + con5nex2 * bes1covr) * cosp1;
@@ -481,7 +481,7 @@ __device__ __forceinline__ void steerer(Particle& p, double blh, double blv, dou
                                         double charge)
 {
   const double gm1 = p.w / mass;
-  const double btgm = ::sqrt(gm1 * (gm1 + 2.0));
+  const double btgm = psqrt(gm1 * (gm1 + 2.0));
   const double pm = btgm * mass / CLIGHT;
   if (blh > 1e-10) p.xp += blh * charge / pm;
   if (blv > 1e-10) p.yp += blv * charge / pm;
@@ -504,7 +504,7 @@ __device__ __forceinline__ void buncher(Particle& p, D* a, uint32_t elem, double
   const double h = freq * wavelen / CLIGHT;
   const double gm1 = w_ref / mass;
   const double gm = gm1 + 1.0;
-  const double btgm = ::sqrt(gm1 * (gm1 + 2.0));
+  const double btgm = psqrt(gm1 * (gm1 + 2.0));
   const double beta = btgm / gm;
   const double k = h * TWOPI / (btgm * wavelen);
   const double k2 = k * k;
@@ -519,11 +519,11 @@ __device__ __forceinline__ void buncher(Particle& p, D* a, uint32_t elem, double
   const double wf = w + dw;
   if (wb_par <= 0.0 || wf <= 0.0) { p.loss = elem; return; }
   double gmtmp = wb_par / mass;
-  const double bav = ::sqrt(gmtmp * (gmtmp + 2.0)) / (gmtmp + 1.0);
+  const double bav = psqrt(gmtmp * (gmtmp + 2.0)) / (gmtmp + 1.0);
   gmtmp = w / mass;
-  const double bg = ::sqrt(gmtmp * (gmtmp + 2.0));
+  const double bg = psqrt(gmtmp * (gmtmp + 2.0));
   gmtmp = wf / mass;
-  const double rbgf = 1.0 / ::sqrt(gmtmp * (gmtmp + 2.0));
+  const double rbgf = 1.0 / psqrt(gmtmp * (gmtmp + 2.0));
   const double del = -con1 * sph * (1.0 - bav * beta) * (0.5 + arg * (0.25 + arg / 24.0)) / bav;
   p.xp = (xp * bg + del * x) * rbgf;
   p.yp = (yp * bg + del * y) * rbgf;
@@ -545,12 +545,12 @@ __device__ __forceinline__ DipoleHalf dipole_half(D* a, double mass)
   const double h2 = h * h;
   const double dl = 0.5 * rho * angle;
   const double yk2 = findx * h2;
-  const double yk = ::sqrt(__builtin_fabs(yk2));
+  const double yk = psqrt(__builtin_fabs(yk2));
   const double inv_yk = 1.0 / yk;
   const double ykl = yk * dl;
   const double xk2 = h2 - yk2;
   const double inv_xk2 = 1.0 / xk2;
-  const double xk = ::sqrt(__builtin_fabs(xk2));
+  const double xk = psqrt(__builtin_fabs(xk2));
   const double inv_xk = 1.0 / xk;
   const double xkl = xk * dl;
   double cxkl = 1.0, sxokx = dl, s, c;
@@ -567,7 +567,7 @@ __device__ __forceinline__ DipoleHalf dipole_half(D* a, double mass)
   o.r52 = -o.r16;
   const double gamma = design_w / mass + 1.0;
   const double beta2 = 1.0 - 1.0 / (gamma * gamma);
-  o.beta = ::sqrt(beta2);
+  o.beta = psqrt(beta2);
   o.r56 = -h2 * (dl * beta2 - sxokx) * inv_xk2 + dl * (1.0 - h2 * inv_xk2) / (gamma * gamma);
   o.dl = dl;
   return o;

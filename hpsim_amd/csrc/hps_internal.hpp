@@ -172,6 +172,8 @@ hipError_t launch_segment_batch(const SegArgs* recs, const uint32_t* map, uint32
                                 int op_class, const uint64_t* roff, hipStream_t s);
 hipError_t launch_sc_build_batch(const BuildArgs* recs, const uint32_t* map, uint32_t nblocks,
                                  uint32_t nr, uint32_t nz, hipStream_t s);
+hipError_t launch_probe(int fn, const double* x, const double* y, double* out, uint32_t n,
+                        hipStream_t s);
 }  // namespace fast
 // Field sum of a batch in launch shape `shape` < field_batch_shapes() (any
 // larger value: the default for the batch size); every shape gives the same

@@ -278,13 +278,13 @@ __device__ __forceinline__ void kick(Particle& p, const KickConst& k, const doub
 {
   const double phi = relative_phi(p.phi, k.phi_ref);   // phi_r of UpdateRelativePhi(true)
   const double gm = p.w / r_mass + 1.0;
-  const double bgz = ::sqrt((gm - 1.0) * (gm + 1.0));
+  const double bgz = psqrt((gm - 1.0) * (gm + 1.0));
   double bgx = bgz * p.xp;
   double bgy = bgz * p.yp;
   double bgzstar = k.gmg * (bgz - k.r_beta * gm);
   const double xxbar = p.x - k.xbar;
   const double yybar = p.y - k.ybar;
-  double r = ::sqrt(xxbar * xxbar / k.epsq + yybar * yybar * k.epsq);
+  double r = psqrt(xxbar * xxbar / k.epsq + yybar * yybar * k.epsq);
   if (r == 0.0) r = 1.0e-8;
   const double xovr = xxbar * k.xfac / r;
   const double yovr = yybar * k.yfac / r;
@@ -297,7 +297,7 @@ __device__ __forceinline__ void kick(Particle& p, const KickConst& k, const doub
   double cbgr, cbgzs;
   if (r > k.nx * k.dr || __builtin_fabs(z) > k.hl) {
     z = -k.c2 * (phi - k.phimc);
-    const double d = ::sqrt(z * z + r * r);
+    const double d = psqrt(z * z + r * r);
     const double invd3 = 1.0 / (d * d * d);
     double rod3 = r * invd3;
     double zod3 = z * invd3;
@@ -306,7 +306,7 @@ __device__ __forceinline__ void kick(Particle& p, const KickConst& k, const doub
       int xi = (int)nb;
       for (int lr = 0; lr < 2; ++lr) {
         const double sz = z + xi * k.pl;
-        const double dd = ::sqrt(sz * sz + r * r);
+        const double dd = psqrt(sz * sz + r * r);
         const double iv = 1.0 / (dd * dd * dd);
         rod3 += r * iv;
         zod3 += sz * iv;
@@ -1275,7 +1275,7 @@ __device__ __forceinline__ bool deposit_one(double x, double y, double phi, cons
   const double dr = d.dr, dzg = d.dzg, drdr = d.drdr, hl = d.hl;
   double z = relative_phi(phi, d.ref);
   const double rsq_sh = (x - d.xbar) * (x - d.xbar) / d.epsq + (y - d.ybar) * (y - d.ybar) * d.epsq;
-  const int i = to_int(::sqrt(rsq_sh) / dr + 1);
+  const int i = to_int(psqrt(rsq_sh) / dr + 1);
   if (i < 1 || i > (int)nx) return false;
   double ddz = z - d.phimc;
   for (int guard = 0; (ddz > PI || ddz < -PI) && guard < 4096; ++guard) {
@@ -1298,7 +1298,7 @@ __device__ __forceinline__ bool deposit_one(double x, double y, double phi, cons
   if (i1 == i) {
     a = 1.0;
   } else {
-    const double sqrtmp = ::sqrt(4.0 * rsq_sh / drdr - 1.0);
+    const double sqrtmp = psqrt(4.0 * rsq_sh / drdr - 1.0);
     const double rminsq = 0.25 * drdr * (sqrtmp - 1.0) * (sqrtmp - 1.0);
     const double rmaxsq = rminsq + sqrtmp * drdr;
     if (i1 < i) a = (rmaxsq - (i - 1) * (i - 1) * drdr) / (rmaxsq - rminsq);
@@ -1746,6 +1746,7 @@ __global__ void k_probe(int fn, const double* x, const double* y, double* out, u
     case 9: r = bessi0c(v); break;
     case 10: r = bessi1c(v); break;
     case 11: r = bessi1p(v); break;
+    case 12: r = psqrt(v); break;
     default: break;
   }
   out[i] = r;

@@ -205,6 +205,39 @@ HPS_HD double tan(double x)
   return s / c;
 }
 
+// Square root from v_rsq_f64.  One Goldschmidt step refines g ~ sqrt(x)
+// and h ~ 1 / (2 sqrt(x)); one Newton correction of g follows.  For
+// 2^-767 <= x < inf the result is within about one unit in the last place.
+// The correctly rounded ::sqrt adds a second correction, scales x below
+// 2^-767, and costs 21 instructions against 12.  +-0 and +inf are returned
+// as they are; negative x and NaN give NaN.  Positive x below 2^-767 is
+// outside its range.  Device code only.
+__device__ __forceinline__ double sqrt_rsq(double x)
+{
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = ::fma(-h, g, 0.5);
+  g = ::fma(g, r, g);
+  h = ::fma(h, r, h);
+  const double d = ::fma(-g, g, x);
+  g = ::fma(d, h, g);
+  return (x == 0.0 || x == kInf) ? x : g;
+}
+
+// The square root of the per-particle maps, the kick and the deposit: the
+// correctly rounded ::sqrt, or sqrt_rsq in the build of hps_push_fast.hip,
+// which serves batches whose results are held to error bounds, not to the
+// oracle's bits.  Internal to each translation unit, so the two builds keep
+// their own.
+static __device__ __forceinline__ double psqrt(double x)
+{
+#ifdef HPS_FAST_PUSH
+  return sqrt_rsq(x);
+#else
+  return ::sqrt(x);
+#endif
+}
+
 // Remainder of x >= 0 by y > 0 (finite), exact.
 HPS_HD double fmod_pos(double x, double y)
 {

