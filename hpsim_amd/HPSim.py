@@ -626,10 +626,11 @@ class Batch:
 def probe_math(fn, x, y=None):
     """Evaluate one of the engine's device math routines over x (for tests)."""
     codes = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "sinh": 4, "cosh": 5, "sqrt": 6,
-             "fmod2pi": 7, "div": 8, "bessi0": 9, "bessi1": 10, "bessi1p": 11, "psqrt": 12}
+             "fmod2pi": 7, "div": 8, "bessi0": 9, "bessi1": 10, "bessi1p": 11, "psqrt": 12,
+             "psin": 13, "pcos": 14, "psinh": 15, "pcosh": 16}
     # the same routines as the reciprocal-division build of the grouped-solve
     # push computes them
-    codes.update({k + "_fast": v + 16 for k, v in list(codes.items())})
+    codes.update({k + "_fast": v + 32 for k, v in list(codes.items())})
     x = np.ascontiguousarray(x, np.float64)
     yy = None if y is None else np.ascontiguousarray(y, np.float64)
     out = np.empty_like(x)

@@ -322,8 +322,8 @@ __device__ __forceinline__ void half_quad(Particle& p, double length, double ape
     const double kds = length * 0.25 * k;
     double s, c;
     double sh, ch;
-    hps::sinhcosh(kds, &sh, &ch);
-    hps::sincos(kds, &s, &c);
+    psinhcosh(kds, &sh, &ch);
+    psincos(kds, &s, &c);
     const double sovk = s * inv_k, shovk = sh * inv_k, sk = s * k, shk = sh * k;
     double xx, yy;
 #pragma unroll
@@ -417,9 +417,9 @@ __device__ __forceinline__ void gap_second_half(Particle& p, const G& gap, const
     phi1 = phi + gap.phase_shift - u.twopi_dg / (beta * wave_len) + u.cell_pi;
   else
     phi1 = phi + gap.phase_shift;
-  hps::sincos(phi1, &sinp1, &cosp1);
+  psincos(phi1, &sinp1, &cosp1);
   const double prome1 = u.neg_con1 * (tp_fac * (sinp1 - sinps) + sp_fac * (cosp1 - cosps));
-  hps::sincos(phi1 + prome1, &sinp1, &cosp1);
+  psincos(phi1 + prome1, &sinp1, &cosp1);
   const double rs_p = x * x + y * y;
   const double rrp = x * xp + y * yp;
   double rpris = 0.0, bes1covr;
@@ -513,7 +513,7 @@ __device__ __forceinline__ void buncher(Particle& p, D* a, uint32_t elem, double
   const double arg = k2 * r2 * 0.25;
   const double ph = h * phi + phase;
   double sph, cph;
-  hps::sincos(ph, &sph, &cph);
+  psincos(ph, &sph, &cph);
   const double dw = q * voltage * cph * (1.0 + arg * (1.0 + arg * (0.25 + arg / 36.0)));
   const double wb_par = w + 0.5 * dw;
   const double wf = w + dw;

@@ -2515,8 +2515,8 @@ extern "C" int hps_probe_math(int fn, const double* x, const double* y, double* 
   HPS_CHECK(hipMemcpy(dx, x, nb, hipMemcpyHostToDevice));
   if (y) HPS_CHECK(hipMemcpy(dy, y, nb, hipMemcpyHostToDevice));
   else HPS_CHECK(hipMemset(dy, 0, nb));
-  if (fn >= 16)
-    HPS_CHECK(fast::launch_probe(fn - 16, dx, dy, dout, n, s));
+  if (fn >= 32)
+    HPS_CHECK(fast::launch_probe(fn - 32, dx, dy, dout, n, s));
   else
     HPS_CHECK(launch_probe(fn, dx, dy, dout, n, s));
   HPS_CHECK(hipStreamSynchronize(s));

@@ -1747,6 +1747,10 @@ __global__ void k_probe(int fn, const double* x, const double* y, double* out, u
     case 10: r = bessi1c(v); break;
     case 11: r = bessi1p(v); break;
     case 12: r = psqrt(v); break;
+    case 13: psincos(v, &s, &c); r = s; break;
+    case 14: psincos(v, &s, &c); r = c; break;
+    case 15: psinhcosh(v, &s, &c); r = s; break;
+    case 16: psinhcosh(v, &s, &c); r = c; break;
     default: break;
   }
   out[i] = r;

@@ -14,7 +14,11 @@
 //   fmod   exact remainder by subtracting power-of-two multiples of the
 //          divisor (each step exact by Sterbenz's lemma).
 // Only correctly rounded primitives are used (+ - * /, sqrt, fma, floor);
-// the library is compiled with -ffp-contract=off.
+// the library is compiled with -ffp-contract=off.  The exception is the
+// second build of the tracking kernels (hps_push_fast.hip).  It serves
+// batches held to error bounds, not to the oracle's bits.  Its maps, kick
+// and deposit take psqrt, psincos and psinhcosh as sqrt_rsq (v_rsq_f64 and
+// two refinement steps) and shorter Taylor polynomials.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -222,6 +226,91 @@ __device__ __forceinline__ double sqrt_rsq(double x)
   const double d = ::fma(-g, g, x);
   g = ::fma(d, h, g);
   return (x == 0.0 || x == kInf) ? x : g;
+}
+
+// sincos with the Taylor polynomials cut at r^13 (sin) and r^14 (cos) after
+// the same reduction: on |r| <= pi/4 the first omitted terms are below
+// 2.1e-14 and 1.1e-15.  Device code only.
+__device__ __forceinline__ void sincos_short(double x, double* sp, double* cp)
+{
+  if (!(__builtin_fabs(x) < kInf)) { *sp = x - x; *cp = x - x; return; }
+  const double kd = ::floor(x * kTwoOverPi + 0.5);
+  double r = ::fma(-kd, kPio2A, x);
+  r = ::fma(-kd, kPio2B, r);
+  r = ::fma(-kd, kPio2C, r);
+  const int q = (int)(((int64_t)kd) & 3);
+  const double z = r * r;
+  // sin: 1/13!, -1/11!, ..., -1/3!
+  double ps = 1.6059043836821613e-10;
+  ps = ::fma(ps, z, -2.505210838544172e-08);
+  ps = ::fma(ps, z, 2.7557319223985893e-06);
+  ps = ::fma(ps, z, -0.0001984126984126984);
+  ps = ::fma(ps, z, 0.008333333333333333);
+  ps = ::fma(ps, z, -0.16666666666666666);
+  const double s = ::fma(r * z, ps, r);
+  // cos: 1/14!, -1/12!, ..., -1/2!
+  double pc = -1.1470745597729725e-11;
+  pc = ::fma(pc, z, 2.08767569878681e-09);
+  pc = ::fma(pc, z, -2.755731922398589e-07);
+  pc = ::fma(pc, z, 2.48015873015873e-05);
+  pc = ::fma(pc, z, -0.001388888888888889);
+  pc = ::fma(pc, z, 0.041666666666666664);
+  pc = ::fma(pc, z, -0.5);
+  const double c = ::fma(z, pc, 1.0);
+  if (q == 0) { *sp = s; *cp = c; }
+  else if (q == 1) { *sp = c; *cp = -s; }
+  else if (q == 2) { *sp = -s; *cp = -c; }
+  else { *sp = -c; *cp = s; }
+}
+
+// sinhcosh with the Taylor series for |x| < 1 cut at x^15 (sinh) and x^16
+// (cosh): the first omitted terms at |x| = 1 are 2.8e-15 and 1.6e-16 of the
+// results.  |x| >= 1 as sinhcosh.  Device code only.
+__device__ __forceinline__ void sinhcosh_short(double x, double* shp, double* chp)
+{
+  const double ax = __builtin_fabs(x);
+  if (ax < 1.0) {
+    const double z = x * x;
+    double p = 7.647163731819816e-13;    // 1/15!
+    p = ::fma(p, z, 1.6059043836821613e-10);
+    p = ::fma(p, z, 2.505210838544172e-08);
+    p = ::fma(p, z, 2.7557319223985893e-06);
+    p = ::fma(p, z, 0.0001984126984126984);
+    p = ::fma(p, z, 0.008333333333333333);
+    p = ::fma(p, z, 0.16666666666666666);
+    *shp = ::fma(x * z, p, x);
+    double c = 4.779477332387385e-14;     // 1/16!
+    c = ::fma(c, z, 1.1470745597729725e-11);
+    c = ::fma(c, z, 2.08767569878681e-09);
+    c = ::fma(c, z, 2.755731922398589e-07);
+    c = ::fma(c, z, 2.48015873015873e-05);
+    c = ::fma(c, z, 0.001388888888888889);
+    c = ::fma(c, z, 0.041666666666666664);
+    c = ::fma(c, z, 0.5);
+    *chp = ::fma(z, c, 1.0);
+    return;
+  }
+  sinhcosh(x, shp, chp);
+}
+
+// sincos and sinhcosh of the per-particle maps: the full polynomials, or the
+// short ones in the build of hps_push_fast.hip (see psqrt below).
+static __device__ __forceinline__ void psincos(double x, double* sp, double* cp)
+{
+#ifdef HPS_FAST_PUSH
+  sincos_short(x, sp, cp);
+#else
+  sincos(x, sp, cp);
+#endif
+}
+
+static __device__ __forceinline__ void psinhcosh(double x, double* shp, double* chp)
+{
+#ifdef HPS_FAST_PUSH
+  sinhcosh_short(x, shp, chp);
+#else
+  sinhcosh(x, shp, chp);
+#endif
 }
 
 // The square root of the per-particle maps, the kick and the deposit: the

@@ -387,10 +387,11 @@ int hps_emittance_noalloc(const double* x, const double* xp, const double* y, co
 /* ---- elementary-function probes (host-callable, run on the device) ---- */
 /* Evaluates fn over n inputs on the GPU: 0 exp, 1 log, 2 sin, 3 cos, 4 sinh,
  * 5 cosh, 6 sqrt, 7 fmod(x, 2pi), 8 x/y, 9 Bessel I0, 10 Bessel I1,
- * 11 Bessel I1', 12 the square root of the per-particle maps.  16 + k runs
- * routine k as the reciprocal-division build of the grouped-solve push
- * computes it (its division and square root are within one unit in the last
- * place of the correctly rounded ones). */
+ * 11 Bessel I1', and the per-particle maps' 12 sqrt, 13 sin, 14 cos, 15 sinh,
+ * 16 cosh.  32 + k runs routine k as the reciprocal-division build of the
+ * grouped-solve push computes it (division and square root within one unit
+ * in the last place of the correctly rounded ones, the maps' sin and cos
+ * within 2.5e-14, sinh and cosh within 4e-15 relative). */
 int hps_probe_math(int fn, const double* x, const double* y, double* out, uint32_t n);
 
 #ifdef __cplusplus

@@ -9,9 +9,9 @@ of a kick point as one FP32 matrix-core product.
   along z), and the mesh is the reference's;
 * at 64 x 256 the frequency-domain product gives the direct product's field
   to FP32 rounding;
-* the push and deposit of these batches divide by refined reciprocals and
-  take square roots from v_rsq_f64: both within one unit in the last place
-  of the correctly rounded results;
+* the push and deposit of these batches divide by refined reciprocals, take
+  square roots from v_rsq_f64 (both within one unit in the last place of the
+  correctly rounded results) and shorter sin/cos/sinh/cosh polynomials;
 * short tracking follows the oracle closely (the implementation check);
 * at the batched-scan configuration's own sizes and lattice (40 periods,
   members up to 1,048,576 particles) the transverse rms emittances,
@@ -195,6 +195,32 @@ def test_fast_push_division_and_sqrt_within_one_ulp():
     assert np.isnan(g[3:]).all(), g
     assert np.array_equal(HPSim.probe_math("psqrt", v).view(np.uint64),
                           np.sqrt(v).view(np.uint64))
+
+
+def test_fast_push_trig_and_hyperbolic_within_bounds():
+    """The same build's sin and cos (Taylor polynomials cut at r^13 and r^14
+    after the pi/2 reduction: truncation below 2.1e-14, with rounding below
+    2.5e-14) against numpy's over |x| <= 1e4, and its sinh and cosh (cut at
+    x^15 and x^16 for |x| < 1) within 4e-15 relative over |x| <= 3; the
+    bit-exact build's maps use the full polynomials, the same bits as sin,
+    cos, sinh and cosh."""
+    from hpsim_amd import HPSim
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-1e4, 1e4, 200000)
+    for fn, ref in (("psin", np.sin), ("pcos", np.cos)):
+        e = np.abs(HPSim.probe_math(fn + "_fast", x) - ref(x)).max()
+        print(f"{fn}: max abs error {e:.2e}")
+        assert e < 2.5e-14, (fn, e)
+        assert np.array_equal(HPSim.probe_math(fn, x).view(np.uint64),
+                              HPSim.probe_math(fn[1:], x).view(np.uint64)), fn
+    h = rng.uniform(-3, 3, 200000)
+    for fn, ref in (("psinh", np.sinh), ("pcosh", np.cosh)):
+        r = ref(h)
+        e = (np.abs(HPSim.probe_math(fn + "_fast", h) - r) / np.abs(r)).max()
+        print(f"{fn}: max relative error {e:.2e}")
+        assert e < 4e-15, (fn, e)
+        assert np.array_equal(HPSim.probe_math(fn, h).view(np.uint64),
+                              HPSim.probe_math(fn[1:], h).view(np.uint64)), fn
 
 
 def test_gemm_tracking_follows_the_oracle(tmp_path):
