@@ -401,8 +401,10 @@ def main_batched(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "dtype_note": ("particles, maps, moments and kicks in FP64; the grouped field product "
-                       "in FP32 (FP32 operands and accumulation), its field tables stored FP64"
+        "dtype_note": ("particles, maps, moments and kicks in FP64 (divisions by refined "
+                       "reciprocals, square roots from the reciprocal square root, shorter "
+                       "sin/cos/sinh/cosh polynomials); the grouped field product in FP32 "
+                       "(FP32 operands and accumulation), its field tables stored FP64"
                        if mode in ("gemm", "surrogate") else "FP64 throughout"),
         "data": ("synthetic (Philox-seeded truncated Gaussian beams drawn on the device, "
                  "generated lattice" +
