@@ -359,6 +359,26 @@ def _timed_batch(w, mode, net, sel, steps, warmup, eager, profile):
         batch.set_profiling(False)
         r["gemm"] = (gi0, batch.gemm_info()) if mode != "fp64" else None
     r["transmitted"] = sum(m.beam.get_size() - m.beam.get_loss_num("t") for m in sims)
+    if profile:
+        # untimed census: particles not lost (loss == 0) at every slice boundary
+        # of one pass through the lattice from the initial beams; the timed
+        # steps ran slices (warmup + j) mod nslice
+        def alive():
+            return sum(m.beam.get_size() - m.beam.get_loss_num("t") for m in sims)
+        ends = []
+        for k in range(nslice):
+            if k == 0:
+                for m in sims:
+                    m.beam.restore_initial_beam()
+            ends.append(alive())
+            batch.simulate(k * sel, k * sel + sel - 1)
+        ends.append(alive())
+        mean_alive = sum(0.5 * (ends[q] + ends[q + 1])
+                         for q in ((warmup + j) % nslice for j in range(steps))) / steps
+        r["alive"] = {"value": mean_alive * sel * steps / dt,
+                      "mean_alive_fraction": mean_alive / total,
+                      "alive_at_slice_ends": ends}
+        _log(f"{mode}: alive census done")
     if mode == "surrogate":
         from hpsim_amd import surrogate as S
         r["surrogate_counters"] = S.info(batch)
@@ -414,6 +434,14 @@ def main_batched(args):
                    "particles_max": max(b.num for b in w.beams), "elements": nel,
                    "elements_per_step": sel, "mesh": f"{w.nr}x{w.nz}", "field": mode,
                    "transmitted_after_profiled_step": transmitted},
+        "alive_weighted": {
+            "value": run["alive"]["value"],
+            "unit": "particle-element pushes/s of particles not lost",
+            "mean_alive_fraction": run["alive"]["mean_alive_fraction"],
+            "definition": ("the timed steps' mean count of particles with loss == 0 (per "
+                           "step the mean of its slice's two ends, counted in an untimed "
+                           "pass through the lattice from the same initial beams) x "
+                           "elements per step / wall time")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(w.name, total),
