@@ -585,6 +585,151 @@ __device__ __forceinline__ void segment_body(const SegArgs& a, uint32_t c, RedSm
   }
 }
 
+#ifdef HPS_FAST_PUSH
+// The batched push of the grouped solve (this file as built by
+// hps_push_fast.hip): the same push, moments and chunk partials as
+// segment_body, but only over the chunk's live particles.  Particles lost
+// before the segment are never pushed.  Scattered at random among the live
+// ones, they would otherwise keep their lanes of every wavefront busy (on
+// the 64-beam stress batch about half the particles are lost, on average,
+// over the lattice).  The workgroup first reads the loss flags of its 1,024
+// particles.  A lost particle adds to the loss count (aperture statistics)
+// or, when not flagged longitudinally, offers its unchanged phase to the
+// largest-phase test.  The live ones are listed in particle order in LDS
+// (ballots per 64-particle group, an exclusive scan over the 16 groups),
+// and thread t pushes list entries t, t + 256, ...  Each sum then adds its
+// terms in another order than the canonical one; that is within this
+// build's error bounds, and the counts and maxima are the same.
+struct LiveSmem {
+  uint16_t list[kChunk];     // chunk offsets (j * 256 + lane) of the live particles
+  uint32_t gcnt[kPerLane * (kBlock / 64)];
+};
+
+template <int kClass>
+__device__ __forceinline__ void segment_body_live(const SegArgs& a, uint32_t c, RedSmem& sm,
+                                                  LiveSmem& ls)
+{
+  static_assert(kBlock == 256 && kPerLane == 4, "16 groups of 64 particles per chunk");
+  const int t = threadIdx.x, wave = t >> 6;
+  const BeamArrays& b = a.b;
+  const double xa = a.st->x_avg, ya = a.st->y_avg;
+  double* const acc = sm.red + t;
+#pragma unroll
+  for (int q = 0; q < kScQ + 1; ++q) acc[q * kBlock] = 0.0;
+  // flags of the lane's four particles; lost ones' contributions
+  uint32_t nlost = 0;
+  double mlost = -__builtin_huge_val();
+  bool live[kPerLane];
+#pragma unroll
+  for (int j = 0; j < (int)kPerLane; ++j) {
+    const uint32_t i = c * kChunk + j * kBlock + t;
+    live[j] = false;
+    if (i < b.n) {
+      if (gld(b.loss, i) == 0) {
+        live[j] = true;
+      } else {
+        ++nlost;
+        if (a.stats == STAT_SC && gld(b.lloss, i) == 0) mlost = ::fmax(mlost, gld(b.phi, i));
+      }
+    }
+  }
+  sm.ured[t] = a.stats == STAT_APERTURE ? nlost : 0u;
+  sm.mred[t] = mlost;
+  uint64_t bal[kPerLane];
+#pragma unroll
+  for (int j = 0; j < (int)kPerLane; ++j) {
+    bal[j] = __ballot(live[j]);
+    if ((t & 63) == 0) ls.gcnt[j * 4 + wave] = (uint32_t)__popcll(bal[j]);
+  }
+  __syncthreads();
+  uint32_t nlive = 0, base[kPerLane] = {0, 0, 0, 0};
+#pragma unroll
+  for (int g = 0; g < (int)(kPerLane * 4); ++g) {
+    const uint32_t n = ls.gcnt[g];
+#pragma unroll
+    for (int j = 0; j < (int)kPerLane; ++j)
+      if (g < j * 4 + wave) base[j] += n;
+    nlive += n;
+  }
+#pragma unroll
+  for (int j = 0; j < (int)kPerLane; ++j) {
+    if (live[j]) {
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(bal[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal[j], 0u));
+      ls.list[base[j] + below] = (uint16_t)(j * kBlock + t);
+    }
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (uint32_t sl = (uint32_t)t; sl < nlive; sl += kBlock) {
+    const uint32_t i = c * kChunk + ls.list[sl];
+    Particle p = load_particle(b, i);
+    if (a.kick) {
+      const ScDev* scp = a.sc;
+      const hps_beam_stats* stp = a.st;
+      asm volatile("" : "+s"(scp), "+s"(stp));
+      const KickConst k = kick_const(scp, stp, a.kick_len);
+      if (p.lloss == 0 && (i & a.kick_mask) != 0) kick(p, k, a.fld2, b.mass);
+    }
+    for (int o = a.op_begin; o < a.op_end; ++o) {
+      const OpConst& op = ((const OpConst*)a.ops)[o];
+      if (p.loss == 0) apply_op<kClass>(op, p, i, b.mass, b.charge, xa, ya);
+    }
+    store_particle(b, i, p);
+    if (a.stats == STAT_SC) {
+      if (p.lloss == 0) sm.mred[t] = ::fmax(sm.mred[t], p.phi);
+      const bool gd = is_good(p.loss, p.lloss);
+      const double vphi = gd ? p.phi : 0.0, vx = gd ? p.x : 0.0, vy = gd ? p.y : 0.0,
+                   vw = gd ? p.w : 0.0;
+      acc[0] = acc[0] + vphi;
+      acc[kBlock] = acc[kBlock] + vx;
+      acc[2 * kBlock] = acc[2 * kBlock] + vx * vx;
+      acc[3 * kBlock] = acc[3 * kBlock] + vy;
+      acc[4 * kBlock] = acc[4 * kBlock] + vy * vy;
+      acc[5 * kBlock] = acc[5 * kBlock] + vw;
+      sm.ured[t] += gd ? 1u : 0u;
+      if (a.rmax && p.loss == 0) {
+        const double tx = p.x - xa, ty = p.y - ya;
+        const double r2 = tx * tx + ty * ty;
+        const double r2m = acc[6 * kBlock];
+        acc[6 * kBlock] = r2m < r2 ? r2 : r2m;
+      }
+    } else if (a.stats == STAT_APERTURE) {
+      const bool ok = p.loss == 0;
+      acc[0] = acc[0] + (ok ? p.x : 0.0);
+      acc[kBlock] = acc[kBlock] + (ok ? p.y : 0.0);
+      sm.ured[t] += ok ? 0u : 1u;
+    }
+  }
+  const Partials& pr = a.pr;
+  if (a.stats == STAT_SC) {
+    double v[kScQ];
+#pragma unroll
+    for (int q = 0; q < kScQ; ++q) v[q] = acc[q * kBlock];
+    const double r2m = acc[6 * kBlock];
+    uint32_t cnt = sm.ured[t];
+    double mx = sm.mred[t];
+    block_reduce<kScQ>(v, cnt, mx, sm.red, sm.ured, sm.mred, t);
+    if (t == 0) {
+#pragma unroll
+      for (int q = 0; q < kScQ; ++q) pr.sums[q * pr.mpad + c] = v[q];
+      pr.cnt[c] = cnt;
+      pr.pmax[c] = mx;
+    }
+    if (a.rmax) {
+      const double m = block_max(r2m, sm.mred, t);
+      if (t == 0) pr.rmax2[c] = m;
+    }
+  } else if (a.stats == STAT_APERTURE) {
+    double ap[2] = {acc[0], acc[kBlock]};
+    uint32_t cnt = sm.ured[t];
+    double m0 = 0.0;
+    block_reduce<2>(ap, cnt, m0, sm.red, sm.ured, sm.mred, t);
+    if (t == 0) { pr.sums[c] = ap[0]; pr.sums[pr.mpad + c] = ap[1]; pr.cnt[c] = cnt; }
+  }
+}
+#endif
+
 template <int kClass, int kGroup>
 __global__ __launch_bounds__(kBlock) void k_segment(SegArgs a)
 {
@@ -610,7 +755,12 @@ __global__ __launch_bounds__(kBlock) void k_segment_batch(const SegArgs* __restr
   __shared__ RedSmem sm;
   const uint32_t e = map[blockIdx.x];
   const SegArgs& a = period_recs(recs, roff)[e >> kBatchLocalBits];
+#ifdef HPS_FAST_PUSH
+  __shared__ LiveSmem ls;
+  segment_body_live<kClass>(a, e & kBatchLocalMask, sm, ls);
+#else
   segment_body<kClass, kGroup>(a, e & kBatchLocalMask, sm);
+#endif
 }
 
 // The same push for beams too small to fill the GPU with k_segment's 256-thread

@@ -22,6 +22,10 @@
 // the batched push drop by 12 % (quad class) and 15 % (RF-gap class), and
 // by 10 % in the batched deposit.
 //
+// The batched push of this build visits only the live particles of each
+// chunk (segment_body_live in hps_kernels.hip).  It adds each chunk's sums
+// in another order than the canonical one.
+//
 // These modes already differ from the FP64 solve: each member's field comes
 // from an FP32 product, within about 1e-6 of the FP64 field.  A last-bit
 // change in a division is ten orders of magnitude below that.  Their results
