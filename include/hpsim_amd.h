@@ -284,7 +284,8 @@ int hps_batch_graph_info(hps_batch* bt, uint64_t* replays, uint32_t* period_laun
  * rounding.  The push and rebuild of such a batch divide by refined
  * reciprocals and take square roots from the reciprocal square root
  * (within one unit in the last place) and shorter sin/cos/sinh/cosh
- * polynomials (within 2.5e-14).  No reference counterpart. */
+ * polynomials (within 2.5e-14); the push visits only live particles and
+ * adds each chunk's moments in its own order.  No reference counterpart. */
 int hps_batch_set_field_mode(hps_batch* bt, int mode, double aspect_step);
 /* Launch shape of the grouped product (index < 9, or UINT32_MAX: autotune
  * at the first rebuild, the default).  Shapes 0-7 run the direct product
